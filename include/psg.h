@@ -1,0 +1,111 @@
+/*
+ * psgpu-mi355x C-ABI: persistent-context pseudospectral steppers on gfx950.
+ *
+ * Every entry point takes plain pointers and sizes; no torch or HIP types
+ * appear in the signatures.  State arrays use the reference storage order:
+ * per field NKT complex doubles (re,im interleaved), index
+ *   ik = (ikz*IKTY + iky)*IKTX + ikx      (3D; 2D drops ikz)
+ * with kx = ikx*alpha, ky = iky-KTY, kz = ikz-KTZ
+ * (3D_NavierStokes/src/init.F90:25-36, 2D_NavierStokes/src_RK4/main.F90:146-166).
+ * Fields are consecutive: 3D (xii, eta, zet), Boussinesq (xii, eta, zet, rho),
+ * 2D (z).  Batched contexts hold `batch` independent members.
+ *
+ * The reference has no persistent context: each timestep_cuda_ call
+ * allocates, plans, copies, steps and frees (3D_NavierStokes/src/cuda.cu:18-151,
+ * init.h:38-101).  psg_create/psg_run replace that call sequence; the legacy
+ * symbols in psg_legacy.h map the reference argument lists onto them.
+ *
+ * Errors: every function returns 0 on success or a negative PSG_E* code;
+ * psg_last_error() returns a message for the calling thread's last failure.
+ */
+#ifndef PSG_H
+#define PSG_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSG_OK 0
+#define PSG_EINVAL -1   /* bad argument / unsupported size */
+#define PSG_EHIP -2     /* HIP runtime failure */
+#define PSG_ENOMEM -3
+#define PSG_ENODEV -4   /* no GPU */
+
+enum psg_flavour {
+  PSG_NS2D = 1,     /* 2D vorticity, RK4 + end-of-step CN  (2D_NavierStokes/src_RK4) */
+  PSG_NS3D = 2,     /* 3D vorticity, Heun + CN             (3D_NavierStokes/src, src_gmres) */
+  PSG_EULER3D = 3,  /* 3D inviscid, Heun                   (3D_Euler/src) */
+  PSG_BOUS3D = 4    /* 3D Boussinesq, RK4 per-stage CN     (3D_Boussinesq/src) */
+};
+enum psg_precision {
+  PSG_FP64 = 0,     /* plain form: FFTs and products in FP64 */
+  PSG_FP32BULK = 1  /* FFTs and products in FP32, state/update/diagnostics FP64 */
+};
+enum psg_forcing {
+  PSG_FORCE_NONE = 0,
+  PSG_FORCE_DNS = 1,    /* xi.re +-A at (0,KF,+-KF)  3D_NavierStokes/src/kernels.h:342-343 */
+  PSG_FORCE_GMRES = 2,  /* eta.im, zet.im at (0,KF,+-KF)  src_gmres/kernels.h:234-238 */
+  PSG_FORCE_KOLMO = 3   /* f=(sin KF y,0,0): zet.re(0,KF,0) += A  3D_Boussinesq/src/kernels.h:382-384;
+                           2D: NZKX -= A at (0,+-KF)  2D_NavierStokes/src_RK4/kernels.h:264-266 */
+};
+
+typedef struct {
+  int flavour;        /* psg_flavour */
+  int precision;      /* psg_precision */
+  int batch;          /* members stepped together (>=1) */
+  int nx, ny, nz;     /* physical grid (nz ignored in 2D); powers of two 16..1024 */
+  double alpha;       /* kx = ikx*alpha */
+  double v2;          /* viscosity 1/Re (ignored for Euler) */
+  double dt;          /* time step */
+  int force;          /* psg_forcing */
+  double kf;          /* forcing wavenumber */
+  double ampfor;      /* forcing amplitude A */
+  double ri, sc, theta; /* Boussinesq: Richardson number, Schmidt number, gravity tilt (deg) */
+} psg_config;
+
+typedef struct psg_ctx psg_ctx;
+
+/* Sizes implied by a config (IKTX, IKTY, IKTZ, NKT, active-mode count). */
+int psg_sizes(const psg_config *cfg, long *iktx, long *ikty, long *iktz, long *nkt, long *nactive);
+
+int psg_create(const psg_config *cfg, psg_ctx **out);
+int psg_destroy(psg_ctx *ctx);
+/* copy member state host<->device: nf*nkt complex (re,im) doubles */
+int psg_set_state(psg_ctx *ctx, int member, const double *host);
+int psg_get_state(psg_ctx *ctx, int member, double *host);
+/* device pointer to a member's state (nf*nkt complex doubles), for zero-copy callers */
+int psg_state_ptr(psg_ctx *ctx, int member, void **dev);
+/* device-to-device state copies (ptr may be any device allocation) */
+int psg_set_state_device(psg_ctx *ctx, int member, const void *dev);
+int psg_get_state_device(psg_ctx *ctx, int member, void *dev);
+/* advance every member by nsteps steps of dt (asynchronous on the context stream) */
+int psg_run(psg_ctx *ctx, int nsteps);
+/* change dt (rebuilds the CN tables, as src_gmres/main.F90:759-768 does per period) */
+int psg_set_dt(psg_ctx *ctx, double dt);
+int psg_sync(psg_ctx *ctx);
+/* FP64 diagnostics of one member: energy = sum_L |w|^2/|k|^2, enstrophy = sum_L |w|^2
+ * (2D: 2*sum_L|z|^2 as out.F90), spectrum[j] = sum over shell j=INT(|k|+0.5) of |w|^2/|k|^2
+ * (spec may be NULL, else length KTY+2).  3D_NavierStokes/src/functions.h:128-161 */
+int psg_diag(psg_ctx *ctx, int member, double *energy, double *enstrophy, double *spec);
+/* spectral nonlinear term dz/dt = N(z)+F (no viscous term), masked, for one member,
+ * written into out (nf*nkt complex); cf. TIMEDERIV src_gmres/timederiv.F90:2-44 */
+int psg_nonlin(psg_ctx *ctx, int member, const double *host_in, double *host_out);
+/* physical-space fields of a member: 6 (3D: w then u) or 3 (2D: w,u,v) arrays of NR doubles,
+ * x fastest, as KR_FFT_ALL produces (3D_NavierStokes/src/functions.h:26-75) */
+int psg_physical(psg_ctx *ctx, int member, double *host_out);
+/* stream used by the context (hipStream_t as void*) */
+void *psg_stream(psg_ctx *ctx);
+const char *psg_last_error(void);
+/* number of visible devices (0 when none) */
+int psg_device_count(void);
+
+/* ---------- timing hooks for measurement ----------
+ * Kernel names of one step in launch order and their average duration (ms)
+ * measured with HIP events on the context stream over the last psg_run call
+ * when profiling is enabled. */
+int psg_profile_enable(psg_ctx *ctx, int on);
+int psg_profile_get(psg_ctx *ctx, int idx, const char **name, double *avg_ms, long *bytes, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,267 @@
+"""psgpu-mi355x: MI355X-native pseudospectral steppers (Python host mirror).
+
+The compute path is the HIP library psgpu_amd/_lib/libpsg.so (hand-written
+gfx950 kernels behind the C-ABI in include/psg.h).  This module is a thin
+ctypes mirror of that C-ABI for tests, benchmarks and Python drivers: there
+is no Python or CPU fallback -- if the library is missing, or no GPU is
+visible, construction raises.
+
+Reference mapping: a ``Stepper`` replaces one reference ``timestep_cuda_``
+call sequence (3D_NavierStokes/src/cuda.cu:18-151 and siblings) with a
+persistent context; ``run(n)`` advances every member by n steps.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(_HERE, "_lib")
+LIBPATH = os.path.join(LIBDIR, "libpsg.so")
+
+NS2D, NS3D, EULER3D, BOUS3D = 1, 2, 3, 4
+FP64, FP32BULK = 0, 1
+FORCE_NONE, FORCE_DNS, FORCE_GMRES, FORCE_KOLMO = 0, 1, 2, 3
+_FLAV = {"ns2d": NS2D, "ns3d": NS3D, "euler": EULER3D, "euler3d": EULER3D, "bous": BOUS3D, "bous3d": BOUS3D}
+_PREC = {"fp64": FP64, "fp32": FP32BULK, "fp32bulk": FP32BULK}
+_FORCE = {"none": FORCE_NONE, "dns": FORCE_DNS, "gmres": FORCE_GMRES, "kolmogorov": FORCE_KOLMO,
+          "kolmo": FORCE_KOLMO}
+
+
+class PsgError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [("flavour", C.c_int), ("precision", C.c_int), ("batch", C.c_int),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("alpha", C.c_double),
+                ("v2", C.c_double), ("dt", C.c_double), ("force", C.c_int), ("kf", C.c_double),
+                ("ampfor", C.c_double), ("ri", C.c_double), ("sc", C.c_double), ("theta", C.c_double)]
+
+
+class HGrid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("alpha", C.c_double), ("ktx", C.c_int), ("kty", C.c_int), ("ktz", C.c_int),
+                ("iktx", C.c_int), ("ikty", C.c_int), ("iktz", C.c_int), ("nx2", C.c_int),
+                ("nkt", C.c_long), ("nr", C.c_long), ("n2", C.c_long)]
+
+
+_lib = None
+_P, _D, _I, _L = C.c_void_p, C.c_double, C.c_int, C.c_long
+
+
+def lib():
+    """Load libpsg.so (raises PsgError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise PsgError("psgpu_amd: %s is not built; run `python -m psgpu_amd.build`" % LIBPATH)
+    L = C.CDLL(LIBPATH)
+    cfgp = C.POINTER(Config)
+    L.psg_last_error.restype = C.c_char_p
+    L.psg_sizes.argtypes = [cfgp] + [C.POINTER(C.c_long)] * 5
+    L.psg_create.argtypes = [cfgp, C.POINTER(_P)]
+    L.psg_destroy.argtypes = [_P]
+    for f in ("psg_set_state", "psg_get_state", "psg_set_state_device", "psg_get_state_device"):
+        getattr(L, f).argtypes = [_P, _I, _P]
+    L.psg_state_ptr.argtypes = [_P, _I, C.POINTER(_P)]
+    L.psg_run.argtypes = [_P, _I]
+    L.psg_set_dt.argtypes = [_P, _D]
+    L.psg_sync.argtypes = [_P]
+    L.psg_diag.argtypes = [_P, _I, C.POINTER(_D), C.POINTER(_D), _P]
+    L.psg_nonlin.argtypes = [_P, _I, _P, _P]
+    L.psg_physical.argtypes = [_P, _I, _P]
+    L.psg_stream.argtypes = [_P]
+    L.psg_stream.restype = _P
+    L.psg_profile_enable.argtypes = [_P, _I]
+    L.psg_profile_get.argtypes = [_P, _I, C.POINTER(C.c_char_p), C.POINTER(_D), C.POINTER(_L), C.POINTER(_I)]
+    hg = C.POINTER(HGrid)
+    L.psgh_grid3.argtypes = [hg, _I, _I, _I, _D]
+    L.psgh_grid2.argtypes = [hg, _I, _I, _D]
+    L.psgh_tables.argtypes = [hg, _P, _P, _P, _P]
+    L.psgh_count_active.argtypes = [hg]
+    L.psgh_count_active.restype = _L
+    L.psgh_nutab.argtypes = [hg, _D, _D, _P, _P, _P]
+    L.psgh_ikf_ikn.argtypes = [hg, _P, _P]
+    L.psgh_twiddles.argtypes = [_I, _P]
+    L.psgh_rng_seed.argtypes = [_I]
+    L.psgh_ranno0.restype = _D
+    L.psgh_ic_ns3d_ref.argtypes = [hg, _I, _D, _P, _P, _P]
+    L.psgh_ic_ns2d_ref.argtypes = [hg, _I, _D, _P]
+    L.psgh_ic_solenoidal.argtypes = [hg, C.c_uint64, _D, _D, _P, _P, _P, _P]
+    L.psgh_ic_ns2d_cfg.argtypes = [hg, C.c_uint64, _D, _P]
+    L.psgh_fortran_f.argtypes = [C.c_char_p, _I, _I]
+    L.psgh_fortran_f.restype = _D
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().psg_last_error().decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise PsgError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def _ptr(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count():
+    return lib().psg_device_count()
+
+
+# ------------------------------------------------------------------ host layer
+def hgrid(flavour, nx, ny, nz=1, alpha=1.0):
+    g = HGrid()
+    if _FLAV.get(flavour, flavour) == NS2D:
+        lib().psgh_grid2(C.byref(g), nx, ny, alpha)
+    else:
+        lib().psgh_grid3(C.byref(g), nx, ny, nz, alpha)
+    return g
+
+
+def tables(g):
+    n = g.nkt
+    kx, ky, kz = np.zeros(n), np.zeros(n), np.zeros(n)
+    L = np.zeros(n, np.int32)
+    lib().psgh_tables(C.byref(g), _ptr(kx), _ptr(ky), _ptr(kz), _ptr(L))
+    return kx, ky, kz, L
+
+
+def nutab(g, v2, dt):
+    n = g.nkt
+    nu, nuzn, nu1 = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().psgh_nutab(C.byref(g), v2, dt, _ptr(nu), _ptr(nuzn), _ptr(nu1))
+    return nu, nuzn, nu1
+
+
+def ic_ns3d_ref(g, iseed, ampv=1.0):
+    s = np.zeros((3, g.nkt), np.complex128)
+    lib().psgh_ic_ns3d_ref(C.byref(g), iseed, ampv, _ptr(s[0]), _ptr(s[1]), _ptr(s[2]))
+    return s
+
+
+def ic_ns2d_ref(g, iseed, ampv=1.0):
+    z = np.zeros((1, g.nkt), np.complex128)
+    lib().psgh_ic_ns2d_ref(C.byref(g), iseed, ampv, _ptr(z[0]))
+    return z
+
+
+def ic_solenoidal(g, seed, energy=0.5, rho_scale=None):
+    nf = 4 if rho_scale is not None else 3
+    s = np.zeros((nf, g.nkt), np.complex128)
+    rho = _ptr(s[3]) if rho_scale is not None else None
+    lib().psgh_ic_solenoidal(C.byref(g), seed, energy, rho_scale or 0.0, _ptr(s[0]), _ptr(s[1]), _ptr(s[2]), rho)
+    return s
+
+
+def ic_ns2d_cfg(g, seed, ampv=1.0):
+    z = np.zeros((1, g.nkt), np.complex128)
+    lib().psgh_ic_ns2d_cfg(C.byref(g), seed, ampv, _ptr(z[0]))
+    return z
+
+
+# ------------------------------------------------------------------ stepper
+def make_config(flavour="ns3d", n=64, nx=None, ny=None, nz=None, alpha=1.0, precision="fp64", batch=1,
+                v2=0.01, dt=0.002, force="kolmogorov", kf=4.0, ampfor=None, ri=0.0, sc=1.0, theta=90.0):
+    fl = _FLAV.get(flavour, flavour)
+    fo = _FORCE.get(force, force)
+    ny = ny or n
+    nz = nz or (1 if fl == NS2D else n)
+    nx = nx or int(round(ny / alpha))
+    if ampfor is None:
+        ampfor = {FORCE_NONE: 0.0, FORCE_DNS: -0.25 * kf * kf, FORCE_GMRES: -0.25 * kf,
+                  FORCE_KOLMO: -0.5 * kf}[fo]
+    return Config(fl, _PREC.get(precision, precision), batch, nx, ny, nz, alpha, v2, dt, fo, kf, ampfor,
+                  ri, sc, theta)
+
+
+class Stepper:
+    """Persistent GPU context for ``batch`` members of one flavour/grid."""
+
+    def __init__(self, cfg=None, **kw):
+        self.cfg = cfg if cfg is not None else make_config(**kw)
+        L = lib()
+        sizes = [C.c_long() for _ in range(5)]
+        _check(L.psg_sizes(C.byref(self.cfg), *[C.byref(s) for s in sizes]), "psg_sizes")
+        self.iktx, self.ikty, self.iktz, self.nkt, self.nactive = (s.value for s in sizes)
+        self.nf = {NS2D: 1, NS3D: 3, EULER3D: 3, BOUS3D: 4}[self.cfg.flavour]
+        h = _P()
+        _check(L.psg_create(C.byref(self.cfg), C.byref(h)), "psg_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psg_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, state, member=0):
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt)
+        _check(lib().psg_set_state(self.h, member, _ptr(s)), "psg_set_state")
+
+    def get_state(self, member=0):
+        s = np.zeros((self.nf, self.nkt), np.complex128)
+        _check(lib().psg_get_state(self.h, member, _ptr(s)), "psg_get_state")
+        return s
+
+    def state_ptr(self, member=0):
+        p = _P()
+        _check(lib().psg_state_ptr(self.h, member, C.byref(p)), "psg_state_ptr")
+        return p.value
+
+    def run(self, nsteps):
+        _check(lib().psg_run(self.h, int(nsteps)), "psg_run")
+
+    def sync(self):
+        _check(lib().psg_sync(self.h), "psg_sync")
+
+    def set_dt(self, dt):
+        _check(lib().psg_set_dt(self.h, dt), "psg_set_dt")
+        self.cfg.dt = dt
+
+    def diag(self, member=0):
+        e, z = C.c_double(), C.c_double()
+        kty = (self.ikty - 1) // 2
+        spec = np.zeros(kty + 2)
+        _check(lib().psg_diag(self.h, member, C.byref(e), C.byref(z), _ptr(spec)), "psg_diag")
+        return e.value, z.value, spec
+
+    def nonlin(self, state):
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt)
+        out = np.zeros_like(s)
+        _check(lib().psg_nonlin(self.h, 0, _ptr(s), _ptr(out)), "psg_nonlin")
+        return out
+
+    def physical(self, member=0):
+        c = self.cfg
+        nfo = 3 if c.flavour == NS2D else 6
+        out = np.zeros((nfo, c.nz if c.flavour != NS2D else 1, c.ny, c.nx))
+        _check(lib().psg_physical(self.h, member, _ptr(out)), "psg_physical")
+        return out
+
+    def profile(self, on=True):
+        _check(lib().psg_profile_enable(self.h, 1 if on else 0), "psg_profile_enable")
+
+    def profile_report(self):
+        rows = []
+        i = 0
+        while True:
+            nm, ms, b, cnt = C.c_char_p(), C.c_double(), C.c_long(), C.c_int()
+            if lib().psg_profile_get(self.h, i, C.byref(nm), C.byref(ms), C.byref(b), C.byref(cnt)) != 0:
+                break
+            rows.append({"kernel": nm.value.decode(), "avg_ms": ms.value, "bytes": b.value, "count": cnt.value})
+            i += 1
+        return rows

@@ -1,0 +1,439 @@
+// Context management, state I/O, diagnostics and the namespaced C-ABI.
+//
+// A context owns every device allocation for the life of a run (state, CN
+// tables, mask, twiddles, pass buffers), so repeated calls pay no allocation
+// or planning cost; the reference allocates, plans and frees on every
+// timestep_cuda_ call and inside every transform wrapper
+// (3D_NavierStokes/src/init.h:38-101, functions.h:46-47,73-74,93,107).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include "psg_launch.hpp"
+#include "../../include/psg_host.h"
+
+namespace psg {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+struct PendProf {
+  int name;
+  long bytes;
+  int e0, e1;
+};
+
+static hipEvent_t get_event(Ctx *c) {
+  if (c->ev_used >= (int)c->evs.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->evs.push_back(e);
+  }
+  return c->evs[c->ev_used++];
+}
+
+struct ProfState {
+  std::vector<PendProf> pend;
+  int cur_name = -1;
+  long cur_bytes = 0;
+  int cur_e0 = -1;
+};
+static ProfState *pstate(Ctx *c) {
+  static thread_local std::vector<std::pair<Ctx *, ProfState *>> tab;
+  for (auto &p : tab)
+    if (p.first == c) return p.second;
+  tab.push_back({c, new ProfState()});
+  return tab.back().second;
+}
+
+void prof_begin(Ctx *c, const char *name, long bytes) {
+  if (!c->prof) return;
+  ProfState *s = pstate(c);
+  int idx = -1;
+  for (size_t i = 0; i < c->profs.size(); i++)
+    if (c->profs[i].name == name) idx = (int)i;
+  if (idx < 0) {
+    ProfEntry e;
+    e.name = name;
+    c->profs.push_back(e);
+    idx = (int)c->profs.size() - 1;
+  }
+  hipEvent_t e0 = get_event(c);
+  if (!e0) return;
+  hipEventRecord(e0, c->stream);
+  s->cur_name = idx;
+  s->cur_bytes = bytes;
+  s->cur_e0 = c->ev_used - 1;
+}
+
+void prof_end(Ctx *c) {
+  if (!c->prof) return;
+  ProfState *s = pstate(c);
+  if (s->cur_name < 0) return;
+  hipEvent_t e1 = get_event(c);
+  if (!e1) return;
+  hipEventRecord(e1, c->stream);
+  s->pend.push_back(PendProf{s->cur_name, s->cur_bytes, s->cur_e0, c->ev_used - 1});
+  s->cur_name = -1;
+}
+
+static void prof_collect(Ctx *c) {
+  ProfState *s = pstate(c);
+  if (s->pend.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto &p : s->pend) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->evs[p.e0], c->evs[p.e1]) == hipSuccess) {
+      ProfEntry &e = c->profs[p.name];
+      e.ms += ms;
+      e.bytes += p.bytes;
+      e.count += 1;
+    }
+  }
+  s->pend.clear();
+  c->ev_used = 0;
+}
+
+// ------------------------------------------------------------------ diag
+__global__ void k_diag(const cpx<double> *__restrict__ st, long fs, int nf, const int8_t *__restrict__ L, Grid g,
+                       int nbins, double *__restrict__ part) {
+  extern __shared__ double sh[];  // [2*blockDim] + nbins
+  double *bins = sh + 2 * blockDim.x;
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) bins[i] = 0.0;
+  __syncthreads();
+  double e = 0.0, z = 0.0;
+  for (long ik = blockIdx.x * (long)blockDim.x + threadIdx.x; ik < g.nkt; ik += (long)gridDim.x * blockDim.x) {
+    if (L[ik] != 1) continue;
+    const int ikx = (int)(ik % g.iktx);
+    const long t = ik / g.iktx;
+    const int iky = (int)(t % g.ikty);
+    const int ikz = (int)(t / g.ikty);
+    const double kx = ikx * g.alpha, ky = iky - g.kty, kz = g.dim == 3 ? (double)(ikz - g.ktz) : 0.0;
+    const double kk = kx * kx + ky * ky + kz * kz;
+    const double wk = kk > 0.001 ? kk : 0.001;
+    double vz = 0.0;
+    for (int f = 0; f < nf; f++) {
+      cpx<double> a = st[f * fs + ik];
+      vz += a.x * a.x + a.y * a.y;
+    }
+    z += vz;
+    e += vz / wk;
+    int j = (int)(sqrt(kk) + 0.5);
+    if (j < nbins) atomicAdd(&bins[j], vz / wk);
+  }
+  sh[threadIdx.x] = e;
+  sh[blockDim.x + threadIdx.x] = z;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[threadIdx.x] += sh[threadIdx.x + s];
+      sh[blockDim.x + threadIdx.x] += sh[blockDim.x + threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  double *o = part + (long)blockIdx.x * (2 + nbins);
+  if (threadIdx.x == 0) { o[0] = sh[0]; o[1] = sh[blockDim.x]; }
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) o[2 + i] = bins[i];
+}
+
+int diag_member(Ctx *c, int member, double *e, double *z, double *spec) {
+  const Grid &g = c->g;
+  const int nbins = g.kty + 2;
+  const int blocks = (int)std::min<long>(256, (g.nkt + 255) / 256);
+  const size_t need = (size_t)blocks * (2 + nbins);
+  if (c->dred_n < need) {
+    if (c->dred) hipFree(c->dred);
+    PSG_HIP(hipMalloc(&c->dred, need * sizeof(double)));
+    c->dred_n = need;
+  }
+  const int nfv = c->cfg.flavour == PSG_BOUS3D ? 3 : (c->g.dim == 3 ? 3 : 1);
+  size_t sh = (2 * 256 + nbins) * sizeof(double);
+  hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), sh, c->stream, c->state + (long)member * c->nf * g.nkt,
+                     g.nkt, nfv, c->mask, g, nbins, c->dred);
+  PSG_HIP(hipGetLastError());
+  std::vector<double> h(need);
+  PSG_HIP(hipMemcpyAsync(h.data(), c->dred, need * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  double E = 0, Z = 0;
+  std::vector<double> sp(nbins, 0.0);
+  for (int b = 0; b < blocks; b++) {
+    E += h[(size_t)b * (2 + nbins)];
+    Z += h[(size_t)b * (2 + nbins) + 1];
+    for (int i = 0; i < nbins; i++) sp[i] += h[(size_t)b * (2 + nbins) + 2 + i];
+  }
+  if (g.dim == 2) Z *= 2.0;
+  if (e) *e = E;
+  if (z) *z = Z;
+  if (spec) memcpy(spec, sp.data(), sizeof(double) * nbins);
+  return 0;
+}
+
+static int upload_tables(Ctx *c) {
+  psgh_grid hg;
+  const Grid &g = c->g;
+  if (g.dim == 3) psgh_grid3(&hg, g.nx, g.ny, g.nz, g.alpha);
+  else psgh_grid2(&hg, g.nx, g.ny, g.alpha);
+  std::vector<int> L(g.nkt);
+  psgh_tables(&hg, nullptr, nullptr, nullptr, L.data());
+  std::vector<int8_t> L8(g.nkt);
+  for (long i = 0; i < g.nkt; i++) L8[i] = (int8_t)L[i];
+  PSG_HIP(hipMemcpy(c->mask, L8.data(), g.nkt, hipMemcpyHostToDevice));
+  if (g.dim == 3) {
+    std::vector<double> nuzn(g.nkt), nu1(g.nkt);
+    psgh_nutab(&hg, c->cfg.v2, c->cfg.dt, nullptr, nuzn.data(), nu1.data());
+    PSG_HIP(hipMemcpy(c->nuzn, nuzn.data(), sizeof(double) * g.nkt, hipMemcpyHostToDevice));
+    PSG_HIP(hipMemcpy(c->nu1, nu1.data(), sizeof(double) * g.nkt, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+static int upload_twiddles(Ctx *c) {
+  const int lens[3] = {c->g.nx, c->g.ny, c->g.dim == 3 ? c->g.nz : 16};
+  for (int a = 0; a < 3; a++) {
+    const int n = lens[a];
+    std::vector<double> w(2 * n);
+    psgh_twiddles(n, w.data());
+    std::vector<float> wf(2 * n);
+    for (int i = 0; i < 2 * n; i++) wf[i] = (float)w[i];
+    PSG_HIP(hipMalloc(&c->tw64[a], sizeof(cpx<double>) * n));
+    PSG_HIP(hipMalloc(&c->tw32[a], sizeof(cpx<float>) * n));
+    PSG_HIP(hipMemcpy(c->tw64[a], w.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    PSG_HIP(hipMemcpy(c->tw32[a], wf.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+static void free_ctx(Ctx *c) {
+  if (!c) return;
+  void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->nuzn, c->nu1, c->mask, c->bufA, c->bufB, c->bufC,
+                  c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  for (auto e : c->evs) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+}
+
+}  // namespace psg
+
+using namespace psg;
+
+extern "C" {
+
+const char *psg_last_error(void) { return psg::g_err; }
+
+int psg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int psg_sizes(const psg_config *cfg, long *iktx, long *ikty, long *iktz, long *nkt, long *nactive) {
+  if (!cfg) return PSG_EINVAL;
+  psgh_grid hg;
+  int r = cfg->flavour == PSG_NS2D ? psgh_grid2(&hg, cfg->nx, cfg->ny, cfg->alpha)
+                                   : psgh_grid3(&hg, cfg->nx, cfg->ny, cfg->nz, cfg->alpha);
+  if (r) { set_error("grid too small"); return PSG_EINVAL; }
+  if (iktx) *iktx = hg.iktx;
+  if (ikty) *ikty = hg.ikty;
+  if (iktz) *iktz = hg.iktz;
+  if (nkt) *nkt = hg.nkt;
+  if (nactive) *nactive = psgh_count_active(&hg);
+  return 0;
+}
+
+int psg_create(const psg_config *cfg, psg_ctx **out) {
+  if (!cfg || !out) { set_error("null argument"); return PSG_EINVAL; }
+  *out = nullptr;
+  const bool is2d = cfg->flavour == PSG_NS2D;
+  if (cfg->flavour != PSG_NS2D && cfg->flavour != PSG_NS3D && cfg->flavour != PSG_EULER3D) {
+    set_error("flavour %d not available in this build", cfg->flavour);
+    return PSG_EINVAL;
+  }
+  if (!supported_len(cfg->nx) || !supported_len(cfg->ny) || (!is2d && !supported_len(cfg->nz))) {
+    set_error("grid %dx%dx%d: each extent must be a power of two in [16,1024]", cfg->nx, cfg->ny, cfg->nz);
+    return PSG_EINVAL;
+  }
+  if (cfg->batch < 1 || (cfg->precision != PSG_FP64 && cfg->precision != PSG_FP32BULK) || !(cfg->dt > 0.0) ||
+      !(cfg->alpha > 0.0)) {
+    set_error("invalid batch/precision/dt/alpha");
+    return PSG_EINVAL;
+  }
+  if (psg_device_count() < 1) { set_error("no HIP device visible"); return PSG_ENODEV; }
+  psg_ctx *c = new psg_ctx();
+  c->cfg = *cfg;
+  c->batch = cfg->batch;
+  c->nf = is2d ? 1 : (cfg->flavour == PSG_BOUS3D ? 4 : 3);
+  psgh_grid hg;
+  if (is2d) psgh_grid2(&hg, cfg->nx, cfg->ny, cfg->alpha);
+  else psgh_grid3(&hg, cfg->nx, cfg->ny, cfg->nz, cfg->alpha);
+  Grid &g = c->g;
+  g.dim = hg.dim; g.nx = hg.nx; g.ny = hg.ny; g.nz = hg.nz;
+  g.ktx = hg.ktx; g.kty = hg.kty; g.ktz = hg.ktz;
+  g.iktx = hg.iktx; g.ikty = hg.ikty; g.iktz = hg.iktz;
+  g.alpha = hg.alpha; g.nkt = hg.nkt;
+  g.xp = (g.iktx + 15) / 16 * 16;
+  g.zsz = is2d ? 0 : (long)g.nz * g.ikty * g.xp;
+  g.ysz = (long)g.nz * g.ny * g.xp;
+  g.inr = 1.0 / (double)hg.nr;
+  g.kmax2 = ((double)g.kty - 0.5) * ((double)g.kty - 0.5);
+  hipGetDevice(&c->device);
+  int rc = 0;
+  do {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = PSG_EHIP; set_error("stream"); break; }
+    const size_t ssz = sizeof(cpx<double>) * c->nf * g.nkt * c->batch;
+    if (hipMalloc(&c->state, ssz) != hipSuccess || hipMalloc(&c->tmp, sizeof(cpx<double>) * c->nf * g.nkt) != hipSuccess ||
+        hipMalloc(&c->mask, g.nkt) != hipSuccess) { rc = PSG_ENOMEM; set_error("state allocation failed"); break; }
+    hipMemset(c->state, 0, ssz);
+    if (!is2d) {
+      if (hipMalloc(&c->nuzn, sizeof(double) * g.nkt) != hipSuccess ||
+          hipMalloc(&c->nu1, sizeof(double) * g.nkt) != hipSuccess) { rc = PSG_ENOMEM; set_error("tables"); break; }
+    }
+    if ((rc = upload_tables(c))) break;
+    if ((rc = upload_twiddles(c))) break;
+    rc = is2d ? ns2d_init(c) : ns3d_init(c);
+    if (rc) break;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = PSG_EHIP; set_error("init sync"); break; }
+  } while (0);
+  if (rc) { free_ctx(c); delete c; return rc; }
+  *out = c;
+  return 0;
+}
+
+int psg_destroy(psg_ctx *c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_ctx(c);
+  delete c;
+  return 0;
+}
+
+static int check_member(psg_ctx *c, int m) {
+  if (!c) { set_error("null context"); return PSG_EINVAL; }
+  if (m < 0 || m >= c->batch) { set_error("member %d out of range [0,%d)", m, c->batch); return PSG_EINVAL; }
+  return 0;
+}
+
+int psg_set_state(psg_ctx *c, int m, const double *host) {
+  if (int r = check_member(c, m)) return r;
+  const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  PSG_HIP(hipMemcpyAsync(c->state + (long)m * c->nf * c->g.nkt, host, n, hipMemcpyHostToDevice, c->stream));
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int psg_get_state(psg_ctx *c, int m, double *host) {
+  if (int r = check_member(c, m)) return r;
+  const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  PSG_HIP(hipMemcpyAsync(host, c->state + (long)m * c->nf * c->g.nkt, n, hipMemcpyDeviceToHost, c->stream));
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int psg_state_ptr(psg_ctx *c, int m, void **dev) {
+  if (int r = check_member(c, m)) return r;
+  *dev = c->state + (long)m * c->nf * c->g.nkt;
+  return 0;
+}
+
+int psg_set_state_device(psg_ctx *c, int m, const void *dev) {
+  if (int r = check_member(c, m)) return r;
+  const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  PSG_HIP(hipMemcpyAsync(c->state + (long)m * c->nf * c->g.nkt, dev, n, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+int psg_get_state_device(psg_ctx *c, int m, void *dev) {
+  if (int r = check_member(c, m)) return r;
+  const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  PSG_HIP(hipMemcpyAsync(dev, c->state + (long)m * c->nf * c->g.nkt, n, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+int psg_run(psg_ctx *c, int nsteps) {
+  if (!c || nsteps < 0) { set_error("invalid run arguments"); return PSG_EINVAL; }
+  if (nsteps == 0) return 0;
+  int r = c->g.dim == 2 ? ns2d_run(c, nsteps) : ns3d_run(c, nsteps);
+  if (c->prof) prof_collect(c);
+  return r;
+}
+
+int psg_set_dt(psg_ctx *c, double dt) {
+  if (!c || !(dt > 0.0)) { set_error("invalid dt"); return PSG_EINVAL; }
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  c->cfg.dt = dt;
+  return upload_tables(c);
+}
+
+int psg_sync(psg_ctx *c) {
+  if (!c) return PSG_EINVAL;
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int psg_diag(psg_ctx *c, int m, double *e, double *z, double *spec) {
+  if (int r = check_member(c, m)) return r;
+  return diag_member(c, m, e, z, spec);
+}
+
+int psg_nonlin(psg_ctx *c, int m, const double *host_in, double *host_out) {
+  if (int r = check_member(c, m)) return r;
+  const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  cpx<double> *dout = nullptr;
+  PSG_HIP(hipMalloc(&dout, n));
+  PSG_HIP(hipMemcpyAsync(c->tmp, host_in, n, hipMemcpyHostToDevice, c->stream));
+  int r = c->g.dim == 2 ? ns2d_nonlin(c, 0, c->tmp, dout) : ns3d_nonlin(c, 0, c->tmp, dout);
+  if (!r) {
+    if (hipMemcpyAsync(host_out, dout, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) { set_error("nonlin copy"); r = PSG_EHIP; }
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(dout);
+  return r;
+}
+
+int psg_physical(psg_ctx *c, int m, double *host_out) {
+  if (int r = check_member(c, m)) return r;
+  const long nr = (long)c->g.nx * c->g.ny * c->g.nz;
+  const int nfo = c->g.dim == 2 ? 3 : 6;
+  double *d = nullptr;
+  PSG_HIP(hipMalloc(&d, sizeof(double) * nr * nfo));
+  int r = c->g.dim == 2 ? ns2d_physical(c, m, d) : ns3d_physical(c, m, d);
+  if (!r) {
+    if (hipMemcpyAsync(host_out, d, sizeof(double) * nr * nfo, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) { set_error("physical copy"); r = PSG_EHIP; }
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(d);
+  return r;
+}
+
+void *psg_stream(psg_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int psg_profile_enable(psg_ctx *c, int on) {
+  if (!c) return PSG_EINVAL;
+  hipStreamSynchronize(c->stream);
+  c->prof = on != 0;
+  c->profs.clear();
+  c->ev_used = 0;
+  return 0;
+}
+
+int psg_profile_get(psg_ctx *c, int idx, const char **name, double *avg_ms, long *bytes, int *count) {
+  if (!c) return PSG_EINVAL;
+  prof_collect(c);
+  if (idx < 0 || idx >= (int)c->profs.size()) return PSG_EINVAL;
+  const ProfEntry &e = c->profs[idx];
+  if (name) *name = e.name.c_str();
+  if (avg_ms) *avg_ms = e.count ? e.ms / e.count : 0.0;
+  if (bytes) *bytes = e.count ? e.bytes / e.count : 0;
+  if (count) *count = e.count;
+  return 0;
+}
+
+}  // extern "C"

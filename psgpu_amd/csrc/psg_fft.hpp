@@ -1,0 +1,147 @@
+// Device FFT building blocks for gfx950 (CDNA4).
+//
+// A pencil of N points (N a power of two) is transformed by a team of
+// TP = N/E threads, each holding E points in registers: thread j holds the
+// points j + TP*m, m < E, both before and after the transform (self-sorting
+// Stockham formulation: every stage reads with stride N/R and writes to
+// (b/Ns)*Ns*R + b%Ns + r*Ns).  Stages are radix E until fewer than E points
+// per sub-transform remain, then one smaller radix; intermediate stages
+// exchange through LDS, the first reads registers and the last writes
+// registers, so a 256-point FP64 pencil with E=8 costs two LDS round trips.
+//
+// LDS addressing is lds[pos*stride + off]: column tiles use stride = tile
+// width and off = column (consecutive lanes hit consecutive columns), row
+// teams use stride = 1 and off = team base.
+//
+// Twiddles come from a per-length table W_N^k = e^{-2 pi i k/N} built on the
+// host in FP64 (glibc sincos) and rounded once for the FP32 table.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace psg {
+
+template <typename T> struct cpx { T x, y; };
+
+template <typename T> __device__ __forceinline__ cpx<T> mk(T a, T b) { return cpx<T>{a, b}; }
+template <typename T> __device__ __forceinline__ cpx<T> add(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ cpx<T> sub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ cpx<T> mul(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by conj(b)
+template <typename T> __device__ __forceinline__ cpx<T> mulc(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV> __device__ __forceinline__ cpx<T> rot90(cpx<T> a) {
+  return INV ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
+}
+
+template <typename T> struct K;
+template <> struct K<double> { static constexpr double r2 = 0.70710678118654752440084436210485; };
+template <> struct K<float> { static constexpr float r2 = 0.70710678118654752440084436210485f; };
+
+template <typename T, bool INV> __device__ __forceinline__ void dft2(cpx<T> &a, cpx<T> &b) {
+  cpx<T> t = a; a = add(t, b); b = sub(t, b);
+}
+
+template <typename T, bool INV> __device__ __forceinline__ void dft4(cpx<T> *v, int s) {
+  // v[0], v[s], v[2s], v[3s]
+  cpx<T> a0 = add(v[0], v[2 * s]), a1 = sub(v[0], v[2 * s]);
+  cpx<T> a2 = add(v[s], v[3 * s]), a3 = sub(v[s], v[3 * s]);
+  a3 = rot90<T, INV>(a3);
+  v[0] = add(a0, a2); v[2 * s] = sub(a0, a2);
+  v[s] = add(a1, a3); v[3 * s] = sub(a1, a3);
+}
+
+template <typename T, bool INV> __device__ __forceinline__ void dft8(cpx<T> *v) {
+  // even/odd split: E = dft4(v0,v2,v4,v6), O = dft4(v1,v3,v5,v7)
+  dft4<T, INV>(v, 2);
+  dft4<T, INV>(v + 1, 2);
+  const T c = K<T>::r2;
+  // twiddle odd part by W8^k (forward e^{-i pi k/4})
+  cpx<T> o1 = v[3], o2 = v[5], o3 = v[7];
+  o1 = INV ? cpx<T>{c * (o1.x - o1.y), c * (o1.x + o1.y)} : cpx<T>{c * (o1.x + o1.y), c * (o1.y - o1.x)};
+  o2 = rot90<T, INV>(o2);
+  o3 = INV ? cpx<T>{-c * (o3.x + o3.y), c * (o3.x - o3.y)} : cpx<T>{c * (o3.y - o3.x), -c * (o3.x + o3.y)};
+  cpx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+  v[0] = add(e0, o0); v[4] = sub(e0, o0);
+  v[1] = add(e1, o1); v[5] = sub(e1, o1);
+  v[2] = add(e2, o2); v[6] = sub(e2, o2);
+  v[3] = add(e3, o3); v[7] = sub(e3, o3);
+}
+
+template <typename T, bool INV, int R> __device__ __forceinline__ void dftR(cpx<T> *v) {
+  if constexpr (R == 2) dft2<T, INV>(v[0], v[1]);
+  else if constexpr (R == 4) dft4<T, INV>(v, 1);
+  else if constexpr (R == 8) dft8<T, INV>(v);
+  else static_assert(R == 1, "unsupported radix");
+}
+
+// One Stockham stage.  NS = product of the previous radices.
+template <typename T, int N, int E, int NS, bool INV>
+__device__ __forceinline__ void fft_stage(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
+                                          const cpx<T> *__restrict__ tw) {
+  constexpr int TP = N / E;
+  constexpr int R = (N / NS >= E) ? E : N / NS;
+  constexpr int G = E / R;            // butterflies per thread
+  constexpr int NB = N / R;
+  constexpr bool FIRST = (NS == 1);
+  constexpr bool LAST = (NS * R == N);
+  if constexpr (!FIRST) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; m++) v[m] = lds[(j + TP * m) * stride + off];
+  }
+#pragma unroll
+  for (int t = 0; t < G; t++) {
+    cpx<T> u[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) u[r] = v[t + r * G];
+    const int b = j + t * TP;
+    const int k = b % NS;
+    if constexpr (!FIRST) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        cpx<T> w = tw[(r * k) * (N / (NS * R))];
+        u[r] = INV ? mulc(u[r], w) : mul(u[r], w);
+      }
+    }
+    dftR<T, INV, R>(u);
+#pragma unroll
+    for (int r = 0; r < R; r++) v[t + r * G] = u[r];
+  }
+  (void)NB;
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < G; t++) {
+      const int b = j + t * TP;
+      const int k = b % NS;
+      const int d = (b / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) lds[(d + r * NS) * stride + off] = v[t + r * G];
+    }
+  }
+}
+
+template <typename T, int N, int E, int NS, bool INV>
+__device__ __forceinline__ void fft_stages(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
+                                           const cpx<T> *__restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int R = (N / NS >= E) ? E : N / NS;
+    fft_stage<T, N, E, NS, INV>(v, lds, j, stride, off, tw);
+    fft_stages<T, N, E, NS * R, INV>(v, lds, j, stride, off, tw);
+  }
+}
+
+// Full N-point transform of the pencil held by a team (see header comment).
+// Every thread of the block must call this (it contains block barriers).
+template <typename T, int N, int E, bool INV>
+__device__ __forceinline__ void fft_pencil(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
+                                           const cpx<T> *__restrict__ tw) {
+  static_assert((N & (N - 1)) == 0 && N >= E, "power-of-two pencil");
+  fft_stages<T, N, E, 1, INV>(v, lds, j, stride, off, tw);
+}
+
+}  // namespace psg

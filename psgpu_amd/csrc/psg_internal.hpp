@@ -1,0 +1,96 @@
+// Internal structures shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/psg.h"
+#include "psg_fft.hpp"
+
+namespace psg {
+
+// Grid description passed by value to every kernel.
+struct Grid {
+  int dim;
+  int nx, ny, nz;
+  int ktx, kty, ktz;
+  int iktx, ikty, iktz;
+  int xp;          // padded kx pitch of the pass buffers (multiple of 8)
+  double alpha;
+  long nkt;        // stored modes per field
+  long zsz;        // per-field size of z-transformed buffers: nz*ikty*xp (2D: 0)
+  long ysz;        // per-field size of y-transformed buffers: nz*ny*xp
+  double inr;      // 1/NR
+  double kmax2;    // (KTY-0.5)^2, only used for documentation/tests
+};
+
+enum { PREC64 = 0, PREC32 = 1 };
+
+struct ProfEntry {
+  std::string name;
+  double ms = 0.0;
+  long bytes = 0;
+  int count = 0;
+};
+
+struct Ctx {
+  psg_config cfg;
+  Grid g;
+  int nf;          // state fields per member
+  int batch;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // state and scheme arrays (FP64 complex), member-major [batch][nf][nkt]
+  cpx<double> *state = nullptr;
+  cpx<double> *rh = nullptr;        // Heun: RH store; RK4: stage arrays
+  cpx<double> *stage = nullptr;     // RK4: Z0 stage input / generic scratch
+  cpx<double> *tmp = nullptr;       // scratch state-sized buffer
+  double *nuzn = nullptr, *nu1 = nullptr;  // CN tables (host-built, reference formula)
+  int8_t *mask = nullptr;           // active mask L
+  // pass buffers (precision P)
+  void *bufA = nullptr, *bufB = nullptr, *bufC = nullptr;
+  size_t szA = 0, szB = 0, szC = 0;
+  // twiddle tables: [0]=x, [1]=y, [2]=z lengths; FP64 and FP32 copies
+  cpx<double> *tw64[3] = {nullptr, nullptr, nullptr};
+  cpx<float> *tw32[3] = {nullptr, nullptr, nullptr};
+  double time = 0.0;
+  // diagnostics scratch
+  double *dred = nullptr;
+  size_t dred_n = 0;
+  // profiling
+  bool prof = false;
+  std::vector<ProfEntry> profs;
+  std::vector<hipEvent_t> evs;
+  int ev_used = 0;
+};
+
+// error helpers (defined in psg_api.hip)
+void set_error(const char *fmt, ...);
+#define PSG_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::psg::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                       __FILE__, __LINE__);                                    \
+      return PSG_EHIP;                                                         \
+    }                                                                          \
+  } while (0)
+
+// profiling marks around launches (no-ops unless ctx->prof)
+void prof_begin(Ctx *c, const char *name, long bytes);
+void prof_end(Ctx *c);
+
+// per-flavour drivers
+int ns3d_init(Ctx *c);
+int ns3d_run(Ctx *c, int nsteps);
+int ns3d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
+int ns3d_physical(Ctx *c, int member, double *out_dev);
+int ns2d_init(Ctx *c);
+int ns2d_run(Ctx *c, int nsteps);
+int ns2d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
+int ns2d_physical(Ctx *c, int member, double *out_dev);
+int diag_member(Ctx *c, int member, double *e, double *z, double *spec);
+
+}  // namespace psg
+
+struct psg_ctx : public psg::Ctx {};

@@ -1,0 +1,54 @@
+// Launch helpers: compile-time shapes per pencil length and runtime dispatch.
+#pragma once
+#include "psg_passes.hpp"
+
+namespace psg {
+
+// column tile: T kx columns x N/8 threads per column
+template <int N> struct Col {
+  static constexpr int TP = N / EPT;
+  static constexpr int T0 = 256 / TP;
+  static constexpr int T = T0 < 8 ? 8 : (T0 > 32 ? 32 : T0);
+  static constexpr int THREADS = T * TP;
+};
+// x-pass row teams
+template <int N> struct Row {
+  static constexpr int TPX = N / EPT;
+  static constexpr int TEAMS = (256 / TPX) < 1 ? 1 : 256 / TPX;
+  static constexpr int THREADS = TEAMS * TPX;
+};
+
+template <typename P> inline const cpx<P> *twp(const Ctx *c, int axis);
+template <> inline const cpx<double> *twp<double>(const Ctx *c, int axis) { return c->tw64[axis]; }
+template <> inline const cpx<float> *twp<float>(const Ctx *c, int axis) { return c->tw32[axis]; }
+
+// raise the dynamic-LDS limit of a kernel once when it needs more than 64 KiB
+template <typename F> inline hipError_t lds_attr(F *kernel, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+#define PSG_CASE_N(NV, ...) \
+  case NV: {                \
+    constexpr int N = NV;   \
+    __VA_ARGS__;            \
+  } break;
+
+#define PSG_DISPATCH_N(n, ...)                                   \
+  switch (n) {                                                   \
+    PSG_CASE_N(16, __VA_ARGS__)                                  \
+    PSG_CASE_N(32, __VA_ARGS__)                                  \
+    PSG_CASE_N(64, __VA_ARGS__)                                  \
+    PSG_CASE_N(128, __VA_ARGS__)                                 \
+    PSG_CASE_N(256, __VA_ARGS__)                                 \
+    PSG_CASE_N(512, __VA_ARGS__)                                 \
+    PSG_CASE_N(1024, __VA_ARGS__)                                \
+    default:                                                     \
+      ::psg::set_error("unsupported transform length %d", (int)(n)); \
+      return PSG_EINVAL;                                         \
+  }
+
+inline bool supported_len(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+
+}  // namespace psg

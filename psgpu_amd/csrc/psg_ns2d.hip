@@ -1,0 +1,254 @@
+// 2D Navier-Stokes vorticity stepper: four RK4 stages without viscosity and
+// an end-of-step Crank-Nicolson factor.
+//   stage 1,2 (Step1):  Zs = dt N(Z0);  Z0 = Z + Zs/2
+//   stage 3   (Step2):  Z3 = dt N(Z0);  Z0 = Z + Z3
+//   final     (Step):   Z = [(1 - dt nu k^2/2) Z + (Z1 + 2 Z2 + 2 Z3 + dt N(Z0))/6] / (1 + dt nu k^2/2)
+// with N = -i(kx FFT(u w) + ky FFT(v w)) and forcing NZKX -= A at (0,+-KF);
+// off-mask modes of Z are zeroed at every stage.
+// (2D_NavierStokes/src_RK4/src_gmres/cuda.cu:44-76, kernels.h:247-435)
+// Each stage is three kernels: y-inverse from the state (fill, mask,
+// Biot-Savart), the shared x-pass, and a y-forward pass ending in the stage
+// update.
+#include "psg_launch.hpp"
+
+namespace psg {
+
+template <int STAGE> struct Upd2D {
+  cpx<double> *z, *z0, *zs;  // state, stage input, stage arrays Z1,Z2,Z3 ([m][3][nkt])
+  long nkt;
+  const int8_t *L;
+  double dt, v2, famp, kf;
+  __device__ __forceinline__ void operator()(int m, long ik, int, double kx, double ky,
+                                             cpx<double> (&N)[2]) const {
+    cpx<double> *Z = z + (long)m * nkt + ik;
+    if (L[ik] == 1) {
+      // NZK[ik] = FFT(v w), NZK[nkt+ik] = FFT(u w)
+      double NZKX = kx * N[1].y + ky * N[0].y;
+      double NZKY = -kx * N[1].x - ky * N[0].x;
+      if (kx == 0.0 && fabs(ky) == kf) NZKX += famp;
+      const double ZX = dt * NZKX, ZY = dt * NZKY;
+      cpx<double> *Z0 = z0 + (long)m * nkt + ik;
+      cpx<double> *S = zs + (long)m * 3 * nkt + ik;
+      const cpx<double> zn = *Z;
+      if constexpr (STAGE == 0 || STAGE == 1) {
+        S[STAGE * nkt] = cpx<double>{ZX, ZY};
+        *Z0 = cpx<double>{zn.x + 0.5 * ZX, zn.y + 0.5 * ZY};
+      } else if constexpr (STAGE == 2) {
+        S[2 * nkt] = cpx<double>{ZX, ZY};
+        *Z0 = cpx<double>{zn.x + ZX, zn.y + ZY};
+      } else {
+        const double wk = kx * kx + ky * ky;
+        const double damp1 = 1 - 0.5 * dt * v2 * wk;
+        const double damp2 = 1. / (1 + 0.5 * dt * v2 * wk);
+        const cpx<double> z1 = S[0], z2 = S[nkt], z3 = S[2 * nkt];
+        cpx<double> r;
+        r.x = damp2 * (damp1 * zn.x + (z1.x + 2 * z2.x + 2 * z3.x + ZX) / 6.);
+        r.y = damp2 * (damp1 * zn.y + (z1.y + 2 * z2.y + 2 * z3.y + ZY) / 6.);
+        *Z = r;
+        *Z0 = r;
+      }
+    } else {
+      *Z = cpx<double>{0.0, 0.0};
+    }
+  }
+};
+
+struct Upd2DStoreNL {
+  cpx<double> *out;
+  long nkt;
+  const int8_t *L;
+  double famp, kf;
+  __device__ __forceinline__ void operator()(int m, long ik, int, double kx, double ky,
+                                             cpx<double> (&N)[2]) const {
+    double NZKX = kx * N[1].y + ky * N[0].y;
+    double NZKY = -kx * N[1].x - ky * N[0].x;
+    if (kx == 0.0 && fabs(ky) == kf) NZKX += famp;
+    const double ll = (double)L[ik];
+    out[(long)m * nkt + ik] = cpx<double>{ll * NZKX, ll * NZKY};
+  }
+};
+
+// masked copy Z0 = L Z (setVelocity, kernels.h:112-147)
+__global__ void k2_setz0(const cpx<double> *__restrict__ z, cpx<double> *__restrict__ z0, const int8_t *L,
+                         long nkt, long total) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long ik = i % nkt;
+    z0[i] = L[ik] ? z[i] : cpx<double>{0.0, 0.0};
+  }
+}
+
+template <typename P> static int launch_yinv2(Ctx *c, const cpx<double> *st, cpx<P> *out, int nm) {
+  const Grid &g = c->g;
+  PSG_DISPATCH_N(g.ny, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    auto k = k2_yinv<P, N, T>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, g.nkt, out, 3 * g.ysz, g,
+                       twp<P>(c, 1), c->mask);
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename P, class UPD> static int launch_yfwd2(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
+  const Grid &g = c->g;
+  PSG_DISPATCH_N(g.ny, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    auto k = k2_yfwd<P, N, T, UPD>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, 2 * g.ysz, g, twp<P>(c, 1), upd);
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename P, int FI, int FO, class PROD>
+static int launch_xpass2(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
+  const Grid &g = c->g;
+  PSG_DISPATCH_N(g.nx, {
+    constexpr int TE = Row<N>::TEAMS;
+    long nrows = nrow_m * nm;
+    dim3 grid((unsigned)((nrows + TE - 1) / TE));
+    size_t lds = (size_t)N * TE * sizeof(cpx<P>);
+    auto k = k_xpass<P, N, TE, FI, FO, PROD>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
+                       prod);
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+#define PSG_TRY2(x)     \
+  do {                  \
+    int r_ = (x);       \
+    if (r_) return r_;  \
+  } while (0)
+
+template <typename P, class UPD> static int rhs2(Ctx *c, const cpx<double> *z0, UPD upd, int nm) {
+  const Grid &g = c->g;
+  cpx<P> *B = (cpx<P> *)c->bufB, *Cb = (cpx<P> *)c->bufC;
+  const long Y = (long)g.iktx * g.ny;
+  const size_t b = sizeof(cpx<P>);
+  prof_begin(c, "yinv2", (long)(g.nkt * 16 + 3 * Y * b) * nm);
+  PSG_TRY2(launch_yinv2<P>(c, z0, B, nm));
+  prof_end(c);
+  prof_begin(c, "xpass2", (long)(5 * Y * b) * nm);
+  PSG_TRY2((launch_xpass2<P, 3, 2>(c, B, Cb, g.ny, nm, Prod2D<P>{})));
+  prof_end(c);
+  prof_begin(c, "yfwd2_update", (long)(2 * Y * b + 4 * g.nkt * 16) * nm);
+  PSG_TRY2((launch_yfwd2<P>(c, Cb, upd, nm)));
+  prof_end(c);
+  return 0;
+}
+
+template <typename P> static int run2_t(Ctx *c, int nsteps) {
+  const Grid &g = c->g;
+  const long total = g.nkt * c->batch;
+  hipLaunchKernelGGL(k2_setz0, dim3((unsigned)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096)),
+                     dim3(256), 0, c->stream, c->state, c->stage, c->mask, g.nkt, total);
+  PSG_HIP(hipGetLastError());
+  const double famp = c->cfg.force == PSG_FORCE_NONE ? 0.0 : c->cfg.ampfor;
+  for (int nt = 0; nt < nsteps; nt++) {
+    Upd2D<0> u0{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
+    PSG_TRY2(rhs2<P>(c, c->stage, u0, c->batch));
+    Upd2D<1> u1{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
+    PSG_TRY2(rhs2<P>(c, c->stage, u1, c->batch));
+    Upd2D<2> u2{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
+    PSG_TRY2(rhs2<P>(c, c->stage, u2, c->batch));
+    Upd2D<3> u3{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
+    PSG_TRY2(rhs2<P>(c, c->stage, u3, c->batch));
+    c->time += c->cfg.dt;
+  }
+  return 0;
+}
+
+int ns2d_run(Ctx *c, int nsteps) {
+  return c->cfg.precision == PSG_FP32BULK ? run2_t<float>(c, nsteps) : run2_t<double>(c, nsteps);
+}
+
+template <typename P> static int nonlin2_t(Ctx *c, const cpx<double> *in, cpx<double> *out) {
+  const double famp = c->cfg.force == PSG_FORCE_NONE ? 0.0 : c->cfg.ampfor;
+  Upd2DStoreNL u{out, c->g.nkt, c->mask, famp, c->cfg.kf};
+  return rhs2<P>(c, in, u, 1);
+}
+
+int ns2d_nonlin(Ctx *c, int, const cpx<double> *in_dev, cpx<double> *out_dev) {
+  return c->cfg.precision == PSG_FP32BULK ? nonlin2_t<float>(c, in_dev, out_dev)
+                                          : nonlin2_t<double>(c, in_dev, out_dev);
+}
+
+template <typename P, int N, int TE>
+__global__ void __launch_bounds__(TE *(N / EPT))
+k2_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, long nrow, Grid g, const cpx<P> *__restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TPX = N / EPT;
+  const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
+  const long row = (long)blockIdx.x * TE + team;
+  const bool valid = row < nrow;
+  const int f = blockIdx.y;
+  const cpx<P> *ra = in + (long)f * g.ysz + (valid ? row : 0) * g.xp;
+  cpx<P> v[EPT];
+#pragma unroll
+  for (int mm = 0; mm < EPT; mm++) {
+    const int k = j + TPX * mm;
+    cpx<P> a{0, 0};
+    if (valid) {
+      if (k < g.iktx) { a = ra[k]; if (k == 0) a.y = 0; }
+      else if (k > N - g.iktx) { a = ra[N - k]; a.y = -a.y; }
+    }
+    v[mm] = a;
+  }
+  fft_pencil<P, N, EPT, true>(v, lds + team * N, j, 1, 0, tw);
+  if (valid) {
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) out[((long)f * nrow + row) * N + j + TPX * mm] = (double)v[mm].x;
+  }
+}
+
+template <typename P> static int physical2_t(Ctx *c, int member, double *out) {
+  const Grid &g = c->g;
+  cpx<P> *B = (cpx<P> *)c->bufB;
+  // physical fields of the masked state (w, u, v)
+  const long total = g.nkt;
+  hipLaunchKernelGGL(k2_setz0, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream,
+                     c->state + (long)member * g.nkt, c->tmp, c->mask, g.nkt, total);
+  PSG_TRY2(launch_yinv2<P>(c, c->tmp, B, 1));
+  long nrow = g.ny;
+  PSG_DISPATCH_N(g.nx, {
+    constexpr int TE = Row<N>::TEAMS;
+    dim3 grid((unsigned)((nrow + TE - 1) / TE), 3);
+    size_t lds = (size_t)N * TE * sizeof(cpx<P>);
+    auto k = k2_xphys<P, N, TE>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, B, out, nrow, g, twp<P>(c, 0));
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+int ns2d_physical(Ctx *c, int member, double *out_dev) {
+  return c->cfg.precision == PSG_FP32BULK ? physical2_t<float>(c, member, out_dev)
+                                          : physical2_t<double>(c, member, out_dev);
+}
+
+int ns2d_init(Ctx *c) {
+  const Grid &g = c->g;
+  size_t b = c->cfg.precision == PSG_FP32BULK ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  c->szB = (size_t)3 * g.ysz * c->batch * b;
+  c->szC = (size_t)2 * g.ysz * c->batch * b;
+  PSG_HIP(hipMalloc(&c->bufB, c->szB));
+  PSG_HIP(hipMalloc(&c->bufC, c->szC));
+  PSG_HIP(hipMalloc(&c->rh, sizeof(cpx<double>) * 3 * g.nkt * c->batch));
+  PSG_HIP(hipMalloc(&c->stage, sizeof(cpx<double>) * g.nkt * c->batch));
+  PSG_HIP(hipMemsetAsync(c->rh, 0, sizeof(cpx<double>) * 3 * g.nkt * c->batch, c->stream));
+  PSG_HIP(hipMemsetAsync(c->stage, 0, sizeof(cpx<double>) * g.nkt * c->batch, c->stream));
+  return 0;
+}
+
+}  // namespace psg

@@ -1,0 +1,410 @@
+// Pencil-pass kernels of the pseudospectral right-hand side on gfx950.
+//
+// One right-hand side (3D) is five HBM passes over pruned pencils:
+//   1 zinv : truncated spectral state -> Hermitian fill of the kx=0 plane,
+//            Biot-Savart velocity, zero-pad in z, inverse z-FFT
+//            state [m][f][nkt] (FP64)          -> A [m][F][NZ][IKTY][XP]
+//   2 yinv : zero-pad in y, inverse y-FFT        A -> B [m][F][NZ][NY][XP]
+//   3 xpass: per (y,z) row: C2R of the fields (two real fields per complex
+//            FFT), pointwise product in registers, R2C, truncation to IKTX;
+//            physical space never reaches HBM    B -> C [m][FO][NZ][NY][XP]
+//   4 yfwd : forward y-FFT, truncation in y      C -> A' [m][FO][NZ][IKTY][XP]
+//   5 zfwd : forward z-FFT, truncation in z, x 1/NR (FP64), then the scheme's
+//            update functor at every stored mode (curl, forcing, mask, CN).
+// 2D uses passes 2-5 shape-wise: y-inverse straight from the state, the same
+// x-pass, and a y-forward pass that ends in the update functor.
+//
+// Column passes use a tile of T consecutive kx columns per block (coalesced
+// along the contiguous kx axis) and N/8 threads per column; the x-pass uses
+// one team of N/8 threads per row.  Every pass is templated on the storage
+// precision P of the pass buffers (double: plain form, float: FP32-bulk);
+// state, tables, update arithmetic and diagnostics stay FP64.
+#pragma once
+#include "psg_internal.hpp"
+
+namespace psg {
+
+constexpr int EPT = 8;  // points per thread in every pencil transform
+
+template <typename P> __device__ __forceinline__ cpx<P> to(cpx<double> a) { return {(P)a.x, (P)a.y}; }
+__device__ __forceinline__ cpx<double> up(cpx<double> a) { return a; }
+__device__ __forceinline__ cpx<double> up(cpx<float> a) { return {(double)a.x, (double)a.y}; }
+
+// kz (or ky) of padded position p, or INT_MIN when p is in the zero band
+__device__ __forceinline__ int wrapk(int p, int n, int kt) {
+  return p <= kt ? p : (p >= n - kt ? p - n : -0x40000000);
+}
+#define PSG_NOK (-0x40000000)
+
+// Biot-Savart u = i k x w / max(|k|^2,1e-3): 3D_NavierStokes/src/kernels.h:187-201
+__device__ __forceinline__ void biot_savart(double kx, double ky, double kz, cpx<double> xi, cpx<double> et,
+                                            cpx<double> zt, double wki, cpx<double> &u, cpx<double> &v,
+                                            cpx<double> &w) {
+  u.x = -(ky * zt.y - kz * et.y) * wki;
+  u.y = -(kz * et.x - ky * zt.x) * wki;
+  v.x = -(kz * xi.y - kx * zt.y) * wki;
+  v.y = -(kx * zt.x - kz * xi.x) * wki;
+  w.x = -(kx * et.y - ky * xi.y) * wki;
+  w.y = -(ky * xi.x - kx * et.x) * wki;
+}
+
+// ---------------------------------------------------------------- pass 1 (3D)
+// out fields: 0..2 vorticity, 3..5 velocity (if BS).  umask: velocity masked
+// by L as setVelocity does before the first step (kernels.h:191).
+template <typename P, int N, int T, bool BS>
+__global__ void __launch_bounds__(T *(N / EPT))
+k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
+       const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TP = N / EPT;
+  const int col = threadIdx.x % T, j = threadIdx.x / T;
+  const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
+  const bool valid = ikx < g.iktx;
+  const int ky = iky - g.kty;
+  const double kx = (double)ikx * g.alpha, kyd = (double)ky;
+  const cpx<double> *s0 = st + (long)m * st_ms;
+  cpx<double> w[3][EPT];
+  double lm[EPT];
+#pragma unroll
+  for (int mm = 0; mm < EPT; mm++) {
+    const int z = j + TP * mm;
+    const int kz = wrapk(z, g.nz, g.ktz);
+    cpx<double> a{0, 0}, b{0, 0}, c{0, 0};
+    double l = 0.0;
+    if (valid && kz != PSG_NOK) {
+      const bool zero = (ikx == 0 && ky == 0 && kz == 0);
+      const bool mirror = (ikx == 0) && (ky < 0 || (ky == 0 && kz < 0));
+      if (!zero) {
+        long idx = mirror ? ((long)(-kz + g.ktz) * g.ikty + (-ky + g.kty)) * g.iktx
+                          : ((long)(kz + g.ktz) * g.ikty + iky) * g.iktx + ikx;
+        a = s0[idx]; b = s0[idx + st_fs]; c = s0[idx + 2 * st_fs];
+        if (mirror) { a.y = -a.y; b.y = -b.y; c.y = -c.y; }
+        l = umask ? (double)mask[idx] : 1.0;
+      }
+    }
+    w[0][mm] = a; w[1][mm] = b; w[2][mm] = c; lm[mm] = l;
+  }
+  constexpr int NFO = BS ? 6 : 3;
+#pragma unroll
+  for (int f = 0; f < NFO; f++) {
+    cpx<P> v[EPT];
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) {
+      if (f < 3) {
+        v[mm] = to<P>(w[f][mm]);
+      } else {
+        const int z = j + TP * mm;
+        const int kz = wrapk(z, g.nz, g.ktz);
+        const double kzd = kz == PSG_NOK ? 0.0 : (double)kz;
+        const double wk = kx * kx + kyd * kyd + kzd * kzd;
+        double wki = 1.0 / (wk > 0.001 ? wk : 0.001);
+        wki *= lm[mm];
+        cpx<double> u, vv, ww;
+        biot_savart(kx, kyd, kzd, w[0][mm], w[1][mm], w[2][mm], wki, u, vv, ww);
+        v[mm] = to<P>(f == 3 ? u : (f == 4 ? vv : ww));
+      }
+    }
+    fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
+    if (valid) {
+      cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz + (long)iky * g.xp + ikx;
+#pragma unroll
+      for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.ikty * g.xp] = v[mm];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- pass 2 (3D)
+template <typename P, int N, int T>
+__global__ void __launch_bounds__(T *(N / EPT))
+k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TP = N / EPT;
+  const int col = threadIdx.x % T, j = threadIdx.x / T;
+  const int ikx = blockIdx.x * T + col, z = blockIdx.y, mf = blockIdx.z;
+  const bool valid = ikx < g.iktx;
+  const cpx<P> *src = in + (long)mf * g.zsz + (long)z * g.ikty * g.xp + ikx;
+  cpx<P> v[EPT];
+#pragma unroll
+  for (int mm = 0; mm < EPT; mm++) {
+    const int y = j + TP * mm;
+    const int ky = wrapk(y, g.ny, g.kty);
+    v[mm] = (valid && ky != PSG_NOK) ? src[(long)(ky + g.kty) * g.xp] : cpx<P>{0, 0};
+  }
+  fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
+  if (valid) {
+    cpx<P> *o = out + (long)mf * g.ysz + (long)z * g.ny * g.xp + ikx;
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.xp] = v[mm];
+  }
+}
+
+// ---------------------------------------------------------------- pass 4 (3D)
+template <typename P, int N, int T>
+__global__ void __launch_bounds__(T *(N / EPT))
+k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TP = N / EPT;
+  const int col = threadIdx.x % T, j = threadIdx.x / T;
+  const int ikx = blockIdx.x * T + col, z = blockIdx.y, mf = blockIdx.z;
+  const bool valid = ikx < g.iktx;
+  const cpx<P> *src = in + (long)mf * g.ysz + (long)z * g.ny * g.xp + ikx;
+  cpx<P> v[EPT];
+#pragma unroll
+  for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[(long)(j + TP * mm) * g.xp] : cpx<P>{0, 0};
+  fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
+  if (valid) {
+    cpx<P> *o = out + (long)mf * g.zsz + (long)z * g.ikty * g.xp + ikx;
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) {
+      const int ky = wrapk(j + TP * mm, g.ny, g.kty);
+      if (ky != PSG_NOK) o[(long)(ky + g.kty) * g.xp] = v[mm];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- pass 5 (3D)
+// UPD::operator()(m, ik, ikx, kx, ky, kz, N[FO]) with N already scaled by 1/NR.
+template <typename P, int N, int T, int FO, class UPD>
+__global__ void __launch_bounds__(T *(N / EPT))
+k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TP = N / EPT;
+  const int col = threadIdx.x % T, j = threadIdx.x / T;
+  const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
+  const bool valid = ikx < g.iktx;
+  cpx<double> res[FO][EPT];
+#pragma unroll
+  for (int f = 0; f < FO; f++) {
+    const cpx<P> *src = in + ((long)m * FO + f) * g.zsz + (long)iky * g.xp + ikx;
+    cpx<P> v[EPT];
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++)
+      v[mm] = valid ? src[(long)(j + TP * mm) * g.ikty * g.xp] : cpx<P>{0, 0};
+    fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) {
+      cpx<double> t = up(v[mm]);
+      res[f][mm] = cpx<double>{t.x * g.inr, t.y * g.inr};
+    }
+  }
+  if (!valid) return;
+  const double kx = (double)ikx * g.alpha, ky = (double)(iky - g.kty);
+#pragma unroll
+  for (int mm = 0; mm < EPT; mm++) {
+    const int kz = wrapk(j + TP * mm, g.nz, g.ktz);
+    if (kz == PSG_NOK) continue;
+    const long ik = ((long)(kz + g.ktz) * g.ikty + iky) * g.iktx + ikx;
+    cpx<double> nn[FO];
+#pragma unroll
+    for (int f = 0; f < FO; f++) nn[f] = res[f][mm];
+    upd(m, ik, ikx, kx, ky, (double)kz, nn);
+  }
+}
+
+// ---------------------------------------------------------------- pass 3 (x)
+// Row teams: TEAMS rows per block, N/8 threads per row.  Fields are paired
+// into complex transforms z = a + i b.  PROD::apply(double in[FI], double out[FO]).
+template <typename P, int N, int TEAMS, int FI, int FO, class PROD>
+__global__ void __launch_bounds__(TEAMS *(N / EPT))
+k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
+        const cpx<P> *__restrict__ tw, PROD prod) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TPX = N / EPT;
+  constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2;
+  const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
+  const long row = (long)blockIdx.x * TEAMS + team;
+  const bool valid = row < nrows;
+  const long m = valid ? row / nrow_m : 0, rr = valid ? row % nrow_m : 0;
+  cpx<P> *tl = lds + team * N;
+  const int ix = g.iktx;
+  P phys[2 * PI][EPT];
+#pragma unroll
+  for (int p = 0; p < PI; p++) {
+    const int fa = 2 * p, fb = 2 * p + 1;
+    const cpx<P> *ra = in + ((long)m * FI + fa) * g.ysz + rr * g.xp;
+    const cpx<P> *rb = in + ((long)m * FI + fb) * g.ysz + rr * g.xp;
+    cpx<P> v[EPT];
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) {
+      const int k = j + TPX * mm;
+      cpx<P> a{0, 0}, b{0, 0};
+      cpx<P> z{0, 0};
+      if (valid) {
+        if (k < ix) {
+          a = ra[k];
+          if (fb < FI) b = rb[k];
+          if (k == 0) { a.y = 0; b.y = 0; }
+          z = cpx<P>{a.x - b.y, a.y + b.x};
+        } else if (k > N - ix) {
+          a = ra[N - k];
+          if (fb < FI) b = rb[N - k];
+          z = cpx<P>{a.x + b.y, b.x - a.y};
+        }
+      }
+      v[mm] = z;
+    }
+    fft_pencil<P, N, EPT, true>(v, tl, j, 1, 0, tw);
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) { phys[fa][mm] = v[mm].x; phys[fb][mm] = v[mm].y; }
+  }
+  P res[2 * PO][EPT];
+#pragma unroll
+  for (int mm = 0; mm < EPT; mm++) {
+    P pin[2 * PI], pout[2 * PO];
+#pragma unroll
+    for (int f = 0; f < 2 * PI; f++) pin[f] = phys[f][mm];
+#pragma unroll
+    for (int f = 0; f < 2 * PO; f++) pout[f] = 0;
+    prod(pin, pout);
+#pragma unroll
+    for (int f = 0; f < 2 * PO; f++) res[f][mm] = pout[f];
+  }
+#pragma unroll
+  for (int q = 0; q < PO; q++) {
+    cpx<P> v[EPT];
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) v[mm] = cpx<P>{res[2 * q][mm], res[2 * q + 1][mm]};
+    fft_pencil<P, N, EPT, false>(v, tl, j, 1, 0, tw);
+    __syncthreads();
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) tl[j + TPX * mm] = v[mm];
+    __syncthreads();
+    if (valid) {
+      cpx<P> *oa = out + ((long)m * FO + 2 * q) * g.ysz + rr * g.xp;
+      cpx<P> *ob = out + ((long)m * FO + 2 * q + 1) * g.ysz + rr * g.xp;
+      for (int k = j; k < ix; k += TPX) {
+        cpx<P> zk = tl[k], zc = tl[(N - k) & (N - 1)];
+        const P h = (P)0.5;
+        cpx<P> A{h * (zk.x + zc.x), h * (zk.y - zc.y)};
+        cpx<P> B{h * (zk.y + zc.y), h * (zc.x - zk.x)};
+        oa[k] = A;
+        if (2 * q + 1 < FO) ob[k] = B;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- products
+// 3D rotational form: (U,V,W) <- -(u x w)  (3D_NavierStokes/src/kernels.h:281-306)
+// inputs: 0..2 vorticity, 3..5 velocity
+template <typename P> struct ProdNS {
+  __device__ __forceinline__ void operator()(const P *in, P *out) const {
+    const P tx = in[0], te = in[1], tz = in[2], tu = in[3], tv = in[4], tw = in[5];
+    const P nx = tv * tz - tw * te;
+    const P ny = tw * tx - tu * tz;
+    const P nz = tu * te - tv * tx;
+    out[0] = -nx; out[1] = -ny; out[2] = -nz;
+  }
+};
+
+// 2D: NZR = (v w, u w)  (2D_NavierStokes/src_RK4/kernels.h:187-201); inputs w,u,v
+template <typename P> struct Prod2D {
+  __device__ __forceinline__ void operator()(const P *in, P *out) const {
+    out[0] = in[2] * in[0];
+    out[1] = in[1] * in[0];
+  }
+};
+
+// ---------------------------------------------------------------- 2D passes
+// y-inverse straight from the state: Hermitian fill of kx=0 (in place in the
+// reference, 2D_NavierStokes/src_RK4/kernels.h:82-96), mask, Biot-Savart
+// (kernels.h:112-147), zero-pad, inverse y-FFT.  out fields w,u,v.
+template <typename P, int N, int T>
+__global__ void __launch_bounds__(T *(N / EPT))
+k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out, long out_ms, Grid g,
+        const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TP = N / EPT;
+  const int col = threadIdx.x % T, j = threadIdx.x / T;
+  const int ikx = blockIdx.x * T + col, m = blockIdx.z;
+  const bool valid = ikx < g.iktx;
+  const double kx = (double)ikx * g.alpha;
+  const cpx<double> *s0 = st + (long)m * st_ms;
+  cpx<double> w[EPT];
+  double kyv[EPT];
+#pragma unroll
+  for (int mm = 0; mm < EPT; mm++) {
+    const int y = j + TP * mm;
+    const int ky = wrapk(y, g.ny, g.kty);
+    cpx<double> a{0, 0};
+    kyv[mm] = ky == PSG_NOK ? 0.0 : (double)ky;
+    if (valid && ky != PSG_NOK) {
+      if (ikx == 0 && ky < 0) {
+        long idx = (long)(-ky + g.kty) * g.iktx;
+        if (mask[idx]) { a = s0[idx]; a.y = -a.y; }
+      } else {
+        long idx = (long)(ky + g.kty) * g.iktx + ikx;
+        if (mask[idx]) a = s0[idx];
+      }
+    }
+    w[mm] = a;
+  }
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    cpx<P> v[EPT];
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) {
+      if (f == 0) v[mm] = to<P>(w[mm]);
+      else {
+        const double ky = kyv[mm];
+        const double wk = kx * kx + ky * ky;
+        const double wki = 1.0 / (wk > 0.001 ? wk : 0.001);
+        cpx<double> r;
+        if (f == 1) r = cpx<double>{-ky * w[mm].y * wki, ky * w[mm].x * wki};
+        else r = cpx<double>{kx * w[mm].y * wki, -kx * w[mm].x * wki};
+        v[mm] = to<P>(r);
+      }
+    }
+    fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
+    if (valid) {
+      cpx<P> *o = out + (long)m * out_ms + (long)f * g.ysz + ikx;
+#pragma unroll
+      for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.xp] = v[mm];
+    }
+  }
+}
+
+// y-forward + update (2D).  UPD::operator()(m, ik, ikx, kx, ky, N[2]) with
+// N[0] = FFT(v w)/NR, N[1] = FFT(u w)/NR  (the reference's NZK halves).
+template <typename P, int N, int T, class UPD>
+__global__ void __launch_bounds__(T *(N / EPT))
+k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TP = N / EPT;
+  const int col = threadIdx.x % T, j = threadIdx.x / T;
+  const int ikx = blockIdx.x * T + col, m = blockIdx.z;
+  const bool valid = ikx < g.iktx;
+  cpx<double> res[2][EPT];
+#pragma unroll
+  for (int f = 0; f < 2; f++) {
+    const cpx<P> *src = in + (long)m * in_ms + (long)f * g.ysz + ikx;
+    cpx<P> v[EPT];
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[(long)(j + TP * mm) * g.xp] : cpx<P>{0, 0};
+    fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) {
+      cpx<double> t = up(v[mm]);
+      res[f][mm] = cpx<double>{t.x * g.inr, t.y * g.inr};
+    }
+  }
+  if (!valid) return;
+  const double kx = (double)ikx * g.alpha;
+#pragma unroll
+  for (int mm = 0; mm < EPT; mm++) {
+    const int ky = wrapk(j + TP * mm, g.ny, g.kty);
+    if (ky == PSG_NOK) continue;
+    const long ik = (long)(ky + g.kty) * g.iktx + ikx;
+    cpx<double> nn[2] = {res[0][mm], res[1][mm]};
+    upd(m, ik, ikx, kx, (double)ky, nn);
+  }
+}
+
+}  // namespace psg

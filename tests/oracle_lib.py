@@ -1,0 +1,259 @@
+"""ctypes wrapper around the FP64 CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.  The oracle restates the reference scheme
+(see oracle/oracle.h for the file:line map) and is the checker for the HIP
+product path, never part of it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j8"])
+        _lib = C.CDLL(LIB)
+        _declare(_lib)
+    return _lib
+
+
+class Grid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("alpha", C.c_double), ("ktx", C.c_int), ("kty", C.c_int), ("ktz", C.c_int),
+                ("iktx", C.c_int), ("ikty", C.c_int), ("iktz", C.c_int), ("nx2", C.c_int),
+                ("nkt", C.c_long), ("nr", C.c_long), ("n2", C.c_long)]
+
+
+P = C.c_void_p
+D = C.c_double
+I = C.c_int
+
+
+def _declare(L):
+    L.or_fft_c2c.argtypes = [P, I, I]
+    L.or_fft_r2c.argtypes = [P, P, I]
+    L.or_fft_c2r.argtypes = [P, P, I]
+    L.or_fft3_c2r.argtypes = [P, P, I, I, I]
+    L.or_fft3_r2c.argtypes = [P, P, I, I, I]
+    L.or_grid3.argtypes = [C.POINTER(Grid), I, I, I, D]
+    L.or_grid2.argtypes = [C.POINTER(Grid), I, I, D]
+    L.or_tables.argtypes = [C.POINTER(Grid), P, P, P, P]
+    L.or_count_active.argtypes = [C.POINTER(Grid)]
+    L.or_count_active.restype = C.c_long
+    L.or_nutab.argtypes = [C.POINTER(Grid), D, D, P, P, P]
+    L.or_ikf_ikn.argtypes = [C.POINTER(Grid), P, P]
+    L.or_rng_seed.argtypes = [I]
+    L.or_ranno0.restype = D
+    L.or_pm_next.argtypes = [I]
+    L.or_pm_next.restype = I
+    L.or_ic_ns3d_ref.argtypes = [C.POINTER(Grid), I, D, P, P, P]
+    L.or_ic_ns2d_ref.argtypes = [C.POINTER(Grid), I, D, P]
+    L.or_ic_solenoidal.argtypes = [C.POINTER(Grid), C.c_uint64, D, D, P, P, P, P]
+    L.or_ic_ns2d_cfg.argtypes = [C.POINTER(Grid), C.c_uint64, D, P]
+    L.or_gauss.argtypes = [C.c_uint64, C.c_uint64]
+    L.or_gauss.restype = D
+    L.or3_create.argtypes = [I, I, I, D, I, D, D, I, D, D]
+    L.or3_create.restype = P
+    L.or3_destroy.argtypes = [P]
+    L.or3_run.argtypes = [P, P, P, P, I]
+    L.or3_nonlin.argtypes = [P, P, P, P, P, P, P]
+    L.or3_physical.argtypes = [P, P, P, P, P]
+    L.or3_diag.argtypes = [P, P, P, P, P, P, P]
+    L.or2_create.argtypes = [I, I, D, D, D, I, D]
+    L.or2_create.restype = P
+    L.or2_destroy.argtypes = [P]
+    L.or2_run.argtypes = [P, P, I]
+    L.or2_nonlin.argtypes = [P, P, P]
+    L.or2_diag.argtypes = [P, P, P, P, P]
+    L.or_set_threads.argtypes = [I]
+    L.or_get_threads.restype = I
+    if hasattr(L, "orb_create"):
+        L.orb_create.argtypes = [I, I, I, D, D, D, D, D, D, D, D]
+        L.orb_create.restype = P
+        L.orb_destroy.argtypes = [P]
+        L.orb_run.argtypes = [P, P, P, P, P, I]
+        L.orb_rhs.argtypes = [P] + [P] * 8
+
+
+def ptr(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------- thin helpers ----------------
+def fft_c2c(x, sign):
+    x = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().or_fft_c2c(ptr(x), x.size, sign)
+    return x
+
+
+def fft_r2c(r):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(r.size // 2 + 1, np.complex128)
+    lib().or_fft_r2c(ptr(r), ptr(out), r.size)
+    return out
+
+
+def fft_c2r(X, n):
+    X = np.ascontiguousarray(X, dtype=np.complex128)
+    r = np.zeros(n, np.float64)
+    lib().or_fft_c2r(ptr(X), ptr(r), n)
+    return r
+
+
+def fft3_c2r(F, nx, ny, nz):
+    F = np.ascontiguousarray(F, dtype=np.complex128).copy()
+    r = np.zeros((nz, ny, nx), np.float64)
+    lib().or_fft3_c2r(ptr(F), ptr(r), nx, ny, nz)
+    return r
+
+
+def fft3_r2c(r):
+    nz, ny, nx = r.shape
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    F = np.zeros((nz, ny, nx // 2 + 1), np.complex128)
+    lib().or_fft3_r2c(ptr(r), ptr(F), nx, ny, nz)
+    return F
+
+
+def grid3(nx, ny, nz, alpha=1.0):
+    g = Grid()
+    lib().or_grid3(C.byref(g), nx, ny, nz, alpha)
+    return g
+
+
+def grid2(nx, ny, alpha=1.0):
+    g = Grid()
+    lib().or_grid2(C.byref(g), nx, ny, alpha)
+    return g
+
+
+def tables(g):
+    n = g.nkt
+    kx, ky, kz = (np.zeros(n) for _ in range(3))
+    L = np.zeros(n, np.int32)
+    lib().or_tables(C.byref(g), ptr(kx), ptr(ky), ptr(kz), ptr(L))
+    return kx, ky, kz, L
+
+
+def nutab(g, v2, dt):
+    n = g.nkt
+    nu, nuzn, nu1 = (np.zeros(n) for _ in range(3))
+    lib().or_nutab(C.byref(g), v2, dt, ptr(nu), ptr(nuzn), ptr(nu1))
+    return nu, nuzn, nu1
+
+
+def ikf_ikn(g):
+    ikf = np.zeros(g.n2, np.int32)
+    ikn = np.zeros(g.nkt, np.int32)
+    lib().or_ikf_ikn(C.byref(g), ptr(ikf), ptr(ikn))
+    return ikf, ikn
+
+
+def ic_ns3d_ref(g, iseed, ampv=1.0):
+    s = np.zeros((3, g.nkt), np.complex128)
+    lib().or_ic_ns3d_ref(C.byref(g), iseed, ampv, ptr(s[0]), ptr(s[1]), ptr(s[2]))
+    return s
+
+
+def ic_ns2d_ref(g, iseed, ampv=1.0):
+    z = np.zeros(g.nkt, np.complex128)
+    lib().or_ic_ns2d_ref(C.byref(g), iseed, ampv, ptr(z))
+    return z
+
+
+def ic_solenoidal(g, seed, energy=0.5, rho_scale=None):
+    s = np.zeros((4 if rho_scale is not None else 3, g.nkt), np.complex128)
+    rho = ptr(s[3]) if rho_scale is not None else None
+    lib().or_ic_solenoidal(C.byref(g), seed, energy, rho_scale or 0.0,
+                           ptr(s[0]), ptr(s[1]), ptr(s[2]), rho)
+    return s
+
+
+def ic_ns2d_cfg(g, seed, ampv=1.0):
+    z = np.zeros(g.nkt, np.complex128)
+    lib().or_ic_ns2d_cfg(C.byref(g), seed, ampv, ptr(z))
+    return z
+
+
+class NS3D:
+    """Oracle 3D stepper. flavour 0 = Navier-Stokes Heun+CN, 1 = Euler."""
+
+    def __init__(self, n, v2=0.01, dt=0.002, flavour=0, force=3, kf=4.0, ampfor=None,
+                 alpha=1.0, nx=None, ny=None, nz=None):
+        nx = nx or int(n / alpha)
+        ny = ny or n
+        nz = nz or n
+        if ampfor is None:
+            ampfor = {0: 0.0, 1: -0.25 * kf * kf, 2: -0.25 * kf, 3: -0.5 * kf}[force]
+        self.h = lib().or3_create(nx, ny, nz, alpha, flavour, v2, dt, force, kf, ampfor)
+        self.g = grid3(nx, ny, nz, alpha)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or3_destroy(self.h)
+            self.h = None
+
+    def run(self, state, nsteps):
+        s = np.ascontiguousarray(state, dtype=np.complex128).copy()
+        lib().or3_run(self.h, ptr(s[0]), ptr(s[1]), ptr(s[2]), nsteps)
+        return s
+
+    def nonlin(self, state):
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        out = np.zeros_like(s)
+        lib().or3_nonlin(self.h, ptr(s[0]), ptr(s[1]), ptr(s[2]), ptr(out[0]), ptr(out[1]), ptr(out[2]))
+        return out
+
+    def physical(self, state):
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        out = np.zeros((6, self.g.nz, self.g.ny, self.g.nx))
+        lib().or3_physical(self.h, ptr(s[0]), ptr(s[1]), ptr(s[2]), ptr(out))
+        return out
+
+    def diag(self, state):
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        e, z = C.c_double(), C.c_double()
+        spec = np.zeros(self.g.kty + 2)
+        lib().or3_diag(self.h, ptr(s[0]), ptr(s[1]), ptr(s[2]), C.byref(e), C.byref(z), ptr(spec))
+        return e.value, z.value, spec
+
+
+class NS2D:
+    def __init__(self, n, v2=1.0 / 40, dt=0.005, alpha=1.0, kf=4, famp=-2.0, nx=None):
+        nx = nx or int(n / alpha)
+        self.h = lib().or2_create(nx, n, alpha, v2, dt, kf, famp)
+        self.g = grid2(nx, n, alpha)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or2_destroy(self.h)
+            self.h = None
+
+    def run(self, z, nsteps):
+        z = np.ascontiguousarray(z, dtype=np.complex128).copy()
+        lib().or2_run(self.h, ptr(z), nsteps)
+        return z
+
+    def nonlin(self, z):
+        z = np.ascontiguousarray(z, dtype=np.complex128)
+        out = np.zeros_like(z)
+        lib().or2_nonlin(self.h, ptr(z), ptr(out))
+        return out
+
+    def diag(self, z):
+        z = np.ascontiguousarray(z, dtype=np.complex128)
+        e, v = C.c_double(), C.c_double()
+        spec = np.zeros(self.g.kty + 2)
+        lib().or2_diag(self.h, ptr(z), C.byref(e), C.byref(v), ptr(spec))
+        return e.value, v.value, spec

@@ -1,0 +1,162 @@
+"""GPU parity: the HIP path (through the C-ABI) against the FP64 CPU oracle on
+the same seeded inputs.  Tolerances: plain FP64 form <= 1e-12 relative for
+energy, enstrophy and shell spectra over 100 steps (the benchmark's stated
+bar), <= 1e-13 for single right-hand-side evaluations; FP32-bulk is checked
+against its own stated bound."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import psgpu_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def diag_rel(dg, do):
+    e = abs(dg[0] - do[0]) / abs(do[0])
+    z = abs(dg[1] - do[1]) / abs(do[1])
+    sp = rel(dg[2], do[2])
+    return e, z, sp
+
+
+def test_nonlin_3d(gpu):
+    n = 32
+    st = O.ic_solenoidal(O.grid3(n, n, n), 3)
+    with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov") as s:
+        got = s.nonlin(st)
+    want = O.NS3D(n, v2=0.01, dt=0.002, force=3).nonlin(st)
+    assert rel(got, want) < 1e-13
+
+
+def test_nonlin_3d_aspect(gpu):
+    # NX = NZ/alpha box as in 3D_Boussinesq (alpha = 0.5)
+    g = O.grid3(64, 32, 32, 0.5)
+    st = O.ic_solenoidal(g, 4)
+    with P.Stepper(flavour="ns3d", nx=64, ny=32, nz=32, alpha=0.5, force="dns") as s:
+        got = s.nonlin(st)
+    want = O.NS3D(32, alpha=0.5, force=1, nx=64, ny=32, nz=32).nonlin(st)
+    assert rel(got, want) < 1e-13
+
+
+def test_nonlin_2d(gpu):
+    g = O.grid2(64, 64)
+    z = O.ic_ns2d_cfg(g, 5)
+    with P.Stepper(flavour="ns2d", n=64, v2=1 / 40, dt=0.005, force="kolmogorov", kf=4.0, ampfor=-2.0) as s:
+        got = s.nonlin(z[None])[0]
+    want = O.NS2D(64).nonlin(z)
+    assert rel(got, want) < 1e-13
+
+
+def test_physical_fields_3d(gpu):
+    n = 32
+    st = O.ic_solenoidal(O.grid3(n, n, n), 8)
+    with P.Stepper(flavour="ns3d", n=n) as s:
+        s.set_state(st)
+        got = s.physical()
+    want = O.NS3D(n).physical(st)
+    assert rel(got, want) < 1e-13
+
+
+@pytest.mark.parametrize("force", ["kolmogorov", "dns", "gmres"])
+def test_ns3d_parity_100_steps(gpu, force):
+    n = 32
+    g = O.grid3(n, n, n)
+    st = O.ic_solenoidal(g, 21) if force == "kolmogorov" else O.ic_ns3d_ref(g, 837920, 1.0)
+    fmode = {"kolmogorov": 3, "dns": 1, "gmres": 2}[force]
+    orc = O.NS3D(n, v2=0.01, dt=0.002, force=fmode)
+    want = orc.run(st, 100)
+    with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force=force) as s:
+        s.set_state(st)
+        s.run(100)
+        got = s.get_state()
+        dg = s.diag()
+    e, z, sp = diag_rel(dg, orc.diag(want))
+    assert e < 1e-12 and z < 1e-12 and sp < 1e-12, (e, z, sp)
+    assert rel(got, want) < 1e-11
+
+
+def test_ns3d_parity_64(gpu):
+    n = 64
+    g = O.grid3(n, n, n)
+    st = O.ic_solenoidal(g, 22)
+    orc = O.NS3D(n, v2=0.01, dt=0.002, force=3)
+    with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002) as s:
+        s.set_state(st)
+        cur = st
+        for chunk in range(4):
+            s.run(25)
+            cur = orc.run(cur, 25)
+            e, z, sp = diag_rel(s.diag(), orc.diag(cur))
+            assert e < 1e-12 and z < 1e-12 and sp < 1e-12, (chunk, e, z, sp)
+
+
+def test_euler_parity(gpu):
+    n = 32
+    st = O.ic_solenoidal(O.grid3(n, n, n), 31)
+    orc = O.NS3D(n, dt=0.005, flavour=1, force=0)
+    want = orc.run(st, 50)
+    with P.Stepper(flavour="euler", n=n, dt=0.005, force="none") as s:
+        s.set_state(st)
+        s.run(50)
+        got = s.get_state()
+    assert rel(got, want) < 1e-11
+    e, z, sp = diag_rel((*orc.diag(got)[:2], orc.diag(got)[2]), orc.diag(want))
+    assert e < 1e-12 and z < 1e-12
+
+
+def test_ns2d_parity_256(gpu):
+    n = 256
+    g = O.grid2(n, n)
+    z0 = O.ic_ns2d_cfg(g, 837920, 1.0)
+    orc = O.NS2D(n, v2=1 / 40, dt=0.005)
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / 40, dt=0.005, force="kolmogorov", kf=4.0, ampfor=-2.0) as s:
+        s.set_state(z0[None])
+        cur = z0
+        for chunk in range(4):
+            s.run(25)
+            cur = orc.run(cur, 25)
+            e, zz, sp = diag_rel(s.diag(), orc.diag(cur))
+            assert e < 1e-12 and zz < 1e-12 and sp < 1e-12, (chunk, e, zz, sp)
+        assert rel(s.get_state()[0], cur) < 1e-11
+
+
+def test_ns2d_reference_ic(gpu):
+    n = 128
+    g = O.grid2(n, n)
+    z0 = O.ic_ns2d_ref(g, 837920, 1.0)
+    orc = O.NS2D(n, v2=1 / 40, dt=0.005)
+    want = orc.run(z0, 40)
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / 40, dt=0.005, force="kolmogorov", kf=4.0, ampfor=-2.0) as s:
+        s.set_state(z0[None])
+        s.run(40)
+        assert rel(s.get_state()[0], want) < 1e-11
+
+
+def test_batch_members_independent(gpu):
+    n = 32
+    g = O.grid3(n, n, n)
+    sts = [O.ic_solenoidal(g, 100 + m) for m in range(3)]
+    orc = O.NS3D(n, v2=0.01, dt=0.002, force=3)
+    with P.Stepper(flavour="ns3d", n=n, batch=3) as s:
+        for m in range(3):
+            s.set_state(sts[m], m)
+        s.run(20)
+        for m in range(3):
+            assert rel(s.get_state(m), orc.run(sts[m], 20)) < 1e-12
+
+
+def test_fp32_bulk_error_bound(gpu):
+    n = 64
+    st = O.ic_solenoidal(O.grid3(n, n, n), 23)
+    orc = O.NS3D(n, v2=0.01, dt=0.002, force=3)
+    want = orc.run(st, 100)
+    with P.Stepper(flavour="ns3d", n=n, precision="fp32") as s:
+        s.set_state(st)
+        s.run(100)
+        e = s.diag()[0]
+    ew = orc.diag(want)[0]
+    assert abs(e - ew) / ew < 1e-6

@@ -1,0 +1,230 @@
+"""CPU pins of the FP64 oracle: golden vectors, reference-held constants,
+closed forms.  Each check is chosen so a plausible mistake in the oracle
+(wrong sign, dropped term, transposed index) fails it."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_golden.npz"))
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 64, 128, 256, 512])
+def test_fft_golden(n):
+    x = GOLD["c_in_%d" % n]
+    tol = 1e-13 * np.abs(GOLD["c_fwd_%d" % n]).max()
+    assert np.abs(O.fft_c2c(x, -1) - GOLD["c_fwd_%d" % n]).max() < tol
+    assert np.abs(O.fft_c2c(x, +1) - GOLD["c_inv_%d" % n]).max() < tol
+    r = GOLD["r_in_%d" % n]
+    assert np.abs(O.fft_r2c(r) - GOLD["r_fwd_%d" % n]).max() < tol
+    # c2r of rfft(r) is n*r (cuFFT Z2D convention)
+    assert np.abs(O.fft_c2r(GOLD["r_fwd_%d" % n], n) - n * r).max() < 1e-12 * n
+
+
+@pytest.mark.parametrize("key", ["8x16x32", "16x16x16", "32x8x16"])
+def test_fft3_golden(key):
+    r = GOLD["r3_in_" + key]
+    F = GOLD["r3_fwd_" + key]
+    assert np.abs(O.fft3_r2c(r) - F).max() < 1e-12 * np.abs(F).max()
+    nz, ny, nx = r.shape
+    assert np.abs(O.fft3_c2r(F, nx, ny, nz) / r.size - r).max() < 1e-13
+
+
+def test_fft_closed_forms():
+    n = 64
+    d = np.zeros(n); d[0] = 1.0
+    assert np.allclose(O.fft_r2c(d), np.ones(n // 2 + 1))          # delta -> constant
+    j = np.arange(n)
+    c = np.cos(2 * np.pi * 5 * j / n)
+    X = O.fft_r2c(c)
+    assert abs(X[5] - n / 2) < 1e-12 and np.abs(np.delete(X, 5)).max() < 1e-12
+    x = np.random.default_rng(3).standard_normal(n)
+    X = O.fft_r2c(x)                                                 # Parseval
+    e = (abs(X[0]) ** 2 + abs(X[-1]) ** 2 + 2 * (np.abs(X[1:-1]) ** 2).sum()) / n
+    assert abs(e - (x ** 2).sum()) < 1e-12 * e
+
+
+def test_reference_held_sizes():
+    # 3D_NavierStokes/src_gmres/kernels.h:12  '#define nkt 310675' (128^3)
+    assert O.grid3(128, 128, 128).nkt == 310675
+    # 2D 256^2: SzV = 22428 = 2 * active (2D_NavierStokes/src_RK4/src_gmres/io_arpack.F90:6)
+    assert 2 * O.lib().or_count_active(O.grid2(256, 256)) == 22428
+    # Boussinesq 128x64x64, alpha=0.5: SzV = 288872 = 2*4*active (3D_Boussinesq/src/src_gmres/io_arpack.F90:6)
+    assert 8 * O.lib().or_count_active(O.grid3(128, 64, 64, 0.5)) == 288872
+    # 256^3 Zk.out header 1,86,171,171 (3D_Euler/bin/Zk.out)
+    g = O.grid3(256, 256, 256)
+    assert (g.iktx, g.ikty, g.iktz) == (86, 171, 171)
+
+
+def test_mask_bruteforce():
+    for (nx, n) in [(12, 12), (24, 24), (24, 12)]:
+        g = O.grid3(nx, n, n, n / nx)
+        kx, ky, kz, L = O.tables(g)
+        kt = n // 3
+        for i in range(g.nkt):
+            half = kx[i] > 0 or (kx[i] == 0 and (ky[i] > 0 or (ky[i] == 0 and kz[i] > 0)))
+            inside = np.sqrt(kx[i] ** 2 + ky[i] ** 2 + kz[i] ** 2) <= kt - 0.5
+            assert L[i] == int(half and inside)
+        # exactly one of k, -k is active for every k in the sphere (but 0)
+        act = {(kx[i], ky[i], kz[i]) for i in range(g.nkt) if L[i]}
+        for (a, b, c) in act:
+            assert (-a, -b, -c) not in act
+
+
+def test_ikf_ikn_inverse():
+    for g in (O.grid3(32, 32, 32), O.grid3(32, 16, 16, 0.5), O.grid2(32, 32)):
+        ikf, ikn = O.ikf_ikn(g)
+        assert (ikf[ikn] == np.arange(g.nkt)).all()
+        assert (ikf >= 0).sum() == g.nkt
+
+
+def test_park_miller_check_value():
+    # Park & Miller (1988): seed 1 -> x_10000 = 1043618065
+    s = 1
+    for _ in range(10000):
+        s = O.lib().or_pm_next(s)
+    assert s == 1043618065
+
+
+def test_ranno_reseeds_every_draw():
+    # ranno(0) draws with a freshly re-seeded ran1 (seed-1, seed-2, ...), so the
+    # n-th draw equals the first draw of a fresh sequence seeded with seed+n-1
+    O.lib().or_rng_seed(1000)
+    a = [O.lib().or_ranno0() for _ in range(3)]
+    O.lib().or_rng_seed(1001)
+    b = O.lib().or_ranno0()
+    assert a[1] == b
+    assert all(-np.pi <= v < np.pi for v in a)
+
+
+def test_nutab_formula():
+    g = O.grid3(16, 16, 16)
+    nu, nuzn, nu1 = O.nutab(g, 0.01, 0.002)
+    kx, ky, kz, L = O.tables(g)
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    on = L == 1
+    assert np.allclose(nu[on], 0.01 * k2[on], rtol=1e-14)
+    assert np.allclose(nu1[on], 0.002 / (1 + 0.001 * nu[on]), rtol=1e-15)
+    assert (nu[~on] == 0).all() and (nu1[~on] == 0).all()
+
+
+def test_ic_solenoidal():
+    g = O.grid3(32, 32, 32)
+    s = O.ic_solenoidal(g, 7)
+    kx, ky, kz, L = O.tables(g)
+    k = np.stack([kx, ky, kz])
+    assert np.abs((k * s).sum(0)).max() < 1e-14           # div w = 0
+    k2 = np.maximum((k ** 2).sum(0), 1e-3)
+    e = (np.abs(s) ** 2).sum(0) / k2
+    assert abs(e[L == 1].sum() - 0.5) < 1e-14              # sum |u|^2 = 0.5
+    assert np.abs(s[:, L == 0]).max() == 0.0
+
+
+def test_ic_reference_enstrophy():
+    g = O.grid3(32, 32, 32)
+    s = O.ic_ns3d_ref(g, 837920, 1.0)
+    assert abs(2 * (np.abs(s) ** 2).sum() - 1.0) < 1e-13
+    g2 = O.grid2(64, 64)
+    z = O.ic_ns2d_ref(g2, 837920, 1.0)
+    assert abs(2 * (np.abs(z) ** 2).sum() - 1.0) < 1e-13
+    kx, ky, _, L = O.tables(g2)
+    assert np.abs(z[kx == 0]).max() == 0.0                  # kx=0 modes zeroed
+
+
+def test_single_mode_cn_decay():
+    # a single Fourier mode has zero nonlinear term: one Heun+CN step is (1-a)/(1+a)
+    s = O.NS3D(16, v2=0.05, dt=0.01, force=0)
+    kx, ky, kz, L = O.tables(s.g)
+    st = np.zeros((3, s.g.nkt), complex)
+    i = np.where((kx == 1) & (ky == 2) & (kz == 0))[0][0]
+    k = np.array([1.0, 2.0, 0.0])
+    u = np.array([2.0, -1.0, 0.0]) * (0.3 + 0.2j)
+    st[:, i] = 1j * np.cross(k, u)
+    nu = 10 ** (np.log10(0.05) + 2 * np.log10(np.sqrt(5.0)))
+    a = 0.5 * nu * 0.01
+    out = s.run(st, 5)
+    assert np.abs(out[:, i] - st[:, i] * ((1 - a) / (1 + a)) ** 5).max() < 4e-15
+    # 2D RK4 + end-of-step CN
+    s2 = O.NS2D(16, v2=0.05, dt=0.01, famp=0.0)
+    kx, ky, _, L = O.tables(s2.g)
+    z = np.zeros(s2.g.nkt, complex)
+    i = np.where((kx == 2) & (ky == -1))[0][0]
+    z[i] = 0.7 - 0.1j
+    a = 0.5 * 0.01 * 0.05 * 5
+    assert abs(s2.run(z, 3)[i] - z[i] * ((1 - a) / (1 + a)) ** 3) < 4e-15
+
+
+def test_kolmogorov_laminar_fixed_point():
+    # u = (Re/KF^2) sin(KF y) x with f=(sin KF y,0,0): zeta(0,KF,0) = -Re/(2 KF)
+    for re_ in (10.0, 40.0):
+        s = O.NS3D(16, v2=1 / re_, dt=0.01, force=3, kf=4.0)
+        kx, ky, kz, L = O.tables(s.g)
+        st = np.zeros((3, s.g.nkt), complex)
+        st[2, np.where((kx == 0) & (ky == 4) & (kz == 0))[0][0]] = -re_ / 8
+        assert np.abs(s.run(st, 10) - st).max() < 1e-13 * re_
+    s2 = O.NS2D(16, v2=1 / 40.0, dt=0.01)
+    kx, ky, _, L = O.tables(s2.g)
+    z = np.zeros(s2.g.nkt, complex)
+    z[np.where((kx == 0) & (ky == 4))[0][0]] = -40.0 / 8   # subLam: +0.125/v2 removes it
+    assert np.abs(s2.run(z, 10) - z).max() < 1e-12
+
+
+def test_euler_energy_conservation_order():
+    # Galerkin-truncated Euler conserves energy in the dt -> 0 limit.  The
+    # reference predictor keeps the cube-truncated modes outside the 2/3 sphere
+    # (3D_Euler/src/kernels.h:339-345 has no mask on w*), which feeds an O(dt)
+    # term into the corrector: the drift over a fixed time is first order.
+    g = O.grid3(16, 16, 16)
+    st0 = O.ic_solenoidal(g, 11, 0.5)
+    drift = []
+    for dt in (0.01, 0.005, 0.0025):
+        s = O.NS3D(16, dt=dt, flavour=1, force=0)
+        out = s.run(st0, int(round(0.2 / dt)))
+        drift.append(abs(s.diag(out)[0] - 0.5))
+    assert drift[0] < 2e-3
+    assert 1.8 < drift[0] / drift[1] < 2.2 and 1.8 < drift[1] / drift[2] < 2.2
+
+
+def test_heun_order():
+    g = O.grid3(16, 16, 16)
+    st0 = O.ic_solenoidal(g, 5, 0.5)
+    T = 0.1
+    ref = O.NS3D(16, v2=0.02, dt=T / 64, force=3).run(st0, 64)
+    errs = []
+    for nst in (8, 16):
+        out = O.NS3D(16, v2=0.02, dt=T / nst, force=3).run(st0, nst)
+        errs.append(np.abs(out - ref).max())
+    assert 3.0 < errs[0] / errs[1] < 5.0
+
+
+def test_nonlinear_term_is_curl_of_product():
+    # independent check of the oracle RHS: compute curl(u x w) directly with numpy FFTs
+    n = 16
+    s = O.NS3D(n, force=0)
+    g = s.g
+    st = O.ic_solenoidal(g, 3, 0.5)
+    kx, ky, kz, L = O.tables(g)
+    # full spectral arrays for numpy (zero-padded, Hermitian)
+    F = np.zeros((3, n, n, n), complex)
+    for f in range(3):
+        for i in np.where(L == 1)[0]:
+            a, b, c = int(kx[i]), int(ky[i]), int(kz[i])
+            F[f, c % n, b % n, a % n] += st[f, i]
+            F[f, (-c) % n, (-b) % n, (-a) % n] += np.conj(st[f, i])
+    K = np.meshgrid(np.fft.fftfreq(n, 1 / n), np.fft.fftfreq(n, 1 / n), np.fft.fftfreq(n, 1 / n), indexing="ij")
+    kzz, kyy, kxx = K
+    kvec = np.stack([kxx, kyy, kzz])
+    k2 = np.maximum((kvec ** 2).sum(0), 1e-3)
+    U = 1j * np.cross(kvec, F, axis=0) / k2
+    w = np.fft.ifftn(F, axes=(1, 2, 3)).real * n ** 3
+    u = np.fft.ifftn(U, axes=(1, 2, 3)).real * n ** 3
+    P = np.fft.fftn(np.cross(u, w, axis=0), axes=(1, 2, 3)) / n ** 3
+    C = 1j * np.cross(kvec, P, axis=0)
+    want = np.zeros_like(st)
+    for i in np.where(L == 1)[0]:
+        a, b, c = int(kx[i]), int(ky[i]), int(kz[i])
+        want[:, i] = C[:, c % n, b % n, a % n]
+    got = s.nonlin(st)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
