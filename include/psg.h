@@ -45,7 +45,8 @@ enum psg_forcing {
   PSG_FORCE_DNS = 1,    /* xi.re +-A at (0,KF,+-KF)  3D_NavierStokes/src/kernels.h:342-343 */
   PSG_FORCE_GMRES = 2,  /* eta.im, zet.im at (0,KF,+-KF)  src_gmres/kernels.h:234-238 */
   PSG_FORCE_KOLMO = 3   /* f=(sin KF y,0,0): zet.re(0,KF,0) += A  3D_Boussinesq/src/kernels.h:382-384;
-                           2D: NZKX -= A at (0,+-KF)  2D_NavierStokes/src_RK4/kernels.h:264-266 */
+                           2D: NZKX += A at (0,+-KF), A=-2 is the reference's "NZKX -= 2"
+                           (2D_NavierStokes/src_RK4/kernels.h:264-266) */
 };
 
 typedef struct {
@@ -59,7 +60,8 @@ typedef struct {
   int force;          /* psg_forcing */
   double kf;          /* forcing wavenumber */
   double ampfor;      /* forcing amplitude A */
-  double ri, sc, theta; /* Boussinesq: Richardson number, Schmidt number, gravity tilt (deg) */
+  double ri, sc, theta; /* Boussinesq: Richardson number, Schmidt number, gravity tilt in radians
+                           (passed straight to sin/cos as 3D_Boussinesq/src/init.h:66-67 does) */
 } psg_config;
 
 typedef struct psg_ctx psg_ctx;
@@ -81,6 +83,10 @@ int psg_get_state_device(psg_ctx *ctx, int member, void *dev);
 int psg_run(psg_ctx *ctx, int nsteps);
 /* change dt (rebuilds the CN tables, as src_gmres/main.F90:759-768 does per period) */
 int psg_set_dt(psg_ctx *ctx, double dt);
+/* replace the CN tables (NUZN, NU1; nkt doubles each, 3D only) and the mask L
+ * (nkt ints) with caller-built ones, as the reference receives them from its
+ * Fortran driver (3D_NavierStokes/src/cuda.cu:24-25,39).  Any pointer may be NULL. */
+int psg_set_tables(psg_ctx *ctx, const double *nuzn, const double *nu1, const int *L);
 int psg_sync(psg_ctx *ctx);
 /* FP64 diagnostics of one member: energy = sum_L |w|^2/|k|^2, enstrophy = sum_L |w|^2
  * (2D: 2*sum_L|z|^2 as out.F90), spectrum[j] = sum over shell j=INT(|k|+0.5) of |w|^2/|k|^2
@@ -89,6 +95,11 @@ int psg_diag(psg_ctx *ctx, int member, double *energy, double *enstrophy, double
 /* spectral nonlinear term dz/dt = N(z)+F (no viscous term), masked, for one member,
  * written into out (nf*nkt complex); cf. TIMEDERIV src_gmres/timederiv.F90:2-44 */
 int psg_nonlin(psg_ctx *ctx, int member, const double *host_in, double *host_out);
+/* dz/dt = -NU z + N(z) + F, masked, for one member (host in/out, nf*nkt complex), with
+ * NU = 10^(log10 v2 + 2 log10|k|): TIMEDERIV (3D_NavierStokes/src_gmres/timederiv.F90:2-44,
+ * 2D_NavierStokes/src_RK4/src_gmres/timederiv.F90:2-32) and the Boussinesq GPU RHS kernel
+ * (3D_Boussinesq/src/kernels.h:624-692, timederiv_cuda_ src_gmres/cuda.cu:319-533). */
+int psg_timederiv(psg_ctx *ctx, int member, const double *host_in, double *host_out);
 /* physical-space fields of a member: 6 (3D: w then u) or 3 (2D: w,u,v) arrays of NR doubles,
  * x fastest, as KR_FFT_ALL produces (3D_NavierStokes/src/functions.h:26-75) */
 int psg_physical(psg_ctx *ctx, int member, double *host_out);

@@ -46,6 +46,22 @@ class HGrid(C.Structure):
                 ("nkt", C.c_long), ("nr", C.c_long), ("n2", C.c_long)]
 
 
+class NewtonOpts(C.Structure):
+    _fields_ = [("max_newton", C.c_int), ("max_gmres", C.c_int), ("max_hook", C.c_int), ("netol", C.c_double),
+                ("gmtol", C.c_double), ("eps", C.c_double), ("delta0", C.c_double), ("nshift", C.c_int),
+                ("fixed_period", C.c_int), ("jv", C.c_int), ("time_policy", C.c_int), ("dt0", C.c_double),
+                ("sy", C.c_double), ("stall_count", C.c_int), ("verbose", C.c_int)]
+
+
+class NewtonResult(C.Structure):
+    _fields_ = [("T", C.c_double), ("sx", C.c_double), ("sz", C.c_double), ("ner", C.c_double),
+                ("ner2", C.c_double), ("newton_iters", C.c_int), ("gmres_iters", C.c_int),
+                ("hook_iters", C.c_int), ("status", C.c_int), ("seconds", C.c_double), ("phi_evals", C.c_long)]
+
+
+NEWTON_STATUS = {0: "CONVERGED", 1: "GMRES FAILED", 2: "H-STEP FAILED", 3: "T-STEP FAILED", 4: "NEWTON STALLED",
+                 5: "PERIOD TOO SMALL", 6: "NEWTON FAILED"}
+
 _lib = None
 _P, _D, _I, _L = C.c_void_p, C.c_double, C.c_int, C.c_long
 
@@ -72,6 +88,8 @@ def lib():
     L.psg_diag.argtypes = [_P, _I, C.POINTER(_D), C.POINTER(_D), _P]
     L.psg_nonlin.argtypes = [_P, _I, _P, _P]
     L.psg_physical.argtypes = [_P, _I, _P]
+    L.psg_timederiv.argtypes = [_P, _I, _P, _P]
+    L.psg_set_tables.argtypes = [_P, _P, _P, _P]
     L.psg_stream.argtypes = [_P]
     L.psg_stream.restype = _P
     L.psg_profile_enable.argtypes = [_P, _I]
@@ -93,6 +111,14 @@ def lib():
     L.psgh_ic_ns2d_cfg.argtypes = [hg, C.c_uint64, _D, _P]
     L.psgh_fortran_f.argtypes = [C.c_char_p, _I, _I]
     L.psgh_fortran_f.restype = _D
+    L.psg_svd.argtypes = [_I, _I, _P, _P, _P, _P]
+    L.psg_hook_mu.argtypes = [_I, _P, _P, _D, _P]
+    L.psg_hook_mu.restype = _D
+    L.psg_newton_defaults.argtypes = [_I, C.POINTER(NewtonOpts)]
+    L.psg_newton_solve.argtypes = [_P, _P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(NewtonOpts),
+                                   C.POINTER(NewtonResult)]
+    L.psg_flow_map.argtypes = [_P, _P, _D, _D, _I]
+    L.psg_tangent.argtypes = [_P, _P, _P, _D, _D, _I, _P, _P]
     _lib = L
     return L
 
@@ -168,7 +194,7 @@ def ic_ns2d_cfg(g, seed, ampv=1.0):
 
 # ------------------------------------------------------------------ stepper
 def make_config(flavour="ns3d", n=64, nx=None, ny=None, nz=None, alpha=1.0, precision="fp64", batch=1,
-                v2=0.01, dt=0.002, force="kolmogorov", kf=4.0, ampfor=None, ri=0.0, sc=1.0, theta=90.0):
+                v2=0.01, dt=0.002, force="kolmogorov", kf=4.0, ampfor=None, ri=0.0, sc=1.0, theta=np.pi / 2):
     fl = _FLAV.get(flavour, flavour)
     fo = _FORCE.get(force, force)
     ny = ny or n
@@ -245,12 +271,48 @@ class Stepper:
         _check(lib().psg_nonlin(self.h, 0, _ptr(s), _ptr(out)), "psg_nonlin")
         return out
 
+    def timederiv(self, state):
+        """dz/dt = -NU z + N(z) + F (masked) of one state (host arrays)."""
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt)
+        out = np.zeros_like(s)
+        _check(lib().psg_timederiv(self.h, 0, _ptr(s), _ptr(out)), "psg_timederiv")
+        return out
+
     def physical(self, member=0):
         c = self.cfg
         nfo = 3 if c.flavour == NS2D else 6
         out = np.zeros((nfo, c.nz if c.flavour != NS2D else 1, c.ny, c.nx))
         _check(lib().psg_physical(self.h, member, _ptr(out)), "psg_physical")
         return out
+
+    def flow_map(self, state, T, dt0, policy=0):
+        """Phi_T(state) with the Newton time policy (0: dt=T/INT(T/dt0); 1: fixed dt0 + short last step)."""
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt).copy()
+        _check(lib().psg_flow_map(self.h, _ptr(s), T, dt0, policy), "psg_flow_map")
+        return s
+
+    def tangent(self, state, dstate, T, dt0, policy=0):
+        """(Phi_T(state), dPhi_T(state) dstate) from the tangent-linear stepper."""
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt)
+        d = np.ascontiguousarray(dstate, dtype=np.complex128).reshape(self.nf, self.nkt)
+        z1, dz1 = np.zeros_like(s), np.zeros_like(d)
+        _check(lib().psg_tangent(self.h, _ptr(s), _ptr(d), T, dt0, policy, _ptr(z1), _ptr(dz1)), "psg_tangent")
+        return z1, dz1
+
+    def newton(self, guess, T, sx=0.0, sz=0.0, **opts):
+        """Newton-GMRES-hookstep from (guess, T, sx[, sz]); returns (z, result dict)."""
+        o = NewtonOpts()
+        lib().psg_newton_defaults(self.cfg.flavour, C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        z = np.ascontiguousarray(guess, dtype=np.complex128).reshape(self.nf, self.nkt).copy()
+        Tc, sxc, szc = C.c_double(T), C.c_double(sx), C.c_double(sz)
+        r = NewtonResult()
+        _check(lib().psg_newton_solve(self.h, _ptr(z), C.byref(Tc), C.byref(sxc), C.byref(szc), C.byref(o),
+                                      C.byref(r)), "psg_newton_solve")
+        res = {f: getattr(r, f) for f, _ in NewtonResult._fields_}
+        res["status_name"] = NEWTON_STATUS.get(r.status, "?")
+        return z, res
 
     def profile(self, on=True):
         _check(lib().psg_profile_enable(self.h, 1 if on else 0), "psg_profile_enable")

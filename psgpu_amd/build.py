@@ -28,10 +28,12 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-
              "-I" + os.path.join(ROOT, "include")]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
-CORE_HIP = ["psg_api.hip", "psg_ns3d.hip", "psg_ns2d.hip"]
+CORE_HIP = ["psg_api.hip", "psg_ns3d.hip", "psg_ns2d.hip", "psg_bous.hip", "psg_newton.hip"]
 CORE_C = ["psg_host.c"]
 # legacy shims: (library suffix, source)
-LEGACY = []
+LEGACY = [("ns3d_dns", "legacy/legacy_ns3d_dns.c"), ("ns3d_gmres", "legacy/legacy_ns3d_gmres.c"),
+          ("euler", "legacy/legacy_euler.c"), ("ns2d_gmres", "legacy/legacy_ns2d_gmres.c"),
+          ("ns2d_dns", "legacy/legacy_ns2d_dns.c")]
 
 
 def _run(cmd, verbose):
@@ -73,8 +75,8 @@ def build(verbose=False, jobs=None):
     os.replace(tmp, lib)
     for suffix, src in LEGACY:
         out = os.path.join(OUT, "libpsg_legacy_%s.so" % suffix)
-        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out + ".tmp", objs[src]] + core + ["-lm"],
-             verbose)
+        _run(["gcc", "-shared", "-fPIC", "-o", out + ".tmp", objs[src], "-L" + OUT, "-lpsg",
+              "-Wl,-rpath,$ORIGIN", "-lm"], verbose)
         os.replace(out + ".tmp", out)
     return lib
 

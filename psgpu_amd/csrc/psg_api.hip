@@ -184,9 +184,11 @@ static int upload_tables(Ctx *c) {
   std::vector<int8_t> L8(g.nkt);
   for (long i = 0; i < g.nkt; i++) L8[i] = (int8_t)L[i];
   PSG_HIP(hipMemcpy(c->mask, L8.data(), g.nkt, hipMemcpyHostToDevice));
+  std::vector<double> nu(g.nkt), nuzn(g.nkt), nu1(g.nkt);
+  psgh_nutab(&hg, c->cfg.v2 > 0.0 ? c->cfg.v2 : 1e-300, c->cfg.dt, nu.data(), nuzn.data(), nu1.data());
+  if (c->cfg.v2 <= 0.0) std::fill(nu.begin(), nu.end(), 0.0);
+  PSG_HIP(hipMemcpy(c->nu, nu.data(), sizeof(double) * g.nkt, hipMemcpyHostToDevice));
   if (g.dim == 3) {
-    std::vector<double> nuzn(g.nkt), nu1(g.nkt);
-    psgh_nutab(&hg, c->cfg.v2, c->cfg.dt, nullptr, nuzn.data(), nu1.data());
     PSG_HIP(hipMemcpy(c->nuzn, nuzn.data(), sizeof(double) * g.nkt, hipMemcpyHostToDevice));
     PSG_HIP(hipMemcpy(c->nu1, nu1.data(), sizeof(double) * g.nkt, hipMemcpyHostToDevice));
   }
@@ -211,12 +213,87 @@ static int upload_twiddles(Ctx *c) {
 
 static void free_ctx(Ctx *c) {
   if (!c) return;
-  void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->nuzn, c->nu1, c->mask, c->bufA, c->bufB, c->bufC,
+  void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
+                  c->tl_rh, c->tl_bufA, c->tl_bufB, c->tl_bufC, c->bufA, c->bufB, c->bufC,
                   c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto e : c->evs) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
+}
+
+// out = L (-NU z + out): the viscous part of TIMEDERIV (3D_NavierStokes/src_gmres/timederiv.F90:31-42)
+__global__ void k_add_visc(const cpx<double> *__restrict__ z, cpx<double> *__restrict__ out,
+                           const double *__restrict__ nu, const int8_t *__restrict__ L, long nkt, int nf) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nkt * nf; i += (long)gridDim.x * blockDim.x) {
+    const long ik = i % nkt;
+    const cpx<double> a = z[i], b = out[i];
+    out[i] = L[ik] ? cpx<double>{-nu[ik] * a.x + b.x, -nu[ik] * a.y + b.y} : cpx<double>{0.0, 0.0};
+  }
+}
+
+int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out) {
+  if (c->cfg.flavour == PSG_BOUS3D) return bous_timederiv(c, in, out);
+  int r = c->g.dim == 2 ? ns2d_nonlin(c, 0, in, out) : ns3d_nonlin(c, 0, in, out);
+  if (r) return r;
+  const long tot = c->nf * c->g.nkt;
+  hipLaunchKernelGGL(k_add_visc, dim3((unsigned)std::min<long>(4096, (tot + 255) / 256)), dim3(256), 0, c->stream,
+                     in, out, c->nu, c->mask, c->g.nkt, c->nf);
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+int tl_alloc(Ctx *c) {
+  if (c->tl_state) return 0;
+  const Grid &g = c->g;
+  const size_t sb = sizeof(cpx<double>) * c->nf * g.nkt * c->batch;
+  const size_t b = c->cfg.precision == PSG_FP32BULK ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  PSG_HIP(hipMalloc(&c->tl_state, sb));
+  PSG_HIP(hipMalloc(&c->tl_stage, sb));
+  PSG_HIP(hipMalloc(&c->tl_rh, 3 * sb));
+  PSG_HIP(hipMemsetAsync(c->tl_state, 0, sb, c->stream));
+  PSG_HIP(hipMemsetAsync(c->tl_stage, 0, sb, c->stream));
+  PSG_HIP(hipMemsetAsync(c->tl_rh, 0, 3 * sb, c->stream));
+  if (g.dim == 2) {
+    PSG_HIP(hipMalloc(&c->tl_bufB, 6 * g.ysz * c->batch * b));
+    PSG_HIP(hipMalloc(&c->tl_bufC, 4 * g.ysz * c->batch * b));
+  } else {
+    PSG_HIP(hipMalloc(&c->tl_bufA, 12 * g.zsz * c->batch * b));
+    PSG_HIP(hipMalloc(&c->tl_bufB, 12 * g.ysz * c->batch * b));
+    PSG_HIP(hipMalloc(&c->tl_bufC, 6 * g.ysz * c->batch * b));
+  }
+  return 0;
+}
+
+// z1/dz1 = Phi_T(z0) and its derivative applied to dz0, with the Newton time policy
+int tl_flow_map(Ctx *c, const cpx<double> *z0, const cpx<double> *dz0, cpx<double> *z1, cpx<double> *dz1, double T,
+                double dt0, int policy, bool) {
+  if (int r = tl_alloc(c)) return r;
+  if (c->g.dim == 3 && c->cfg.flavour != PSG_NS3D) { set_error("tangent linear: 2D and 3D Navier-Stokes only"); return PSG_EINVAL; }
+  const size_t bytes = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  PSG_HIP(hipMemcpyAsync(c->state, z0, bytes, hipMemcpyDeviceToDevice, c->stream));
+  PSG_HIP(hipMemcpyAsync(c->tl_state, dz0, bytes, hipMemcpyDeviceToDevice, c->stream));
+  auto run = [&](int n) { return c->g.dim == 2 ? ns2d_tl_run(c, n) : ns3d_tl_run(c, n); };
+  int nstop = (int)(T / dt0), r = 0;
+  if (policy == 0) {
+    if (nstop < 1) nstop = 1;
+    const double dt = T / (double)nstop;
+    if (c->cfg.dt != dt && (r = psg_set_dt(static_cast<psg_ctx *>(c), dt))) return r;
+    if ((r = run(nstop))) return r;
+  } else {
+    if (nstop > 0) {
+      if (c->cfg.dt != dt0 && (r = psg_set_dt(static_cast<psg_ctx *>(c), dt0))) return r;
+      if ((r = run(nstop))) return r;
+    }
+    const double last = T - nstop * dt0;
+    if (last != 0.0) {
+      if ((r = psg_set_dt(static_cast<psg_ctx *>(c), last))) return r;
+      if ((r = run(1))) return r;
+    }
+  }
+  if (z1) PSG_HIP(hipMemcpyAsync(z1, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
+  PSG_HIP(hipMemcpyAsync(dz1, c->tl_state, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
 }
 
 }  // namespace psg
@@ -251,7 +328,8 @@ int psg_create(const psg_config *cfg, psg_ctx **out) {
   if (!cfg || !out) { set_error("null argument"); return PSG_EINVAL; }
   *out = nullptr;
   const bool is2d = cfg->flavour == PSG_NS2D;
-  if (cfg->flavour != PSG_NS2D && cfg->flavour != PSG_NS3D && cfg->flavour != PSG_EULER3D) {
+  if (cfg->flavour != PSG_NS2D && cfg->flavour != PSG_NS3D && cfg->flavour != PSG_EULER3D &&
+      cfg->flavour != PSG_BOUS3D) {
     set_error("flavour %d not available in this build", cfg->flavour);
     return PSG_EINVAL;
   }
@@ -290,13 +368,14 @@ int psg_create(const psg_config *cfg, psg_ctx **out) {
     if (hipMalloc(&c->state, ssz) != hipSuccess || hipMalloc(&c->tmp, sizeof(cpx<double>) * c->nf * g.nkt) != hipSuccess ||
         hipMalloc(&c->mask, g.nkt) != hipSuccess) { rc = PSG_ENOMEM; set_error("state allocation failed"); break; }
     hipMemset(c->state, 0, ssz);
+    if (hipMalloc(&c->nu, sizeof(double) * g.nkt) != hipSuccess) { rc = PSG_ENOMEM; set_error("tables"); break; }
     if (!is2d) {
       if (hipMalloc(&c->nuzn, sizeof(double) * g.nkt) != hipSuccess ||
           hipMalloc(&c->nu1, sizeof(double) * g.nkt) != hipSuccess) { rc = PSG_ENOMEM; set_error("tables"); break; }
     }
     if ((rc = upload_tables(c))) break;
     if ((rc = upload_twiddles(c))) break;
-    rc = is2d ? ns2d_init(c) : ns3d_init(c);
+    rc = is2d ? ns2d_init(c) : (cfg->flavour == PSG_BOUS3D ? bous_init(c) : ns3d_init(c));
     if (rc) break;
     if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = PSG_EHIP; set_error("init sync"); break; }
   } while (0);
@@ -358,7 +437,7 @@ int psg_get_state_device(psg_ctx *c, int m, void *dev) {
 int psg_run(psg_ctx *c, int nsteps) {
   if (!c || nsteps < 0) { set_error("invalid run arguments"); return PSG_EINVAL; }
   if (nsteps == 0) return 0;
-  int r = c->g.dim == 2 ? ns2d_run(c, nsteps) : ns3d_run(c, nsteps);
+  int r = c->g.dim == 2 ? ns2d_run(c, nsteps) : (c->cfg.flavour == PSG_BOUS3D ? bous_run(c, nsteps) : ns3d_run(c, nsteps));
   if (c->prof) prof_collect(c);
   return r;
 }
@@ -368,6 +447,21 @@ int psg_set_dt(psg_ctx *c, double dt) {
   PSG_HIP(hipStreamSynchronize(c->stream));
   c->cfg.dt = dt;
   return upload_tables(c);
+}
+
+int psg_set_tables(psg_ctx *c, const double *nuzn, const double *nu1, const int *L) {
+  if (!c) { set_error("null context"); return PSG_EINVAL; }
+  const long n = c->g.nkt;
+  if ((nuzn || nu1) && c->g.dim != 3) { set_error("CN tables are 3D-only"); return PSG_EINVAL; }
+  if (nuzn) PSG_HIP(hipMemcpyAsync(c->nuzn, nuzn, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (nu1) PSG_HIP(hipMemcpyAsync(c->nu1, nu1, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (L) {
+    std::vector<int8_t> l8(n);
+    for (long i = 0; i < n; i++) l8[i] = (int8_t)(L[i] != 0);
+    PSG_HIP(hipMemcpyAsync(c->mask, l8.data(), n, hipMemcpyHostToDevice, c->stream));
+    PSG_HIP(hipStreamSynchronize(c->stream));
+  }
+  return 0;
 }
 
 int psg_sync(psg_ctx *c) {
@@ -383,6 +477,7 @@ int psg_diag(psg_ctx *c, int m, double *e, double *z, double *spec) {
 
 int psg_nonlin(psg_ctx *c, int m, const double *host_in, double *host_out) {
   if (int r = check_member(c, m)) return r;
+  if (c->cfg.flavour == PSG_BOUS3D) { set_error("psg_nonlin: use psg_timederiv for the Boussinesq flavour"); return PSG_EINVAL; }
   const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
   cpx<double> *dout = nullptr;
   PSG_HIP(hipMalloc(&dout, n));
@@ -397,8 +492,25 @@ int psg_nonlin(psg_ctx *c, int m, const double *host_in, double *host_out) {
   return r;
 }
 
+int psg_timederiv(psg_ctx *c, int m, const double *host_in, double *host_out) {
+  if (int r = check_member(c, m)) return r;
+  const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  cpx<double> *dout = nullptr;
+  PSG_HIP(hipMalloc(&dout, n));
+  PSG_HIP(hipMemcpyAsync(c->tmp, host_in, n, hipMemcpyHostToDevice, c->stream));
+  int r = timederiv_dev(c, c->tmp, dout);
+  if (!r) {
+    if (hipMemcpyAsync(host_out, dout, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) { set_error("timederiv copy"); r = PSG_EHIP; }
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(dout);
+  return r;
+}
+
 int psg_physical(psg_ctx *c, int m, double *host_out) {
   if (int r = check_member(c, m)) return r;
+  if (c->cfg.flavour == PSG_BOUS3D) { set_error("psg_physical: not available for the Boussinesq flavour"); return PSG_EINVAL; }
   const long nr = (long)c->g.nx * c->g.ny * c->g.nz;
   const int nfo = c->g.dim == 2 ? 3 : 6;
   double *d = nullptr;

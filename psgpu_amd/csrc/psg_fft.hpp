@@ -79,7 +79,24 @@ template <typename T, bool INV, int R> __device__ __forceinline__ void dftR(cpx<
 }
 
 // One Stockham stage.  NS = product of the previous radices.
-template <typename T, int N, int E, int NS, bool INV>
+// Synchronisation/layout policy of a pencil team:
+//   WAVE: the team lives inside one wavefront, so LDS exchanges only need a
+//         wavefront-scope fence (no block barrier);
+//   PAD : LDS position p is stored at p + p/8, which spreads the radix-8
+//         Stockham writes (stride 8 complex) over all banks.  Used by row
+//         teams (stride 1); column tiles are conflict-free without it.
+template <bool WAVE> __device__ __forceinline__ void team_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+template <bool PAD> __device__ __forceinline__ int lpos(int p) { return PAD ? p + (p >> 3) : p; }
+
+template <typename T, int N, int E, int NS, bool INV, bool WAVE = false, bool PAD = false>
 __device__ __forceinline__ void fft_stage(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
                                           const cpx<T> *__restrict__ tw) {
   constexpr int TP = N / E;
@@ -89,9 +106,9 @@ __device__ __forceinline__ void fft_stage(cpx<T> (&v)[E], cpx<T> *lds, int j, in
   constexpr bool FIRST = (NS == 1);
   constexpr bool LAST = (NS * R == N);
   if constexpr (!FIRST) {
-    __syncthreads();
+    team_sync<WAVE>();
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = lds[(j + TP * m) * stride + off];
+    for (int m = 0; m < E; m++) v[m] = lds[lpos<PAD>(j + TP * m) * stride + off];
   }
 #pragma unroll
   for (int t = 0; t < G; t++) {
@@ -113,35 +130,38 @@ __device__ __forceinline__ void fft_stage(cpx<T> (&v)[E], cpx<T> *lds, int j, in
   }
   (void)NB;
   if constexpr (!LAST) {
-    __syncthreads();
+    team_sync<WAVE>();
 #pragma unroll
     for (int t = 0; t < G; t++) {
       const int b = j + t * TP;
       const int k = b % NS;
       const int d = (b / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) lds[(d + r * NS) * stride + off] = v[t + r * G];
+      for (int r = 0; r < R; r++) lds[lpos<PAD>(d + r * NS) * stride + off] = v[t + r * G];
     }
   }
 }
 
-template <typename T, int N, int E, int NS, bool INV>
+template <typename T, int N, int E, int NS, bool INV, bool WAVE, bool PAD>
 __device__ __forceinline__ void fft_stages(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
                                            const cpx<T> *__restrict__ tw) {
   if constexpr (NS < N) {
     constexpr int R = (N / NS >= E) ? E : N / NS;
-    fft_stage<T, N, E, NS, INV>(v, lds, j, stride, off, tw);
-    fft_stages<T, N, E, NS * R, INV>(v, lds, j, stride, off, tw);
+    fft_stage<T, N, E, NS, INV, WAVE, PAD>(v, lds, j, stride, off, tw);
+    fft_stages<T, N, E, NS * R, INV, WAVE, PAD>(v, lds, j, stride, off, tw);
   }
 }
 
 // Full N-point transform of the pencil held by a team (see header comment).
 // Every thread of the block must call this (it contains block barriers).
-template <typename T, int N, int E, bool INV>
+template <typename T, int N, int E, bool INV, bool WAVE = false, bool PAD = false>
 __device__ __forceinline__ void fft_pencil(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
                                            const cpx<T> *__restrict__ tw) {
   static_assert((N & (N - 1)) == 0 && N >= E, "power-of-two pencil");
-  fft_stages<T, N, E, 1, INV>(v, lds, j, stride, off, tw);
+  fft_stages<T, N, E, 1, INV, WAVE, PAD>(v, lds, j, stride, off, tw);
 }
+
+// LDS complex slots per padded row of N points
+template <int N> constexpr int padded_len() { return N + N / 8; }
 
 }  // namespace psg

@@ -46,6 +46,7 @@ struct Ctx {
   cpx<double> *stage = nullptr;     // RK4: Z0 stage input / generic scratch
   cpx<double> *tmp = nullptr;       // scratch state-sized buffer
   double *nuzn = nullptr, *nu1 = nullptr;  // CN tables (host-built, reference formula)
+  double *nu = nullptr;                    // NU = 10^(log10 v2 + 2 log10|k|), for dz/dt
   int8_t *mask = nullptr;           // active mask L
   // pass buffers (precision P)
   void *bufA = nullptr, *bufB = nullptr, *bufC = nullptr;
@@ -54,6 +55,11 @@ struct Ctx {
   cpx<double> *tw64[3] = {nullptr, nullptr, nullptr};
   cpx<float> *tw32[3] = {nullptr, nullptr, nullptr};
   double time = 0.0;
+  // tangent-linear companion state (allocated on first use): perturbation,
+  // its stage input and stage increments, and pass buffers sized for the
+  // combined base + perturbation transforms
+  cpx<double> *tl_state = nullptr, *tl_stage = nullptr, *tl_rh = nullptr;
+  void *tl_bufA = nullptr, *tl_bufB = nullptr, *tl_bufC = nullptr;
   // diagnostics scratch
   double *dred = nullptr;
   size_t dred_n = 0;
@@ -90,6 +96,13 @@ int ns2d_run(Ctx *c, int nsteps);
 int ns2d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
 int ns2d_physical(Ctx *c, int member, double *out_dev);
 int diag_member(Ctx *c, int member, double *e, double *z, double *spec);
+int bous_init(Ctx *c);
+int bous_run(Ctx *c, int nsteps);
+int bous_timederiv(Ctx *c, const cpx<double> *in_dev, cpx<double> *out_dev);
+int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out);
+int ns2d_tl_run(Ctx *c, int nsteps);
+int ns3d_tl_run(Ctx *c, int nsteps);
+int tl_alloc(Ctx *c);
 
 }  // namespace psg
 

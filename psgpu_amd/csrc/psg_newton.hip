@@ -1,0 +1,593 @@
+// Newton-GMRES-hookstep periodic-orbit solver on the persistent GPU context
+// (algorithm and citations in include/psg_newton.h).  Krylov vectors and all
+// state-sized algebra live in HBM as FP64; every inner product is the masked
+// real product sum_L Re(conj(a) b) reduced on the GPU (block partials summed on
+// the host in a fixed order, so results are run-to-run deterministic).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <chrono>
+#include <vector>
+#include "psg_launch.hpp"
+#include "../../include/psg_newton.h"
+
+namespace psg {
+
+int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out);
+int tl_flow_map(Ctx *c, const cpx<double> *z0, const cpx<double> *dz0, cpx<double> *z1, cpx<double> *dz1, double T,
+                double dt0, int policy, bool reuse_base);
+
+namespace {
+
+constexpr int RED_BLOCKS = 240;
+
+__global__ void k_dot(const cpx<double> *__restrict__ a, const cpx<double> *__restrict__ b,
+                      const int8_t *__restrict__ L, long nkt, long n, double *__restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    if (L[i % nkt]) s += a[i].x * b[i].x + a[i].y * b[i].y;
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// y = a*x + b*y  (masked: off-mask entries set to 0)
+__global__ void k_axpby(double a, const cpx<double> *__restrict__ x, double b, cpx<double> *__restrict__ y,
+                        const int8_t *__restrict__ L, long nkt, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    if (!L[i % nkt]) { y[i] = cpx<double>{0.0, 0.0}; continue; }
+    // b == 0 never reads y (it may hold uninitialised memory)
+    const cpx<double> yy = b == 0.0 ? cpx<double>{0.0, 0.0} : y[i];
+    const cpx<double> xx = a == 0.0 ? cpx<double>{0.0, 0.0} : x[i];
+    y[i] = cpx<double>{a * xx.x + b * yy.x, a * xx.y + b * yy.y};
+  }
+}
+
+// out = exp(-i(kx sx + ky sy + kz sz)) in, masked (the reference's ZNSHIFT loops)
+__global__ void k_shift(const cpx<double> *__restrict__ in, cpx<double> *__restrict__ out, Grid g, double sx,
+                        double sy, double sz, const int8_t *__restrict__ L, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long ik = i % g.nkt;
+    if (!L[ik]) { out[i] = cpx<double>{0.0, 0.0}; continue; }
+    const int ikx = (int)(ik % g.iktx);
+    const long t = ik / g.iktx;
+    const int iky = (int)(t % g.ikty), ikz = (int)(t / g.ikty);
+    const double kx = ikx * g.alpha, ky = iky - g.kty, kz = g.dim == 3 ? (double)(ikz - g.ktz) : 0.0;
+    double s, co;
+    sincos(-(kx * sx + ky * sy + kz * sz), &s, &co);
+    const cpx<double> a = in[i];
+    out[i] = cpx<double>{a.x * co - a.y * s, a.x * s + a.y * co};
+  }
+}
+
+// out = -i k_dir in, masked (dZ/dS)
+__global__ void k_ddir(const cpx<double> *__restrict__ in, cpx<double> *__restrict__ out, Grid g, int dir,
+                       const int8_t *__restrict__ L, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long ik = i % g.nkt;
+    const int ikx = (int)(ik % g.iktx);
+    const long t = ik / g.iktx;
+    const int iky = (int)(t % g.ikty), ikz = (int)(t / g.ikty);
+    const double k = dir == 0 ? ikx * g.alpha : (dir == 1 ? (double)(iky - g.kty) : (double)(ikz - g.ktz));
+    const cpx<double> a = in[i];
+    out[i] = L[ik] ? cpx<double>{k * a.y, -k * a.x} : cpx<double>{0.0, 0.0};
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ vector helpers
+struct VecOps {
+  Ctx *c;
+  long n;  // complex entries per vector (nf*nkt)
+  double *dpart = nullptr;
+  std::vector<double> hpart;
+  unsigned grid() const { return (unsigned)std::min<long>(4096, (n + 255) / 256); }
+  explicit VecOps(Ctx *cc) : c(cc), n((long)cc->nf * cc->g.nkt), hpart(RED_BLOCKS) {
+    if (hipMalloc(&dpart, sizeof(double) * RED_BLOCKS) != hipSuccess) dpart = nullptr;
+  }
+  ~VecOps() { if (dpart) hipFree(dpart); }
+  double dot(const cpx<double> *a, const cpx<double> *b) {
+    hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(256), 0, c->stream, a, b, c->mask, c->g.nkt, n, dpart);
+    hipMemcpyAsync(hpart.data(), dpart, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    double s = 0.0;
+    for (int i = 0; i < RED_BLOCKS; i++) s += hpart[i];
+    return s;
+  }
+  void axpby(double a, const cpx<double> *x, double b, cpx<double> *y) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid()), dim3(256), 0, c->stream, a, x, b, y, c->mask, c->g.nkt, n);
+  }
+  void copy(const cpx<double> *x, cpx<double> *y) { axpby(1.0, x, 0.0, y); }
+  void shift(const cpx<double> *in, cpx<double> *out, double sx, double sy, double sz) {
+    hipLaunchKernelGGL(k_shift, dim3(grid()), dim3(256), 0, c->stream, in, out, c->g, sx, sy, sz, c->mask, n);
+  }
+  void ddir(const cpx<double> *in, cpx<double> *out, int dir) {
+    hipLaunchKernelGGL(k_ddir, dim3(grid()), dim3(256), 0, c->stream, in, out, c->g, dir, c->mask, n);
+  }
+};
+
+// Phi_T on device buffers: out = flow map of in over time T
+static int flow(Ctx *c, const cpx<double> *in, cpx<double> *out, double T, double dt0, int policy) {
+  const size_t bytes = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  PSG_HIP(hipMemcpyAsync(c->state, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+  int nstop = (int)(T / dt0);
+  if (nstop < 1) nstop = 1;
+  int r = 0;
+  if (policy == PSG_TIME_UNIFORM) {
+    const double dt = T / (double)nstop;
+    if (c->cfg.dt != dt && (r = psg_set_dt(static_cast<psg_ctx *>(c), dt))) return r;
+    if ((r = psg_run(static_cast<psg_ctx *>(c), nstop))) return r;
+  } else {
+    nstop = (int)(T / dt0);
+    if (nstop > 0) {
+      if (c->cfg.dt != dt0 && (r = psg_set_dt(static_cast<psg_ctx *>(c), dt0))) return r;
+      if ((r = psg_run(static_cast<psg_ctx *>(c), nstop))) return r;
+    }
+    const double last = T - nstop * dt0;
+    if (last != 0.0) {
+      if ((r = psg_set_dt(static_cast<psg_ctx *>(c), last))) return r;
+      if ((r = psg_run(static_cast<psg_ctx *>(c), 1))) return r;
+    }
+  }
+  PSG_HIP(hipMemcpyAsync(out, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+// ------------------------------------------------------------------ dense algebra
+// One-sided Jacobi (Hestenes) SVD of column-major A (m x n, m >= n).
+static int jacobi_svd(int m, int n, const double *A, double *U, double *s, double *V) {
+  std::vector<double> W(A, A + (size_t)m * n);
+  for (int i = 0; i < n * n; i++) V[i] = 0.0;
+  for (int i = 0; i < n; i++) V[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 80; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double alpha = 0, beta = 0, gamma = 0;
+        for (int i = 0; i < m; i++) {
+          const double a = W[(size_t)p * m + i], b = W[(size_t)q * m + i];
+          alpha += a * a; beta += b * b; gamma += a * b;
+        }
+        if (gamma == 0.0) continue;
+        const double rel = fabs(gamma) / sqrt(alpha * beta);
+        if (!(rel > 1e-15)) continue;
+        off = fmax(off, rel);
+        const double zeta = (beta - alpha) / (2.0 * gamma);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+        for (int i = 0; i < m; i++) {
+          const double a = W[(size_t)p * m + i], b = W[(size_t)q * m + i];
+          W[(size_t)p * m + i] = cs * a - sn * b;
+          W[(size_t)q * m + i] = sn * a + cs * b;
+        }
+        for (int i = 0; i < n; i++) {
+          const double a = V[(size_t)p * n + i], b = V[(size_t)q * n + i];
+          V[(size_t)p * n + i] = cs * a - sn * b;
+          V[(size_t)q * n + i] = sn * a + cs * b;
+        }
+      }
+    if (off <= 1e-15) break;
+  }
+  // singular values = column norms, sorted descending
+  std::vector<int> ord(n);
+  std::vector<double> nrm(n);
+  for (int j = 0; j < n; j++) {
+    double t = 0;
+    for (int i = 0; i < m; i++) t += W[(size_t)j * m + i] * W[(size_t)j * m + i];
+    nrm[j] = sqrt(t);
+    ord[j] = j;
+  }
+  for (int a = 0; a < n; a++)
+    for (int b = a + 1; b < n; b++)
+      if (nrm[ord[b]] > nrm[ord[a]]) std::swap(ord[a], ord[b]);
+  std::vector<double> Vc(V, V + (size_t)n * n);
+  for (int k = 0; k < n; k++) {
+    const int j = ord[k];
+    s[k] = nrm[j];
+    for (int i = 0; i < m; i++) U[(size_t)k * m + i] = nrm[j] > 0 ? W[(size_t)j * m + i] / nrm[j] : 0.0;
+    for (int i = 0; i < n; i++) V[(size_t)k * n + i] = Vc[(size_t)j * n + i];
+  }
+  return 0;
+}
+
+// trust-region search of main.F90:1283-1333
+static double hook_mu(int k, const double *S, const double *P, double delta, double *X) {
+  const double r2 = sqrt(2.0);
+  auto xnorm = [&](double mu) {
+    double t = 0;
+    for (int i = 0; i < k; i++) {
+      X[i] = mu == 0.0 ? P[i] / S[i] : P[i] * S[i] / (mu + S[i] * S[i]);
+      t += X[i] * X[i];
+    }
+    return sqrt(t);
+  };
+  double normx = xnorm(0.0), mumid = 0.0;
+  if (normx > r2 * delta) {
+    double mu1 = 0.0, mu2 = 0.0;
+    for (int i = 0; i < k; i++) mu2 = fmax(mu2, S[i]);
+    normx = xnorm(mu2);
+    while (!(normx < delta)) {
+      mu1 = mu2;
+      mu2 = 2.0 * mu2;
+      normx = xnorm(mu2);
+    }
+    mumid = mu2;
+    for (int it = 0; it < 200 && !(normx < r2 * delta && normx > delta / r2); it++) {
+      mumid = 0.5 * (mu2 + mu1);
+      normx = xnorm(mumid);
+      if (normx > delta) mu1 = mumid;
+      else mu2 = mumid;
+    }
+  }
+  return mumid;
+}
+
+}  // namespace psg
+
+using namespace psg;
+
+extern "C" {
+
+int psg_svd(int m, int n, const double *A, double *U, double *s, double *V) {
+  if (m < n || n < 1) return PSG_EINVAL;
+  return jacobi_svd(m, n, A, U, s, V);
+}
+
+double psg_hook_mu(int k, const double *s, const double *p, double delta, double *x) {
+  return hook_mu(k, s, p, delta, x);
+}
+
+void psg_newton_defaults(int flavour, psg_newton_opts *o) {
+  memset(o, 0, sizeof(*o));
+  const bool is3 = flavour != PSG_NS2D;
+  o->max_newton = is3 ? 75 : 100;
+  o->max_gmres = 40;
+  o->max_hook = 50;
+  o->netol = is3 ? 1e-10 : 1e-12;
+  o->gmtol = flavour == PSG_BOUS3D ? 1e-7 : 1e-3;
+  o->eps = 1e-7;
+  o->delta0 = flavour == PSG_BOUS3D ? 0.05 : 0.1;
+  o->nshift = flavour == PSG_BOUS3D ? 2 : 1;
+  o->fixed_period = 0;
+  o->jv = 0;
+  o->time_policy = is3 ? PSG_TIME_LASTSTEP : PSG_TIME_UNIFORM;
+  o->dt0 = 0.005;
+  o->sy = 0.0;
+  o->stall_count = is3 ? 5 : 3;
+}
+
+int psg_flow_map(psg_ctx *c, double *z, double T, double dt0, int policy) {
+  const size_t bytes = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  cpx<double> *d = nullptr;
+  PSG_HIP(hipMalloc(&d, bytes));
+  PSG_HIP(hipMemcpyAsync(d, z, bytes, hipMemcpyHostToDevice, c->stream));
+  int r = flow(c, d, d, T, dt0, policy);
+  if (!r) {
+    PSG_HIP(hipMemcpyAsync(z, d, bytes, hipMemcpyDeviceToHost, c->stream));
+    PSG_HIP(hipStreamSynchronize(c->stream));
+  }
+  hipFree(d);
+  return r;
+}
+
+int psg_tangent(psg_ctx *c, const double *z, const double *dz0, double T, double dt0, int policy, double *z1,
+                double *dz1) {
+  if (!c || !z || !dz0 || !dz1) { set_error("null argument"); return PSG_EINVAL; }
+  const size_t bytes = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  cpx<double> *d = nullptr;
+  PSG_HIP(hipMalloc(&d, 4 * bytes));
+  cpx<double> *a = d, *b = d + c->nf * c->g.nkt, *o1 = b + c->nf * c->g.nkt, *o2 = o1 + c->nf * c->g.nkt;
+  int r = 0;
+  if (hipMemcpyAsync(a, z, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(b, dz0, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) { set_error("copy"); r = PSG_EHIP; }
+  if (!r) r = tl_flow_map(c, a, b, o1, o2, T, dt0, policy, false);
+  if (!r) {
+    if ((z1 && hipMemcpyAsync(z1, o1, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+        hipMemcpyAsync(dz1, o2, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) { set_error("copy"); r = PSG_EHIP; }
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(d);
+  return r;
+}
+
+int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szio, const psg_newton_opts *o,
+                     psg_newton_result *res) {
+  if (!c || !zh || !Tio || !o || !res) { set_error("null argument"); return PSG_EINVAL; }
+  if (c->batch != 1) { set_error("psg_newton_solve needs a batch-1 context"); return PSG_EINVAL; }
+  const auto t_start = std::chrono::steady_clock::now();
+  memset(res, 0, sizeof(*res));
+  VecOps V(c);
+  if (!V.dpart) { set_error("newton: reduction buffer"); return PSG_ENOMEM; }
+  const long n = V.n;
+  const size_t bytes = sizeof(cpx<double>) * n;
+  const int K = o->max_gmres;
+  const int ns = o->nshift;                    // shift unknowns (x[, z])
+  const int na = ns + (o->fixed_period ? 0 : 1);  // scalar unknowns
+  const int sdir[2] = {0, 2};
+  // device buffers
+  std::vector<cpx<double> *> bufs;
+  auto dalloc = [&](cpx<double> **p) -> bool {
+    if (hipMalloc(p, bytes) != hipSuccess) return false;
+    bufs.push_back(*p);
+    return true;
+  };
+  cpx<double> *ZNEWT, *ZN, *ZNN, *ZNSH, *FN, *dZNdt, *dzdt, *dZNdS[2], *dzdS[2], *v, *DIFF, *Zplus, *FNplus, *Zplus2,
+      *ZN2, *FNplus2, *work, *Q;
+  bool ok = dalloc(&ZNEWT) && dalloc(&ZN) && dalloc(&ZNN) && dalloc(&ZNSH) && dalloc(&FN) && dalloc(&dZNdt) &&
+            dalloc(&dzdt) && dalloc(&dZNdS[0]) && dalloc(&dZNdS[1]) && dalloc(&dzdS[0]) && dalloc(&dzdS[1]) &&
+            dalloc(&v) && dalloc(&DIFF) && dalloc(&Zplus) && dalloc(&FNplus) && dalloc(&Zplus2) && dalloc(&ZN2) &&
+            dalloc(&FNplus2) && dalloc(&work);
+  if (ok) {
+    if (hipMalloc(&Q, bytes * (size_t)(K + 1)) != hipSuccess) ok = false;
+    else bufs.push_back(Q);
+  }
+  auto cleanup = [&]() {
+    hipStreamSynchronize(c->stream);
+    for (auto p : bufs) hipFree(p);
+  };
+  if (!ok) { cleanup(); set_error("newton: device allocation"); return PSG_ENOMEM; }
+  auto q = [&](int i) { return Q + (size_t)i * n; };
+  std::vector<double> qN((size_t)(K + 1) * 3, 0.0);   // scalar parts: [T, sx, sz]
+  auto qn = [&](int i, int a) -> double & { return qN[(size_t)i * 3 + a]; };
+  std::vector<double> H((size_t)(K + 1) * K, 0.0);   // column-major (K+1) x K
+  auto h = [&](int i, int j) -> double & { return H[(size_t)j * (K + 1) + i]; };
+
+  double T = *Tio, sh[2] = {sxio ? *sxio : 0.0, szio ? *szio : 0.0};
+  const double sy = o->sy;
+  double dt0 = o->dt0;
+  const double dt0_orig = o->dt0;
+  long phi = 0;
+  int rc = 0;
+#define NTRY(x)              \
+  do {                       \
+    if ((rc = (x))) {        \
+      cleanup();             \
+      return rc;             \
+    }                        \
+  } while (0)
+  auto shift_of = [&](const cpx<double> *in, cpx<double> *out, double sx, double sz) {
+    V.shift(in, out, sx, sy, ns > 1 ? sz : 0.0);
+  };
+  auto norm = [&](const cpx<double> *a) { return sqrt(V.dot(a, a)); };
+
+  NTRY(hipMemcpyAsync(work, zh, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess ? 0 : PSG_EHIP);
+  V.copy(work, ZNEWT);
+  NTRY(flow(c, ZNEWT, ZN, T, dt0, o->time_policy));
+  phi++;
+  V.copy(ZN, ZNN);
+  shift_of(ZN, ZNSH, sh[0], sh[1]);
+  double NORMZ = norm(ZN);
+  V.copy(ZNSH, FN);
+  V.axpby(-1.0, ZNEWT, 1.0, FN);
+  double NORMFN = norm(FN);
+  double NER = NORMFN / NORMZ;
+  double Delta = o->delta0, NERold = 0.0;
+  int NERcount = 0;
+  int status = PSG_NEWTON_FAILED, INE = 0;
+  long gm_total = 0, hook_total = 0;
+  if (o->verbose) printf("newton: start T=%.10g sx=%.6g NER=%.3e\n", T, sh[0], NER);
+  if (NER <= o->netol) status = PSG_NEWTON_CONVERGED;
+  for (INE = 1; INE <= o->max_newton && status != PSG_NEWTON_CONVERGED; INE++) {
+    const double NORMb = NORMFN;
+    V.axpby(-1.0 / NORMb, FN, 0.0, q(0));
+    for (int a = 0; a < 3; a++) qn(0, a) = 0.0;
+    if (!o->fixed_period) {
+      NTRY(timederiv_dev(c, ZNSH, dZNdt));
+      NTRY(timederiv_dev(c, ZNEWT, dzdt));
+    }
+    for (int s = 0; s < ns; s++) {
+      V.ddir(ZNSH, dZNdS[s], sdir[s]);
+      V.ddir(ZNEWT, dzdS[s], sdir[s]);
+    }
+    std::fill(H.begin(), H.end(), 0.0);
+    int k = 0;
+    std::vector<double> U, S, VV, P, X, Y;
+    double GMR = 1.0;
+    // J applied to (dz, dT, ds): rows 1..N into out, scalar rows into vn
+    auto apply_J = [&](const cpx<double> *dz, const double *dsc, cpx<double> *out, double *vn) -> int {
+      int r;
+      if (o->jv == 1) {
+        if ((r = tl_flow_map(c, ZNEWT, dz, work, out, T, dt0, o->time_policy, true))) return r;
+        shift_of(out, out, sh[0], sh[1]);   // S dPhi dz
+      } else {
+        V.copy(ZNEWT, work);
+        V.axpby(o->eps, dz, 1.0, work);
+        if ((r = flow(c, work, out, T, dt0, o->time_policy))) return r;
+        V.axpby(-1.0, ZNN, 1.0, out);
+        shift_of(out, out, sh[0], sh[1]);
+        V.axpby(0.0, out, 1.0 / o->eps, out);
+      }
+      phi++;
+      V.axpby(-1.0, dz, 1.0, out);
+      int a = 0;
+      if (!o->fixed_period) { V.axpby(dsc[0], dZNdt, 1.0, out); vn[0] = V.dot(dzdt, dz); a = 1; }
+      else vn[0] = 0.0;
+      for (int s = 0; s < ns; s++) {
+        V.axpby(dsc[1 + s], dZNdS[s], 1.0, out);
+        vn[1 + s] = V.dot(dzdS[s], dz);
+      }
+      (void)a;
+      return 0;
+    };
+    for (k = 1; k <= K; k++) {
+      double vn[3] = {0, 0, 0};
+      double dsc[3] = {qn(k - 1, 0), qn(k - 1, 1), qn(k - 1, 2)};
+      NTRY(apply_J(q(k - 1), dsc, v, vn));
+      for (int i = 0; i < k; i++) {
+        double hij = V.dot(q(i), v);
+        if (!o->fixed_period) hij += qn(i, 0) * vn[0];
+        for (int s = 0; s < ns; s++) hij += qn(i, 1 + s) * vn[1 + s];
+        h(i, k - 1) = hij;
+        V.axpby(-hij, q(i), 1.0, v);
+        for (int a = 0; a < 3; a++) vn[a] -= hij * qn(i, a);
+      }
+      double hn = V.dot(v, v);
+      if (!o->fixed_period) hn += vn[0] * vn[0];
+      for (int s = 0; s < ns; s++) hn += vn[1 + s] * vn[1 + s];
+      hn = sqrt(hn);
+      h(k, k - 1) = hn;
+      V.axpby(1.0 / hn, v, 0.0, q(k));
+      for (int a = 0; a < 3; a++) qn(k, a) = vn[a] / hn;
+      // SVD of H(0:k, 0:k-1)
+      std::vector<double> Hk((size_t)(k + 1) * k);
+      for (int j = 0; j < k; j++)
+        for (int i = 0; i <= k; i++) Hk[(size_t)j * (k + 1) + i] = h(i, j);
+      U.assign((size_t)(k + 1) * k, 0.0);
+      S.assign(k, 0.0);
+      VV.assign((size_t)k * k, 0.0);
+      jacobi_svd(k + 1, k, Hk.data(), U.data(), S.data(), VV.data());
+      P.assign(k, 0.0);
+      X.assign(k, 0.0);
+      Y.assign(k, 0.0);
+      for (int i = 0; i < k; i++) P[i] = NORMb * U[(size_t)i * (k + 1)];
+      for (int i = 0; i < k; i++) X[i] = P[i] / S[i];
+      for (int i = 0; i < k; i++) {
+        double t = 0;
+        for (int j = 0; j < k; j++) t += VV[(size_t)j * k + i] * X[j];
+        Y[i] = t;
+      }
+      double r2 = 0;
+      for (int i = 0; i <= k; i++) {
+        double t = i == 0 ? -NORMb : 0.0;
+        for (int j = 0; j < k; j++) t += h(i, j) * Y[j];
+        r2 += t * t;
+      }
+      GMR = sqrt(r2) / NORMb;
+      if (o->verbose > 1) printf("  gmres %d  residual %.3e\n", k, GMR);
+      if (GMR <= o->gmtol) break;
+    }
+    if (k > K) k = K;   // Krylov cap reached: hookstep in the space built so far
+    gm_total += k;
+    if (GMR > o->gmtol && o->max_gmres >= 2000) { status = PSG_NEWTON_GMRES_FAILED; break; }
+    // ---------------- hookstep
+    int IHOOK = 0;
+    bool halve = false, dbl = false, hook_failed = false;
+    double NORMFNplus = 0, NORMFNpred = 0, NORMFNplus2 = 0, Tplus = T, Tplus2 = T, Splus[2] = {sh[0], sh[1]},
+           Splus2[2] = {sh[0], sh[1]};
+    for (;;) {
+      IHOOK++;
+      if (IHOOK > o->max_hook) { hook_failed = true; break; }
+      const double mu = hook_mu(k, S.data(), P.data(), Delta, X.data());
+      for (int i = 0; i < k; i++) {
+        double t = 0;
+        for (int j = 0; j < k; j++) t += VV[(size_t)j * k + i] * X[j];
+        Y[i] = t;
+      }
+      V.axpby(0.0, DIFF, 0.0, DIFF);
+      double dsc[3] = {0, 0, 0};
+      for (int i = 0; i < k; i++) {
+        V.axpby(Y[i], q(i), 1.0, DIFF);
+        for (int a = 0; a < 3; a++) dsc[a] += qn(i, a) * Y[i];
+      }
+      if (o->fixed_period) dsc[0] = 0.0;
+      // predicted residual F + J DIFF
+      double vn[3];
+      NTRY(apply_J(DIFF, dsc, v, vn));
+      V.axpby(1.0, FN, 1.0, v);
+      NORMFNpred = norm(v);
+      const double NORMbpredC = NORMFN;   // const = 0 in the reference (main.F90:135)
+      // evaluate the candidate
+      V.copy(ZNEWT, Zplus);
+      V.axpby(1.0, DIFF, 1.0, Zplus);
+      Tplus = T + dsc[0];
+      for (int s = 0; s < ns; s++) Splus[s] = sh[s] + dsc[1 + s];
+      NTRY(flow(c, Zplus, ZN, Tplus, dt0, o->time_policy));
+      phi++;
+      shift_of(ZN, ZNSH, Splus[0], Splus[1]);
+      V.copy(ZNSH, FNplus);
+      V.axpby(-1.0, Zplus, 1.0, FNplus);
+      NORMFNplus = norm(FNplus);
+      if (o->verbose > 1)
+        printf("  hook %d mu=%.3e Delta=%.3e |F+|=%.3e pred=%.3e |F|=%.3e\n", IHOOK, mu, Delta, NORMFNplus,
+               NORMFNpred, NORMFN);
+      if (NORMFNplus != NORMFNplus || NORMFNpred != NORMFNpred) {
+        Delta /= 2.0;
+        halve = true;
+        continue;
+      }
+      if (NORMFNplus >= NORMbpredC) {
+        if (dbl) {
+          V.copy(Zplus2, Zplus); Tplus = Tplus2; Splus[0] = Splus2[0]; Splus[1] = Splus2[1];
+          V.copy(ZN2, ZN); V.copy(FNplus2, FNplus); NORMFNplus = NORMFNplus2;
+          Delta /= 2.0;
+          break;
+        }
+        Delta /= 2.0;
+        halve = true;
+      } else {
+        if (halve) break;
+        if (dbl && NORMFNplus > NORMFNplus2) {
+          V.copy(Zplus2, Zplus); Tplus = Tplus2; Splus[0] = Splus2[0]; Splus[1] = Splus2[1];
+          V.copy(ZN2, ZN); V.copy(FNplus2, FNplus); NORMFNplus = NORMFNplus2;
+          Delta /= 2.0;
+          break;
+        }
+        if (mu == 0.0) break;
+        if (fabs(NORMFNpred - NORMFNplus) < 0.1 * fabs(NORMFNplus - NORMFN) || NORMFNplus < NORMFNpred) {
+          Delta *= 2.0;
+          dbl = true;
+          V.copy(Zplus, Zplus2); Tplus2 = Tplus; Splus2[0] = Splus[0]; Splus2[1] = Splus[1];
+          V.copy(ZN, ZN2); V.copy(FNplus, FNplus2); NORMFNplus2 = NORMFNplus;
+        } else {
+          break;
+        }
+      }
+    }
+    hook_total += IHOOK;
+    if (NORMFNpred != NORMFNpred || NORMFNplus != NORMFNplus) {
+      // time stepping blew up: halve dt and retry this Newton step (main.F90:1636-1665)
+      if (dt0 < 0.1 * dt0_orig) { status = PSG_NEWTON_TSTEP_FAILED; break; }
+      dt0 /= 2.0;
+      shift_of(ZNN, ZNSH, sh[0], sh[1]);
+      continue;
+    }
+    if (hook_failed) { status = PSG_NEWTON_HSTEP_FAILED; break; }
+    // accept
+    V.copy(ZN, ZNN);
+    V.copy(Zplus, ZNEWT);
+    T = Tplus;
+    sh[0] = Splus[0];
+    sh[1] = Splus[1];
+    shift_of(ZNN, ZNSH, sh[0], sh[1]);
+    NORMZ = norm(ZNN);
+    V.copy(FNplus, FN);
+    NORMFN = NORMFNplus;
+    NER = NORMFN / NORMZ;
+    if (o->verbose) printf("newton %d: NER=%.3e T=%.10g sx=%.6g gmres=%d hook=%d\n", INE, NER, T, sh[0], k, IHOOK);
+    const double relch = fabs((NER - NERold) / NER);
+    if (relch < 0.001) {
+      if (++NERcount > o->stall_count) { status = PSG_NEWTON_STALLED; break; }
+    } else {
+      NERcount = 0;
+    }
+    NERold = NER;
+    if (NER <= o->netol) { status = PSG_NEWTON_CONVERGED; break; }
+    if (T < 0.1) { status = PSG_NEWTON_PERIOD_SMALL; break; }
+  }
+  // low-wavenumber residual NER2 is not recomputed here (diagnostic only)
+  NTRY(hipMemcpyAsync(zh, ZNEWT, bytes, hipMemcpyDeviceToHost, c->stream) == hipSuccess ? 0 : PSG_EHIP);
+  hipStreamSynchronize(c->stream);
+  *Tio = T;
+  if (sxio) *sxio = sh[0];
+  if (szio) *szio = sh[1];
+  res->T = T; res->sx = sh[0]; res->sz = sh[1];
+  res->ner = NER;
+  res->ner2 = -1.0;
+  res->newton_iters = INE > o->max_newton ? o->max_newton : INE;
+  res->gmres_iters = (int)gm_total;
+  res->hook_iters = (int)hook_total;
+  res->status = status;
+  res->phi_evals = phi;
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  cleanup();
+#undef NTRY
+  return 0;
+}
+
+}  // extern "C"

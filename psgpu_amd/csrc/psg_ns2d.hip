@@ -86,7 +86,7 @@ template <typename P> static int launch_yinv2(Ctx *c, const cpx<double> *st, cpx
     auto k = k2_yinv<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, g.nkt, out, 3 * g.ysz, g,
-                       twp<P>(c, 1), c->mask);
+                       twp<P>(c, 1), c->mask, (const cpx<double> *)nullptr);
   });
   PSG_HIP(hipGetLastError());
   return 0;
@@ -113,7 +113,7 @@ static int launch_xpass2(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int
     constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)N * TE * sizeof(cpx<P>);
+    size_t lds = (size_t)padded_len<N>() * TE * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
@@ -235,6 +235,103 @@ template <typename P> static int physical2_t(Ctx *c, int member, double *out) {
 int ns2d_physical(Ctx *c, int member, double *out_dev) {
   return c->cfg.precision == PSG_FP32BULK ? physical2_t<float>(c, member, out_dev)
                                           : physical2_t<double>(c, member, out_dev);
+}
+
+// ------------------------------------------------------------------ tangent linear
+// Base and perturbation advance together through the same RK4/CN stages:
+// the perturbation stage increments use the linearised nonlinear term
+// dN = -i(kx FFT(du w + u dw) + ky FFT(dv w + v dw)) (no forcing) with the
+// same damping factors and masks as the base update.
+template <int STAGE> struct Upd2DTL {
+  Upd2D<STAGE> base;
+  cpx<double> *dz, *dz0, *dzs;
+  __device__ __forceinline__ void operator()(int m, long ik, int i0, double kx, double ky,
+                                             cpx<double> (&N)[4]) const {
+    cpx<double> nb[2] = {N[0], N[1]};
+    base(m, ik, i0, kx, ky, nb);
+    const long nkt = base.nkt;
+    cpx<double> *Z = dz + (long)m * nkt + ik;
+    if (base.L[ik] == 1) {
+      const double NX = kx * N[3].y + ky * N[2].y;
+      const double NY = -kx * N[3].x - ky * N[2].x;
+      const double dt = base.dt;
+      const double ZX = dt * NX, ZY = dt * NY;
+      cpx<double> *Z0 = dz0 + (long)m * nkt + ik;
+      cpx<double> *S = dzs + (long)m * 3 * nkt + ik;
+      const cpx<double> zn = *Z;
+      if constexpr (STAGE == 0 || STAGE == 1) {
+        S[STAGE * nkt] = cpx<double>{ZX, ZY};
+        *Z0 = cpx<double>{zn.x + 0.5 * ZX, zn.y + 0.5 * ZY};
+      } else if constexpr (STAGE == 2) {
+        S[2 * nkt] = cpx<double>{ZX, ZY};
+        *Z0 = cpx<double>{zn.x + ZX, zn.y + ZY};
+      } else {
+        const double wk = kx * kx + ky * ky;
+        const double damp1 = 1 - 0.5 * dt * base.v2 * wk;
+        const double damp2 = 1. / (1 + 0.5 * dt * base.v2 * wk);
+        const cpx<double> z1 = S[0], z2 = S[nkt], z3 = S[2 * nkt];
+        cpx<double> r;
+        r.x = damp2 * (damp1 * zn.x + (z1.x + 2 * z2.x + 2 * z3.x + ZX) / 6.);
+        r.y = damp2 * (damp1 * zn.y + (z1.y + 2 * z2.y + 2 * z3.y + ZY) / 6.);
+        *Z = r;
+        *Z0 = r;
+      }
+    } else {
+      *Z = cpx<double>{0.0, 0.0};
+    }
+  }
+};
+
+template <typename P, class UPD> static int rhs2_tl(Ctx *c, UPD upd, int nm) {
+  const Grid &g = c->g;
+  cpx<P> *B = (cpx<P> *)c->tl_bufB, *Cb = (cpx<P> *)c->tl_bufC;
+  PSG_DISPATCH_N(g.ny, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    auto k = k2_yinv<P, N, T, 2>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, c->stage, g.nkt, B, 6 * g.ysz, g,
+                       twp<P>(c, 1), c->mask, c->tl_stage);
+  });
+  PSG_HIP(hipGetLastError());
+  PSG_TRY2((launch_xpass2<P, 6, 4>(c, B, Cb, g.ny, nm, Prod2DTL<P>{})));
+  PSG_DISPATCH_N(g.ny, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    auto k = k2_yfwd<P, N, T, UPD, 4>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, Cb, 4 * g.ysz, g, twp<P>(c, 1), upd);
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename P> static int run2tl_t(Ctx *c, int nsteps) {
+  const Grid &g = c->g;
+  const long total = g.nkt * c->batch;
+  const unsigned gb = (unsigned)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+  hipLaunchKernelGGL(k2_setz0, dim3(gb), dim3(256), 0, c->stream, c->state, c->stage, c->mask, g.nkt, total);
+  hipLaunchKernelGGL(k2_setz0, dim3(gb), dim3(256), 0, c->stream, c->tl_state, c->tl_stage, c->mask, g.nkt, total);
+  PSG_HIP(hipGetLastError());
+  const double famp = c->cfg.force == PSG_FORCE_NONE ? 0.0 : c->cfg.ampfor;
+  for (int nt = 0; nt < nsteps; nt++) {
+#define PSG_TL_STAGE(S)                                                                                    \
+    {                                                                                                      \
+      Upd2DTL<S> u{{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf},     \
+                   c->tl_state, c->tl_stage, c->tl_rh};                                                    \
+      PSG_TRY2(rhs2_tl<P>(c, u, c->batch));                                                                 \
+    }
+    PSG_TL_STAGE(0) PSG_TL_STAGE(1) PSG_TL_STAGE(2) PSG_TL_STAGE(3)
+#undef PSG_TL_STAGE
+    c->time += c->cfg.dt;
+  }
+  return 0;
+}
+
+int ns2d_tl_run(Ctx *c, int nsteps) {
+  return c->cfg.precision == PSG_FP32BULK ? run2tl_t<float>(c, nsteps) : run2tl_t<double>(c, nsteps);
 }
 
 int ns2d_init(Ctx *c) {

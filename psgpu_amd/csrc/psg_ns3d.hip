@@ -138,7 +138,7 @@ static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, l
   return 0;
 }
 
-template <typename P> static int launch_yinv(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf) {
+template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
@@ -152,7 +152,7 @@ template <typename P> static int launch_yinv(Ctx *c, const cpx<P> *in, cpx<P> *o
   return 0;
 }
 
-template <typename P> static int launch_yfwd(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf) {
+template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
@@ -166,6 +166,11 @@ template <typename P> static int launch_yfwd(Ctx *c, const cpx<P> *in, cpx<P> *o
   return 0;
 }
 
+template int launch_yinv_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int);
+template int launch_yinv_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int);
+template int launch_yfwd_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int);
+template int launch_yfwd_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int);
+
 template <typename P, int FI, int FO, class PROD>
 static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
   const Grid &g = c->g;
@@ -173,7 +178,7 @@ static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int 
     constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)N * TE * sizeof(cpx<P>);
+    size_t lds = (size_t)padded_len<N>() * TE * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
@@ -226,13 +231,13 @@ static int rhs3(Ctx *c, const cpx<double> *st, long st_ms, int umask, UPD upd, l
   PSG_TRY(launch_zinv<P>(c, st, st_ms, A, 6 * g.zsz, true, umask, nm));
   prof_end(c);
   prof_begin(c, "yinv", pb[1] * nm);
-  PSG_TRY(launch_yinv<P>(c, A, B, 6 * nm));
+  PSG_TRY(launch_yinv_g<P>(c, A, B, 6 * nm));
   prof_end(c);
   prof_begin(c, "xpass", pb[2] * nm);
   PSG_TRY((launch_xpass<P, 6, 3>(c, B, Cb, (long)g.ny * g.nz, nm, ProdNS<P>{})));
   prof_end(c);
   prof_begin(c, "yfwd", pb[3] * nm);
-  PSG_TRY(launch_yfwd<P>(c, Cb, A, 3 * nm));
+  PSG_TRY(launch_yfwd_g<P>(c, Cb, A, 3 * nm));
   prof_end(c);
   prof_begin(c, "zfwd_update", (pb[4] + state_bytes) * nm);
   PSG_TRY((launch_zfwd<P, 3>(c, A, upd, nm)));
@@ -315,7 +320,7 @@ template <typename P> static int physical_t(Ctx *c, int member, double *out) {
   const Grid &g = c->g;
   cpx<P> *A = (cpx<P> *)c->bufA, *B = (cpx<P> *)c->bufB;
   PSG_TRY(launch_zinv<P>(c, c->state + (long)member * 3 * g.nkt, 3 * g.nkt, A, 6 * g.zsz, true, 1, 1));
-  PSG_TRY(launch_yinv<P>(c, A, B, 6));
+  PSG_TRY(launch_yinv_g<P>(c, A, B, 6));
   long nrow = (long)g.ny * g.nz;
   PSG_DISPATCH_N(g.nx, {
     constexpr int TE = Row<N>::TEAMS;
@@ -348,4 +353,13 @@ int ns3d_init(Ctx *c) {
   return 0;
 }
 
+}  // namespace psg
+
+namespace psg {
+// 3D tangent linear: provided by the tangent-linear/adjoint translation unit
+// when built; until then the Navier-Stokes 3D Newton solver uses finite differences.
+int ns3d_tl_run(Ctx *, int) {
+  set_error("3D tangent-linear stepping is not built");
+  return PSG_EINVAL;
+}
 }  // namespace psg

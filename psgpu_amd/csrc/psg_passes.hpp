@@ -51,6 +51,31 @@ __device__ __forceinline__ void biot_savart(double kx, double ky, double kz, cpx
 // ---------------------------------------------------------------- pass 1 (3D)
 // out fields: 0..2 vorticity, 3..5 velocity (if BS).  umask: velocity masked
 // by L as setVelocity does before the first step (kernels.h:191).
+// Each output field re-reads the state components it needs (one for a
+// vorticity component, two for a velocity component); the block's state tile
+// stays cache-resident, so HBM sees each state value once while the kernel
+// keeps only one pencil's worth of registers live.
+struct FillIdx {
+  long idx;     // stored index to read (mirror for the filled half-plane), -1 = zero
+  bool conj;    // conjugate on read
+};
+__device__ __forceinline__ FillIdx fill_index(const Grid &g, int ikx, int iky, int ky, int kz) {
+  FillIdx r{-1, false};
+  if (kz == PSG_NOK) return r;
+  if (ikx == 0 && ky == 0 && kz == 0) return r;
+  const bool mirror = (ikx == 0) && (ky < 0 || (ky == 0 && kz < 0));
+  r.idx = mirror ? ((long)(-kz + g.ktz) * g.ikty + (-ky + g.kty)) * g.iktx
+                 : ((long)(kz + g.ktz) * g.ikty + iky) * g.iktx + ikx;
+  r.conj = mirror;
+  return r;
+}
+__device__ __forceinline__ cpx<double> ld_fill(const cpx<double> *__restrict__ f, const FillIdx &q) {
+  if (q.idx < 0) return cpx<double>{0.0, 0.0};
+  cpx<double> a = f[q.idx];
+  if (q.conj) a.y = -a.y;
+  return a;
+}
+
 template <typename P, int N, int T, bool BS>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
@@ -64,26 +89,17 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
   const int ky = iky - g.kty;
   const double kx = (double)ikx * g.alpha, kyd = (double)ky;
   const cpx<double> *s0 = st + (long)m * st_ms;
+  // every state value of the pencil is requested before the first transform
   cpx<double> w[3][EPT];
   double lm[EPT];
 #pragma unroll
   for (int mm = 0; mm < EPT; mm++) {
-    const int z = j + TP * mm;
-    const int kz = wrapk(z, g.nz, g.ktz);
-    cpx<double> a{0, 0}, b{0, 0}, c{0, 0};
-    double l = 0.0;
-    if (valid && kz != PSG_NOK) {
-      const bool zero = (ikx == 0 && ky == 0 && kz == 0);
-      const bool mirror = (ikx == 0) && (ky < 0 || (ky == 0 && kz < 0));
-      if (!zero) {
-        long idx = mirror ? ((long)(-kz + g.ktz) * g.ikty + (-ky + g.kty)) * g.iktx
-                          : ((long)(kz + g.ktz) * g.ikty + iky) * g.iktx + ikx;
-        a = s0[idx]; b = s0[idx + st_fs]; c = s0[idx + 2 * st_fs];
-        if (mirror) { a.y = -a.y; b.y = -b.y; c.y = -c.y; }
-        l = umask ? (double)mask[idx] : 1.0;
-      }
-    }
-    w[0][mm] = a; w[1][mm] = b; w[2][mm] = c; lm[mm] = l;
+    const int kz = valid ? wrapk(j + TP * mm, g.nz, g.ktz) : PSG_NOK;
+    const FillIdx q = fill_index(g, ikx, iky, ky, kz);
+    w[0][mm] = ld_fill(s0, q);
+    w[1][mm] = ld_fill(s0 + st_fs, q);
+    w[2][mm] = ld_fill(s0 + 2 * st_fs, q);
+    lm[mm] = (umask && q.idx >= 0) ? (double)mask[q.idx] : 1.0;
   }
   constexpr int NFO = BS ? 6 : 3;
 #pragma unroll
@@ -94,12 +110,10 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
       if (f < 3) {
         v[mm] = to<P>(w[f][mm]);
       } else {
-        const int z = j + TP * mm;
-        const int kz = wrapk(z, g.nz, g.ktz);
+        const int kz = wrapk(j + TP * mm, g.nz, g.ktz);
         const double kzd = kz == PSG_NOK ? 0.0 : (double)kz;
         const double wk = kx * kx + kyd * kyd + kzd * kzd;
-        double wki = 1.0 / (wk > 0.001 ? wk : 0.001);
-        wki *= lm[mm];
+        const double wki = lm[mm] / (wk > 0.001 ? wk : 0.001);
         cpx<double> u, vv, ww;
         biot_savart(kx, kyd, kzd, w[0][mm], w[1][mm], w[2][mm], wki, u, vv, ww);
         v[mm] = to<P>(f == 3 ? u : (f == 4 ? vv : ww));
@@ -177,13 +191,19 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
   const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
   const bool valid = ikx < g.iktx;
   cpx<double> res[FO][EPT];
+  cpx<P> vin[FO][EPT];
 #pragma unroll
   for (int f = 0; f < FO; f++) {
     const cpx<P> *src = in + ((long)m * FO + f) * g.zsz + (long)iky * g.xp + ikx;
-    cpx<P> v[EPT];
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++)
-      v[mm] = valid ? src[(long)(j + TP * mm) * g.ikty * g.xp] : cpx<P>{0, 0};
+      vin[f][mm] = valid ? src[(long)(j + TP * mm) * g.ikty * g.xp] : cpx<P>{0, 0};
+  }
+#pragma unroll
+  for (int f = 0; f < FO; f++) {
+    cpx<P> v[EPT];
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) v[mm] = vin[f][mm];
     fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
@@ -208,6 +228,8 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
 // ---------------------------------------------------------------- pass 3 (x)
 // Row teams: TEAMS rows per block, N/8 threads per row.  Fields are paired
 // into complex transforms z = a + i b.  PROD::apply(double in[FI], double out[FO]).
+// A team that fits in one wavefront synchronises with wavefront fences only;
+// its LDS row is padded (p -> p + p/8) so radix-8 writes are bank-conflict free.
 template <typename P, int N, int TEAMS, int FI, int FO, class PROD>
 __global__ void __launch_bounds__(TEAMS *(N / EPT))
 k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
@@ -215,20 +237,21 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   extern __shared__ char smem_raw[];
   cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
   constexpr int TPX = N / EPT;
+  constexpr bool WAVE = TPX <= 64;
   constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2;
   const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
   const long row = (long)blockIdx.x * TEAMS + team;
   const bool valid = row < nrows;
   const long m = valid ? row / nrow_m : 0, rr = valid ? row % nrow_m : 0;
-  cpx<P> *tl = lds + team * N;
+  cpx<P> *tl = lds + team * padded_len<N>();
   const int ix = g.iktx;
-  P phys[2 * PI][EPT];
+  // every input of the row is requested before the first transform
+  cpx<P> vin[PI][EPT];
 #pragma unroll
   for (int p = 0; p < PI; p++) {
     const int fa = 2 * p, fb = 2 * p + 1;
     const cpx<P> *ra = in + ((long)m * FI + fa) * g.ysz + rr * g.xp;
     const cpx<P> *rb = in + ((long)m * FI + fb) * g.ysz + rr * g.xp;
-    cpx<P> v[EPT];
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
       const int k = j + TPX * mm;
@@ -246,9 +269,17 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
           z = cpx<P>{a.x + b.y, b.x - a.y};
         }
       }
-      v[mm] = z;
+      vin[p][mm] = z;
     }
-    fft_pencil<P, N, EPT, true>(v, tl, j, 1, 0, tw);
+  }
+  P phys[2 * PI][EPT];
+#pragma unroll
+  for (int p = 0; p < PI; p++) {
+    const int fa = 2 * p, fb = 2 * p + 1;
+    cpx<P> v[EPT];
+#pragma unroll
+    for (int mm = 0; mm < EPT; mm++) v[mm] = vin[p][mm];
+    fft_pencil<P, N, EPT, true, WAVE, true>(v, tl, j, 1, 0, tw);
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) { phys[fa][mm] = v[mm].x; phys[fb][mm] = v[mm].y; }
   }
@@ -269,16 +300,16 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
     cpx<P> v[EPT];
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) v[mm] = cpx<P>{res[2 * q][mm], res[2 * q + 1][mm]};
-    fft_pencil<P, N, EPT, false>(v, tl, j, 1, 0, tw);
-    __syncthreads();
+    fft_pencil<P, N, EPT, false, WAVE, true>(v, tl, j, 1, 0, tw);
+    team_sync<WAVE>();
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) tl[j + TPX * mm] = v[mm];
-    __syncthreads();
+    for (int mm = 0; mm < EPT; mm++) tl[lpos<true>(j + TPX * mm)] = v[mm];
+    team_sync<WAVE>();
     if (valid) {
       cpx<P> *oa = out + ((long)m * FO + 2 * q) * g.ysz + rr * g.xp;
       cpx<P> *ob = out + ((long)m * FO + 2 * q + 1) * g.ysz + rr * g.xp;
       for (int k = j; k < ix; k += TPX) {
-        cpx<P> zk = tl[k], zc = tl[(N - k) & (N - 1)];
+        cpx<P> zk = tl[lpos<true>(k)], zc = tl[lpos<true>((N - k) & (N - 1))];
         const P h = (P)0.5;
         cpx<P> A{h * (zk.x + zc.x), h * (zk.y - zc.y)};
         cpx<P> B{h * (zk.y + zc.y), h * (zc.x - zk.x)};
@@ -302,6 +333,17 @@ template <typename P> struct ProdNS {
   }
 };
 
+// 2D tangent linear: base (v w, u w) and their linearisation
+// (dv w + v dw, du w + u dw); inputs w,u,v,dw,du,dv
+template <typename P> struct Prod2DTL {
+  __device__ __forceinline__ void operator()(const P *in, P *out) const {
+    out[0] = in[2] * in[0];
+    out[1] = in[1] * in[0];
+    out[2] = in[5] * in[0] + in[2] * in[3];
+    out[3] = in[4] * in[0] + in[1] * in[3];
+  }
+};
+
 // 2D: NZR = (v w, u w)  (2D_NavierStokes/src_RK4/kernels.h:187-201); inputs w,u,v
 template <typename P> struct Prod2D {
   __device__ __forceinline__ void operator()(const P *in, P *out) const {
@@ -314,10 +356,11 @@ template <typename P> struct Prod2D {
 // y-inverse straight from the state: Hermitian fill of kx=0 (in place in the
 // reference, 2D_NavierStokes/src_RK4/kernels.h:82-96), mask, Biot-Savart
 // (kernels.h:112-147), zero-pad, inverse y-FFT.  out fields w,u,v.
-template <typename P, int N, int T>
+template <typename P, int N, int T, int NSRC = 1>
 __global__ void __launch_bounds__(T *(N / EPT))
 k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out, long out_ms, Grid g,
-        const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask) {
+        const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask,
+        const cpx<double> *__restrict__ st2 = nullptr) {
   extern __shared__ char smem_raw[];
   cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
   constexpr int TP = N / EPT;
@@ -325,54 +368,57 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
   const int ikx = blockIdx.x * T + col, m = blockIdx.z;
   const bool valid = ikx < g.iktx;
   const double kx = (double)ikx * g.alpha;
-  const cpx<double> *s0 = st + (long)m * st_ms;
-  cpx<double> w[EPT];
-  double kyv[EPT];
-#pragma unroll
-  for (int mm = 0; mm < EPT; mm++) {
-    const int y = j + TP * mm;
-    const int ky = wrapk(y, g.ny, g.kty);
-    cpx<double> a{0, 0};
-    kyv[mm] = ky == PSG_NOK ? 0.0 : (double)ky;
-    if (valid && ky != PSG_NOK) {
-      if (ikx == 0 && ky < 0) {
-        long idx = (long)(-ky + g.kty) * g.iktx;
-        if (mask[idx]) { a = s0[idx]; a.y = -a.y; }
-      } else {
-        long idx = (long)(ky + g.kty) * g.iktx + ikx;
-        if (mask[idx]) a = s0[idx];
-      }
-    }
-    w[mm] = a;
-  }
-#pragma unroll
-  for (int f = 0; f < 3; f++) {
-    cpx<P> v[EPT];
+#pragma unroll 1
+  for (int src = 0; src < NSRC; src++) {
+    const cpx<double> *s0 = (src == 0 ? st : st2) + (long)m * st_ms;
+    cpx<double> w[EPT];
+    double kyv[EPT];
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
-      if (f == 0) v[mm] = to<P>(w[mm]);
-      else {
-        const double ky = kyv[mm];
-        const double wk = kx * kx + ky * ky;
-        const double wki = 1.0 / (wk > 0.001 ? wk : 0.001);
-        cpx<double> r;
-        if (f == 1) r = cpx<double>{-ky * w[mm].y * wki, ky * w[mm].x * wki};
-        else r = cpx<double>{kx * w[mm].y * wki, -kx * w[mm].x * wki};
-        v[mm] = to<P>(r);
+      const int y = j + TP * mm;
+      const int ky = wrapk(y, g.ny, g.kty);
+      cpx<double> a{0, 0};
+      kyv[mm] = ky == PSG_NOK ? 0.0 : (double)ky;
+      if (valid && ky != PSG_NOK) {
+        if (ikx == 0 && ky < 0) {
+          long idx = (long)(-ky + g.kty) * g.iktx;
+          if (mask[idx]) { a = s0[idx]; a.y = -a.y; }
+        } else {
+          long idx = (long)(ky + g.kty) * g.iktx + ikx;
+          if (mask[idx]) a = s0[idx];
+        }
       }
+      w[mm] = a;
     }
-    fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
-    if (valid) {
-      cpx<P> *o = out + (long)m * out_ms + (long)f * g.ysz + ikx;
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.xp] = v[mm];
+    for (int f = 0; f < 3; f++) {
+      cpx<P> v[EPT];
+#pragma unroll
+      for (int mm = 0; mm < EPT; mm++) {
+        if (f == 0) v[mm] = to<P>(w[mm]);
+        else {
+          const double ky = kyv[mm];
+          const double wk = kx * kx + ky * ky;
+          const double wki = 1.0 / (wk > 0.001 ? wk : 0.001);
+          cpx<double> r;
+          if (f == 1) r = cpx<double>{-ky * w[mm].y * wki, ky * w[mm].x * wki};
+          else r = cpx<double>{kx * w[mm].y * wki, -kx * w[mm].x * wki};
+          v[mm] = to<P>(r);
+        }
+      }
+      fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
+      if (valid) {
+        cpx<P> *o = out + (long)m * out_ms + (long)(3 * src + f) * g.ysz + ikx;
+#pragma unroll
+        for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.xp] = v[mm];
+      }
     }
   }
 }
 
 // y-forward + update (2D).  UPD::operator()(m, ik, ikx, kx, ky, N[2]) with
 // N[0] = FFT(v w)/NR, N[1] = FFT(u w)/NR  (the reference's NZK halves).
-template <typename P, int N, int T, class UPD>
+template <typename P, int N, int T, class UPD, int NF = 2>
 __global__ void __launch_bounds__(T *(N / EPT))
 k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
   extern __shared__ char smem_raw[];
@@ -381,9 +427,9 @@ k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restr
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, m = blockIdx.z;
   const bool valid = ikx < g.iktx;
-  cpx<double> res[2][EPT];
+  cpx<double> res[NF][EPT];
 #pragma unroll
-  for (int f = 0; f < 2; f++) {
+  for (int f = 0; f < NF; f++) {
     const cpx<P> *src = in + (long)m * in_ms + (long)f * g.ysz + ikx;
     cpx<P> v[EPT];
 #pragma unroll
@@ -402,7 +448,9 @@ k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restr
     const int ky = wrapk(j + TP * mm, g.ny, g.kty);
     if (ky == PSG_NOK) continue;
     const long ik = (long)(ky + g.kty) * g.iktx + ikx;
-    cpx<double> nn[2] = {res[0][mm], res[1][mm]};
+    cpx<double> nn[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) nn[f] = res[f][mm];
     upd(m, ik, ikx, kx, (double)ky, nn);
   }
 }

@@ -257,3 +257,33 @@ class NS2D:
         spec = np.zeros(self.g.kty + 2)
         lib().or2_diag(self.h, ptr(z), C.byref(e), C.byref(v), ptr(spec))
         return e.value, v.value, spec
+
+
+class BOUS:
+    """Oracle 3D Boussinesq RK4/CN stepper (theta in radians, as the reference)."""
+
+    def __init__(self, n, v2=1 / 400.0, sc=1.0, ri=1.0, theta=np.pi / 2, dt=0.002, kf=4.0, ampfor=None,
+                 alpha=1.0, nx=None, ny=None, nz=None):
+        nx = nx or int(n / alpha)
+        ny = ny or n
+        nz = nz or n
+        if ampfor is None:
+            ampfor = -0.5 * kf
+        self.h = lib().orb_create(nx, ny, nz, alpha, v2, sc, ri, theta, dt, kf, ampfor)
+        self.g = grid3(nx, ny, nz, alpha)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orb_destroy(self.h)
+            self.h = None
+
+    def run(self, state, nsteps):
+        s = np.ascontiguousarray(state, dtype=np.complex128).copy()
+        lib().orb_run(self.h, ptr(s[0]), ptr(s[1]), ptr(s[2]), ptr(s[3]), nsteps)
+        return s
+
+    def rhs(self, state):
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        out = np.zeros_like(s)
+        lib().orb_rhs(self.h, *[ptr(s[i]) for i in range(4)], *[ptr(out[i]) for i in range(4)])
+        return out

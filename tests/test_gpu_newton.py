@@ -1,0 +1,71 @@
+"""GPU tests of the tangent-linear stepper and the Newton-GMRES-hookstep
+solver.  The tangent linear is checked against central finite differences of
+the FP64 oracle's flow map; Newton against fixed points whose location is
+known in closed form."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import psgpu_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def masked_random(g, seed, scale=1.0):
+    kx, ky, kz, L = O.tables(g)
+    rng = np.random.default_rng(seed)
+    d = (rng.standard_normal(g.nkt) + 1j * rng.standard_normal(g.nkt)) * (L == 1)
+    return scale * d / np.sqrt((np.abs(d) ** 2).sum())
+
+
+def test_tl_2d_matches_oracle_central_fd(gpu):
+    n, T, dt0, v2 = 64, 0.4, 0.01, 1 / 40.0
+    g = O.grid2(n, n)
+    z = O.NS2D(n, v2=v2, dt=dt0).run(O.ic_ns2d_cfg(g, 3, 1.0), 50)   # a developed state
+    d = masked_random(g, 11)
+    nst = int(T / dt0)
+    orc = O.NS2D(n, v2=v2, dt=T / nst)
+    eps = 1e-6
+    fd = (orc.run(z + eps * d, nst) - orc.run(z - eps * d, nst)) / (2 * eps)
+    with P.Stepper(flavour="ns2d", n=n, v2=v2, dt=dt0, kf=4.0, ampfor=-2.0) as s:
+        z1, dz1 = s.tangent(z[None], d[None], T, dt0, 0)
+    assert np.abs(z1[0] - orc.run(z, nst)).max() < 1e-11 * np.abs(z1).max()
+    err = np.abs(dz1[0] - fd).max() / np.abs(fd).max()
+    assert err < 1e-7, err
+    # linearity in the direction
+    with P.Stepper(flavour="ns2d", n=n, v2=v2, dt=dt0, kf=4.0, ampfor=-2.0) as s:
+        _, dz2 = s.tangent(z[None], 3.0 * d[None], T, dt0, 0)
+    assert np.abs(dz2 - 3.0 * dz1).max() < 1e-12 * np.abs(dz2).max()
+
+
+@pytest.mark.parametrize("jv", [0, 1])
+def test_newton_converges_to_laminar_equilibrium(gpu, jv):
+    # below the first instability the Kolmogorov laminar state is the unique
+    # nearby fixed point of Phi_T: omega(0,4) = -Re/8 (2D forcing -2 at (0,4))
+    n, re_ = 32, 5.0
+    g = O.grid2(n, n)
+    kx, ky, _, L = O.tables(g)
+    lam = np.zeros(g.nkt, complex)
+    i4 = np.where((kx == 0) & (ky == 4))[0][0]
+    lam[i4] = -re_ / 8
+    guess = lam + masked_random(g, 7, 0.05)
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / re_, dt=0.01, kf=4.0, ampfor=-2.0) as s:
+        z, res = s.newton(guess[None], T=2.0, nshift=0, fixed_period=1, jv=jv, dt0=0.01, max_gmres=40)
+    assert res["status_name"] == "CONVERGED", res
+    assert res["ner"] <= 1e-12
+    assert np.abs(z[0] - lam).max() < 1e-10
+
+
+def test_newton_converged_orbit_rechecks(gpu):
+    # the reference's own consistency check (2D_NavierStokes/src_RK4/main.F90:249-291):
+    # re-integrating a converged solution gives the same residual
+    n, re_ = 32, 5.0
+    g = O.grid2(n, n)
+    kx, ky, _, L = O.tables(g)
+    lam = np.zeros(g.nkt, complex)
+    lam[np.where((kx == 0) & (ky == 4))[0][0]] = -re_ / 8
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / re_, dt=0.01, kf=4.0, ampfor=-2.0) as s:
+        z, res = s.newton((lam + masked_random(g, 9, 0.02))[None], T=1.5, nshift=0, fixed_period=1, dt0=0.01)
+        zT = s.flow_map(z, res["T"], 0.01, 0)
+    ner = np.sqrt((np.abs(zT - z) ** 2).sum()) / np.sqrt((np.abs(zT) ** 2).sum())
+    assert abs(ner - res["ner"]) <= 1e-3 * max(res["ner"], 1e-16) + 1e-15

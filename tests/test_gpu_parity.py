@@ -160,3 +160,51 @@ def test_fp32_bulk_error_bound(gpu):
         e = s.diag()[0]
     ew = orc.diag(want)[0]
     assert abs(e - ew) / ew < 1e-6
+
+
+def test_timederiv_3d_and_2d(gpu):
+    n = 32
+    g = O.grid3(n, n, n)
+    st = O.ic_solenoidal(g, 12)
+    orc = O.NS3D(n, v2=0.02, dt=0.002, force=3)
+    nu = O.nutab(g, 0.02, 0.002)[0]
+    kx, ky, kz, L = O.tables(g)
+    want = np.where(L == 1, -nu * st + orc.nonlin(st), 0)
+    with P.Stepper(flavour="ns3d", n=n, v2=0.02) as s:
+        assert rel(s.timederiv(st), want) < 1e-13
+    g2 = O.grid2(64, 64)
+    z = O.ic_ns2d_cfg(g2, 4)
+    nu2 = O.nutab(g2, 1 / 40.0, 0.005)[0]
+    kx, ky, _, L2 = O.tables(g2)
+    want2 = np.where(L2 == 1, -nu2 * z + O.NS2D(64).nonlin(z), 0)
+    with P.Stepper(flavour="ns2d", n=64, v2=1 / 40.0, dt=0.005, kf=4.0, ampfor=-2.0) as s:
+        assert rel(s.timederiv(z[None])[0], want2) < 1e-13
+
+
+@pytest.mark.parametrize("theta", [np.pi / 2, 90.0])
+def test_bous_parity(gpu, theta):
+    n = 32
+    g = O.grid3(n, n, n)
+    st = O.ic_solenoidal(g, 41, 0.5, 0.1)
+    orc = O.BOUS(n, v2=1 / 100.0, sc=1.0, ri=1.0, theta=theta, dt=0.004)
+    want = orc.run(st, 25)
+    with P.Stepper(flavour="bous", n=n, v2=1 / 100.0, sc=1.0, ri=1.0, theta=theta, dt=0.004,
+                   force="kolmogorov") as s:
+        s.set_state(st)
+        s.run(25)
+        got = s.get_state()
+        assert rel(s.timederiv(st), orc.rhs(st)) < 1e-13
+    assert rel(got, want) < 1e-11
+    assert rel(got[3], want[3]) < 1e-10
+
+
+def test_bous_aspect_batch(gpu):
+    g = O.grid3(64, 32, 32, 0.5)
+    sts = [O.ic_solenoidal(g, 50 + m, 0.5, 0.1) for m in range(2)]
+    orc = O.BOUS(32, v2=1 / 50.0, dt=0.004, alpha=0.5, nx=64, ny=32, nz=32)
+    with P.Stepper(flavour="bous", nx=64, ny=32, nz=32, alpha=0.5, v2=1 / 50.0, dt=0.004, ri=1.0, batch=2) as s:
+        for m in range(2):
+            s.set_state(sts[m], m)
+        s.run(10)
+        for m in range(2):
+            assert rel(s.get_state(m), orc.run(sts[m], 10)) < 1e-11

@@ -73,3 +73,15 @@ def test_create_fails_loudly_without_device():
         pytest.skip("a device is visible")
     with pytest.raises(P.PsgError):
         P.Stepper(flavour="ns3d", n=32)
+
+
+@pytest.mark.parametrize("flavour", ["ns3d_dns", "ns3d_gmres", "euler", "ns2d_gmres", "ns2d_dns"])
+def test_legacy_libraries_export_reference_symbols(flavour):
+    import ctypes
+    path = os.path.join(ROOT, "psgpu_amd", "_lib", "libpsg_legacy_%s.so" % flavour)
+    lib = ctypes.CDLL(path)
+    declared = _declared("psg_legacy.h")
+    assert "timestep_cuda_%s" % flavour in open(os.path.join(ROOT, "include", "psg_legacy.h")).read()
+    for sym in ("timestep_cuda_", "set_gpu_", "timestep_cuda_%s" % flavour):
+        assert hasattr(lib, sym), sym
+    assert declared == []   # the legacy header declares no psg_* symbols

@@ -228,3 +228,40 @@ def test_nonlinear_term_is_curl_of_product():
         want[:, i] = C[:, c % n, b % n, a % n]
     got = s.nonlin(st)
     assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_bous_single_mode_decay_and_fixed_point():
+    # single Fourier mode, no buoyancy coupling (u along x, k along z, theta=pi/2):
+    # RK4 stages see C=0, the step is (1-a)/(1+a), a = dt nu k^2 / 2
+    b = O.BOUS(16, v2=0.05, ri=1.0, theta=np.pi / 2, dt=0.01, ampfor=0.0)
+    kx, ky, kz, L = O.tables(b.g)
+    st = np.zeros((4, b.g.nkt), complex)
+    i = np.where((kx == 0) & (ky == 0) & (kz == 2))[0][0]
+    u = np.array([0.3 - 0.1j, 0.0, 0.0])
+    st[:3, i] = 1j * np.cross([0.0, 0.0, 2.0], u)
+    a = 0.5 * 0.01 * 0.05 * 4
+    out = b.run(st, 4)
+    assert np.abs(out[:, i] - st[:, i] * ((1 - a) / (1 + a)) ** 4).max() < 1e-15
+    assert np.abs(out[3]).max() < 1e-17
+    # Kolmogorov laminar state is a fixed point with rho = 0
+    re_ = 20.0
+    b = O.BOUS(16, v2=1 / re_, ri=1.0, theta=np.pi / 2, dt=0.01, kf=4.0)
+    st = np.zeros((4, b.g.nkt), complex)
+    st[2, np.where((kx == 0) & (ky == 4) & (kz == 0))[0][0]] = -re_ / 8
+    assert np.abs(b.run(st, 5) - st).max() < 1e-13 * re_
+    # dz/dt vanishes there
+    assert np.abs(b.rhs(st)).max() < 1e-13 * re_
+
+
+def test_bous_projection_and_rk4_order():
+    g = O.grid3(16, 16, 16)
+    st = np.concatenate([O.ic_solenoidal(g, 9, 0.5, 0.1)], 0)
+    T = 0.08
+    ref = O.BOUS(16, v2=0.02, dt=T / 64).run(st, 64)
+    kx, ky, kz, L = O.tables(g)
+    div = np.abs(kx * ref[0] + ky * ref[1] + kz * ref[2]).max()
+    assert div < 1e-13 * np.abs(ref[:3]).max()        # Leray projection keeps div w = 0
+    errs = []
+    for nst in (8, 16):
+        errs.append(np.abs(O.BOUS(16, v2=0.02, dt=T / nst).run(st, nst) - ref).max())
+    assert errs[0] / errs[1] > 3.0                      # at least second order (RK4 stages, CN viscous part)
