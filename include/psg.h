@@ -100,8 +100,9 @@ int psg_nonlin(psg_ctx *ctx, int member, const double *host_in, double *host_out
  * 2D_NavierStokes/src_RK4/src_gmres/timederiv.F90:2-32) and the Boussinesq GPU RHS kernel
  * (3D_Boussinesq/src/kernels.h:624-692, timederiv_cuda_ src_gmres/cuda.cu:319-533). */
 int psg_timederiv(psg_ctx *ctx, int member, const double *host_in, double *host_out);
-/* physical-space fields of a member: 6 (3D: w then u) or 3 (2D: w,u,v) arrays of NR doubles,
- * x fastest, as KR_FFT_ALL produces (3D_NavierStokes/src/functions.h:26-75) */
+/* physical-space fields of a member: 6 (3D: w then u), 7 (Boussinesq: w, u, rho) or
+ * 3 (2D: w,u,v) arrays of NR doubles, x fastest, as KR_FFT_ALL produces
+ * (3D_NavierStokes/src/functions.h:26-75; 3D_Boussinesq/src/out_low.h:79-110) */
 int psg_physical(psg_ctx *ctx, int member, double *host_out);
 /* stream used by the context (hipStream_t as void*) */
 void *psg_stream(psg_ctx *ctx);

@@ -9,6 +9,8 @@
  *   libpsg_legacy_euler.so       timestep_cuda_ (28 args), set_gpu_
  *   libpsg_legacy_ns2d_gmres.so  timestep_cuda_ (19 args), set_gpu_
  *   libpsg_legacy_ns2d_dns.so    timestep_cuda_ (23 args), set_gpu_
+ *   libpsg_legacy_bous_dns.so    timestep_cuda_ (37 args), set_gpu_
+ *   libpsg_legacy_bous_gmres.so  timestep_cuda_ (37 args), timederiv_cuda_ (37 args), set_gpu_
  *
  * Semantics kept from the reference: state arrays are overwritten in place
  * with the final state, *Time is updated, the caller-built CN tables and
@@ -66,6 +68,32 @@ void timestep_cuda_ns2d_dns(psg_double2 *ZO, double *ZR, double *KX, double *KY,
                             double *DELT, double *alpha, double *v2, double *SHIFTXin, double *ResThresh,
                             int *IKF, int *IKN, int *IKTX, int *IKTY, int *KTY, int *NX, int *NY, int *NIT,
                             int *NOUT, int *L, int *RCFLAG, int *STFLAG);
+
+/* 3D_Boussinesq/src/cuda.cu:21-58: steps while tme <= Tstep; adaptive dt
+ * (adaptFLAG) and JOB<rank>/{U,V,W,rho}NNN.raw dumps as out_low.h:5-110 */
+void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet, psg_double2 *rho, double *kx,
+                            double *ky, double *kz, double *Time, double *TSTART, double *Tstep, double *AMPFOR,
+                            double *DELT, double *ResThresh, double *KFY, double *V2, double *Ri, double *Sc,
+                            double *Theta, double *alpha, double *Dtarget, int *ikF, int *ikN, int *LC, int *IKTX,
+                            int *IKTY, int *IKTZ, int *KTZ, int *NKT, int *NX, int *NY, int *NZ, int *NOUT,
+                            int *NOUTV, int *statsFLAG, int *RecFLAG, int *adaptFLAG, int *RANK);
+
+/* 3D_Boussinesq/src/src_gmres/cuda.cu:12-49: steps of DELT while Tstep-tme > DELT,
+ * then one short step to reach Tstep */
+void timestep_cuda_bous_gmres(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet, psg_double2 *rho, double *kx,
+                              double *ky, double *kz, double *Time, double *TSTART, double *Tstep, double *AMPFOR,
+                              double *DELT, double *ResThresh, double *KFY, double *V2, double *Ri, double *Sc,
+                              double *Theta, double *alpha, double *Dtarget, int *ikF, int *ikN, int *LC,
+                              int *IKTX, int *IKTY, int *IKTZ, int *KTZ, int *NKT, int *NX, int *NY, int *NZ,
+                              int *NOUT, int *NOUTV, int *statsFLAG, int *RecFLAG, int *adaptFLAG, int *RANK);
+
+/* 3D_Boussinesq/src/src_gmres/cuda.cu:319-356: dz/dt written over the 4 state fields */
+void timederiv_cuda_(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet, psg_double2 *rho, double *kx,
+                     double *ky, double *kz, double *Time, double *TSTART, double *Tstep, double *AMPFOR,
+                     double *DELT, double *ResThresh, double *KFY, double *V2, double *Ri, double *Sc,
+                     double *Theta, double *alpha, double *Dtarget, int *ikF, int *ikN, int *LC, int *IKTX,
+                     int *IKTY, int *IKTZ, int *KTZ, int *NKT, int *NX, int *NY, int *NZ, int *NOUT, int *NOUTV,
+                     int *statsFLAG, int *RecFLAG, int *adaptFLAG, int *RANK);
 
 /* In each libpsg_legacy_<flavour>.so the function above is exported as
  * timestep_cuda_ (an alias), which is what the Fortran call sites bind. */

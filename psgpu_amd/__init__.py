@@ -111,6 +111,9 @@ def lib():
     L.psgh_ic_ns2d_cfg.argtypes = [hg, C.c_uint64, _D, _P]
     L.psgh_fortran_f.argtypes = [C.c_char_p, _I, _I]
     L.psgh_fortran_f.restype = _D
+    L.psg_tangent_steps.argtypes = [_P, _P, _P, _I, _P, _P]
+    L.psg_adjoint.argtypes = [_P, _P, _P, _I, _I, _P, _P]
+    L.psg_residual_grad.argtypes = [_P, _P, _I, _I, _D, _D, _D, C.POINTER(_D), _P, _P]
     L.psg_svd.argtypes = [_I, _I, _P, _P, _P, _P]
     L.psg_hook_mu.argtypes = [_I, _P, _P, _D, _P]
     L.psg_hook_mu.restype = _D
@@ -221,6 +224,21 @@ class Stepper:
         _check(L.psg_create(C.byref(self.cfg), C.byref(h)), "psg_create")
         self.h = h
 
+    @property
+    def grid(self):
+        """Host grid description (sizes and k tables) of this context."""
+        c = self.cfg
+        if c.flavour == NS2D:
+            return hgrid(NS2D, c.nx, c.ny, 1, c.alpha)
+        return hgrid(c.flavour, c.nx, c.ny, c.nz, c.alpha)
+
+    @property
+    def mask(self):
+        """Active-mode mask L (float 0/1, length NKT)."""
+        if not hasattr(self, "_mask"):
+            self._mask = tables(self.grid)[3].astype(np.float64)
+        return self._mask
+
     def close(self):
         if getattr(self, "h", None):
             lib().psg_destroy(self.h)
@@ -280,7 +298,7 @@ class Stepper:
 
     def physical(self, member=0):
         c = self.cfg
-        nfo = 3 if c.flavour == NS2D else 6
+        nfo = 3 if c.flavour == NS2D else (7 if c.flavour == BOUS3D else 6)
         out = np.zeros((nfo, c.nz if c.flavour != NS2D else 1, c.ny, c.nx))
         _check(lib().psg_physical(self.h, member, _ptr(out)), "psg_physical")
         return out
@@ -298,6 +316,30 @@ class Stepper:
         z1, dz1 = np.zeros_like(s), np.zeros_like(d)
         _check(lib().psg_tangent(self.h, _ptr(s), _ptr(d), T, dt0, policy, _ptr(z1), _ptr(dz1)), "psg_tangent")
         return z1, dz1
+
+    def tangent_steps(self, state, dstate, nsteps):
+        """(Phi_n(state), dPhi_n(state) dstate) for n steps of dt (3D Navier-Stokes)."""
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt)
+        d = np.ascontiguousarray(dstate, dtype=np.complex128).reshape(self.nf, self.nkt)
+        z1, dz1 = np.zeros_like(s), np.zeros_like(d)
+        _check(lib().psg_tangent_steps(self.h, _ptr(s), _ptr(d), nsteps, _ptr(z1), _ptr(dz1)), "psg_tangent_steps")
+        return z1, dz1
+
+    def adjoint(self, state, lam, nsteps, ckpt=20):
+        """(Phi_n(state), dPhi_n(state)^T lam) with checkpoints every ckpt steps (3D Navier-Stokes)."""
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt)
+        lt = np.ascontiguousarray(lam, dtype=np.complex128).reshape(self.nf, self.nkt)
+        zT, l0 = np.zeros_like(s), np.zeros_like(lt)
+        _check(lib().psg_adjoint(self.h, _ptr(s), _ptr(lt), nsteps, ckpt, _ptr(zT), _ptr(l0)), "psg_adjoint")
+        return zT, l0
+
+    def residual_grad(self, state, nsteps, ckpt=20, shift=(0.0, 0.0, 0.0)):
+        """(J, dJ/dz0, Phi_n(z0)) for J = ||Phi_n(z0) - S z0||^2 (3D Navier-Stokes)."""
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt)
+        g, zT, J = np.zeros_like(s), np.zeros_like(s), C.c_double()
+        _check(lib().psg_residual_grad(self.h, _ptr(s), nsteps, ckpt, shift[0], shift[1], shift[2], C.byref(J),
+                                       _ptr(g), _ptr(zT)), "psg_residual_grad")
+        return J.value, g, zT
 
     def newton(self, guess, T, sx=0.0, sz=0.0, **opts):
         """Newton-GMRES-hookstep from (guess, T, sx[, sz]); returns (z, result dict)."""

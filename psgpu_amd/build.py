@@ -28,12 +28,13 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-
              "-I" + os.path.join(ROOT, "include")]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
-CORE_HIP = ["psg_api.hip", "psg_ns3d.hip", "psg_ns2d.hip", "psg_bous.hip", "psg_newton.hip"]
+CORE_HIP = ["psg_api.hip", "psg_ns3d.hip", "psg_ns2d.hip", "psg_bous.hip", "psg_newton.hip", "psg_tladj.hip"]
 CORE_C = ["psg_host.c"]
 # legacy shims: (library suffix, source)
 LEGACY = [("ns3d_dns", "legacy/legacy_ns3d_dns.c"), ("ns3d_gmres", "legacy/legacy_ns3d_gmres.c"),
           ("euler", "legacy/legacy_euler.c"), ("ns2d_gmres", "legacy/legacy_ns2d_gmres.c"),
-          ("ns2d_dns", "legacy/legacy_ns2d_dns.c")]
+          ("ns2d_dns", "legacy/legacy_ns2d_dns.c"), ("bous_dns", "legacy/legacy_bous_dns.c"),
+          ("bous_gmres", "legacy/legacy_bous_gmres.c")]
 
 
 def _run(cmd, verbose):

@@ -22,7 +22,8 @@ static void lg_fail(const char *where) {
 static psg_ctx *lg_context(const psg_config *c) {
   if (lg_ctx && lg_cfg.flavour == c->flavour && lg_cfg.nx == c->nx && lg_cfg.ny == c->ny && lg_cfg.nz == c->nz &&
       lg_cfg.alpha == c->alpha && lg_cfg.v2 == c->v2 && lg_cfg.force == c->force && lg_cfg.kf == c->kf &&
-      lg_cfg.ampfor == c->ampfor && lg_cfg.precision == c->precision && lg_cfg.batch == c->batch) {
+      lg_cfg.ampfor == c->ampfor && lg_cfg.precision == c->precision && lg_cfg.batch == c->batch &&
+      lg_cfg.ri == c->ri && lg_cfg.sc == c->sc && lg_cfg.theta == c->theta) {
     if (lg_cfg.dt != c->dt) {
       if (psg_set_dt(lg_ctx, c->dt)) { lg_fail("set_dt"); return NULL; }
       lg_cfg.dt = c->dt;
@@ -34,6 +35,13 @@ static psg_ctx *lg_context(const psg_config *c) {
   if (psg_create(c, &lg_ctx)) { lg_fail("create"); lg_ctx = NULL; return NULL; }
   lg_cfg = *c;
   return lg_ctx;
+}
+
+/* dt changes inside a call go through here so the cache key stays true */
+static int lg_set_dt(psg_ctx *ctx, double dt) {
+  if (psg_set_dt(ctx, dt)) return -1;
+  lg_cfg.dt = dt;
+  return 0;
 }
 
 static int lg_precision(void) {

@@ -510,12 +510,12 @@ int psg_timederiv(psg_ctx *c, int m, const double *host_in, double *host_out) {
 
 int psg_physical(psg_ctx *c, int m, double *host_out) {
   if (int r = check_member(c, m)) return r;
-  if (c->cfg.flavour == PSG_BOUS3D) { set_error("psg_physical: not available for the Boussinesq flavour"); return PSG_EINVAL; }
+  const bool bous = c->cfg.flavour == PSG_BOUS3D;
   const long nr = (long)c->g.nx * c->g.ny * c->g.nz;
-  const int nfo = c->g.dim == 2 ? 3 : 6;
+  const int nfo = c->g.dim == 2 ? 3 : (bous ? 7 : 6);
   double *d = nullptr;
   PSG_HIP(hipMalloc(&d, sizeof(double) * nr * nfo));
-  int r = c->g.dim == 2 ? ns2d_physical(c, m, d) : ns3d_physical(c, m, d);
+  int r = c->g.dim == 2 ? ns2d_physical(c, m, d) : (bous ? bous_physical(c, m, d) : ns3d_physical(c, m, d));
   if (!r) {
     if (hipMemcpyAsync(host_out, d, sizeof(double) * nr * nfo, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) { set_error("physical copy"); r = PSG_EHIP; }

@@ -15,11 +15,12 @@
 
 namespace psg {
 
-// pass 1: outputs w(3), u(3), i kx rho, i ky rho, i kz rho
+// pass 1: outputs w(3), u(3), i kx rho, i ky rho, i kz rho; with nout = 7 the
+// seventh field is rho itself (physical-space dumps, 3D_Boussinesq/src/out_low.h:79-110)
 template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
-            const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask) {
+            const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask, int nout) {
   extern __shared__ char smem_raw[];
   cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
   constexpr int TP = N / EPT;
@@ -41,7 +42,7 @@ k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__r
     lm[mm] = (umask && q.idx >= 0) ? (double)mask[q.idx] : 1.0;
   }
 #pragma unroll 1
-  for (int f = 0; f < 9; f++) {
+  for (int f = 0; f < nout; f++) {
     cpx<P> v[EPT];
     if (f == 6) {
       // vorticity no longer needed: reuse its registers for rho
@@ -64,6 +65,8 @@ k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__r
         cpx<double> u, vv, ww;
         biot_savart(kx, kyd, kzd, w[0][mm], w[1][mm], w[2][mm], wki, u, vv, ww);
         r = f == 3 ? u : (f == 4 ? vv : ww);
+      } else if (nout == 7) {
+        r = w[0][mm];
       } else {
         const double kk = f == 6 ? kx : (f == 7 ? kyd : kzd);   // setFFK: i k rho
         r = cpx<double>{-kk * w[0][mm].y, kk * w[0][mm].x};
@@ -202,7 +205,8 @@ __global__ void k_copy(const cpx<double> *__restrict__ a, cpx<double> *__restric
     if (r_) return r_;  \
   } while (0)
 
-template <typename P> static int launch_zinv_b(Ctx *c, const cpx<double> *st, long st_ms, int umask, int nm) {
+template <typename P>
+static int launch_zinv_b(Ctx *c, const cpx<double> *st, long st_ms, int umask, int nm, int nout = 9) {
   const Grid &g = c->g;
   cpx<P> *A = (cpx<P> *)c->bufA;
   PSG_DISPATCH_N(g.nz, {
@@ -212,7 +216,7 @@ template <typename P> static int launch_zinv_b(Ctx *c, const cpx<double> *st, lo
     auto k = k_zinv_bous<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, A, 9 * g.zsz, g,
-                       twp<P>(c, 2), c->mask, umask);
+                       twp<P>(c, 2), c->mask, umask, nout);
   });
   PSG_HIP(hipGetLastError());
   return 0;
@@ -221,6 +225,7 @@ template <typename P> static int launch_zinv_b(Ctx *c, const cpx<double> *st, lo
 // generic column/row passes shared with the Navier-Stokes path
 template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
 template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
+template <typename P> int launch_xphys(Ctx *c, const cpx<P> *in, double *out, int nf);
 
 template <typename P, int FI, int FO, class PROD>
 static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
@@ -318,6 +323,18 @@ template <typename P> static int rhsb_t(Ctx *c, const cpx<double> *in, cpx<doubl
 
 int bous_timederiv(Ctx *c, const cpx<double> *in_dev, cpx<double> *out_dev) {
   return c->cfg.precision == PSG_FP32BULK ? rhsb_t<float>(c, in_dev, out_dev) : rhsb_t<double>(c, in_dev, out_dev);
+}
+
+// physical fields of one member: w(3), u(3), rho, NR doubles each, x fastest
+template <typename P> static int physb_t(Ctx *c, int member, double *out) {
+  const Grid &g = c->g;
+  PSG_TRYB(launch_zinv_b<P>(c, c->state + (long)member * 4 * g.nkt, 4 * g.nkt, 1, 1, 7));
+  PSG_TRYB(launch_yinv_g<P>(c, (const cpx<P> *)c->bufA, (cpx<P> *)c->bufB, 7));
+  return launch_xphys<P>(c, (const cpx<P> *)c->bufB, out, 7);
+}
+
+int bous_physical(Ctx *c, int member, double *out_dev) {
+  return c->cfg.precision == PSG_FP32BULK ? physb_t<float>(c, member, out_dev) : physb_t<double>(c, member, out_dev);
 }
 
 int bous_init(Ctx *c) {

@@ -99,9 +99,13 @@ int diag_member(Ctx *c, int member, double *e, double *z, double *spec);
 int bous_init(Ctx *c);
 int bous_run(Ctx *c, int nsteps);
 int bous_timederiv(Ctx *c, const cpx<double> *in_dev, cpx<double> *out_dev);
+int bous_physical(Ctx *c, int member, double *out_dev);
 int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out);
 int ns2d_tl_run(Ctx *c, int nsteps);
 int ns3d_tl_run(Ctx *c, int nsteps);
+int ns3d_step_rec(Ctx *c, int umask, cpx<double> *wn, cpx<double> *wstar);
+int ns3d_adjoint(Ctx *c, const cpx<double> *z0, const cpx<double> *lamT, int nsteps, int ckpt, cpx<double> *zT,
+                 cpx<double> *lam0);
 int tl_alloc(Ctx *c);
 
 }  // namespace psg

@@ -10,38 +10,9 @@
 // The reference stores the velocity UK,VK,WK after every update kernel; here
 // it is recomputed from the vorticity inside the first transform pass, which
 // is the same Biot-Savart expression and saves 6 NKT complex of traffic.
-#include "psg_launch.hpp"
+#include "psg_ns3d.hpp"
 
 namespace psg {
-
-struct Force {
-  int mode;
-  double kf, a;
-};
-
-// forcing on the curl result: 3D_NavierStokes/src/kernels.h:342-343 (DNS),
-// src_gmres/kernels.h:234-238 (GMRES), Kolmogorov f=(sin kf y,0,0) as
-// 3D_Boussinesq/src/kernels.h:382-384
-__device__ __forceinline__ void add_forcing(const Force &F, double kx, double ky, double kz, double &xiX,
-                                            double &etY, double &ztX, double &ztY) {
-  if (F.mode == PSG_FORCE_DNS) {
-    if (kx == 0.0 && kz == F.kf && ky == F.kf) xiX += F.a;
-    if (kx == 0.0 && kz == -F.kf && ky == F.kf) xiX -= F.a;
-  } else if (F.mode == PSG_FORCE_GMRES) {
-    if (kx == 0.0 && kz == F.kf && ky == F.kf) { etY += F.a; ztY -= F.a; }
-    if (kx == 0.0 && kz == -F.kf && ky == F.kf) { etY += F.a; ztY += F.a; }
-  } else if (F.mode == PSG_FORCE_KOLMO) {
-    if (kx == 0.0 && kz == 0.0 && ky == F.kf) ztX += F.a;
-  }
-}
-
-// curl of the transformed product (the last part of convol, kernels.h:326-334)
-#define PSG_CURL                                                      \
-  const double UKX = N[0].x, UKY = N[0].y, VKX = N[1].x, VKY = N[1].y; \
-  const double WKX = N[2].x, WKY = N[2].y;                            \
-  double xiX = ky * WKY - kz * VKY, xiY = kz * VKX - ky * WKX;        \
-  double etX = kz * UKY - kx * WKY, etY = kx * WKX - kz * UKX;        \
-  double ztX = kx * VKY - ky * UKY, ztY = ky * UKX - kx * VKX;
 
 template <int PHASE, bool EULER> struct UpdHeun {
   cpx<double> *st;
@@ -113,31 +84,7 @@ struct UpdStoreNL {
   }
 };
 
-// ------------------------------------------------------------------ launches
-template <typename P>
-static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, long out_ms, bool bs, int umask,
-                       int nm) {
-  const Grid &g = c->g;
-  PSG_DISPATCH_N(g.nz, {
-    constexpr int T = Col<N>::T;
-    dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
-    if (bs) {
-      auto k = k_zinv<P, N, T, true>;
-      PSG_HIP(lds_attr(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
-                         twp<P>(c, 2), c->mask, umask);
-    } else {
-      auto k = k_zinv<P, N, T, false>;
-      PSG_HIP(lds_attr(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
-                         twp<P>(c, 2), c->mask, umask);
-    }
-  });
-  PSG_HIP(hipGetLastError());
-  return 0;
-}
-
+// generic column passes (also used by the Boussinesq and tangent-linear units)
 template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.ny, {
@@ -170,55 +117,6 @@ template int launch_yinv_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int);
 template int launch_yinv_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int);
 template int launch_yfwd_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int);
 template int launch_yfwd_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int);
-
-template <typename P, int FI, int FO, class PROD>
-static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
-  const Grid &g = c->g;
-  PSG_DISPATCH_N(g.nx, {
-    constexpr int TE = Row<N>::TEAMS;
-    long nrows = nrow_m * nm;
-    dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)padded_len<N>() * TE * sizeof(cpx<P>);
-    auto k = k_xpass<P, N, TE, FI, FO, PROD>;
-    PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
-                       prod);
-  });
-  PSG_HIP(hipGetLastError());
-  return 0;
-}
-
-template <typename P, int FO, class UPD> static int launch_zfwd(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
-  const Grid &g = c->g;
-  PSG_DISPATCH_N(g.nz, {
-    constexpr int T = Col<N>::T;
-    dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
-    auto k = k_zfwd<P, N, T, FO, UPD>;
-    PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);
-  });
-  PSG_HIP(hipGetLastError());
-  return 0;
-}
-
-#define PSG_TRY(x)          \
-  do {                      \
-    int r_ = (x);           \
-    if (r_) return r_;      \
-  } while (0)
-
-// bytes of the five passes, per member, for the profiler (algorithmic:
-// every buffer element read or written once)
-static void pass_bytes(const Ctx *c, int fi, int fo, size_t b, long out[5]) {
-  const Grid &g = c->g;
-  long Z = (long)g.iktx * g.ikty * g.nz, Y = (long)g.iktx * g.ny * g.nz;
-  out[0] = 3 * g.nkt * 16 + fi * Z * (long)b;
-  out[1] = fi * (Z + Y) * (long)b;
-  out[2] = (fi + fo) * Y * (long)b;
-  out[3] = fo * (Y + Z) * (long)b;
-  out[4] = fo * Z * (long)b;
-}
 
 // One right-hand side evaluation for all members, ending in functor upd.
 template <typename P, class UPD>
@@ -273,6 +171,29 @@ int ns3d_run(Ctx *c, int nsteps) {
   return c->cfg.precision == PSG_FP32BULK ? run_t<float>(c, nsteps) : run_t<double>(c, nsteps);
 }
 
+// One Navier-Stokes step of member 0 that records w^n (before the step) and
+// the predictor state w* into wn / wstar (device, 3*nkt complex each, either
+// may be null): the trajectory the adjoint sweep of a checkpoint segment needs.
+template <typename P> static int step_rec_t(Ctx *c, int umask, cpx<double> *wn, cpx<double> *wstar) {
+  const Grid &g = c->g;
+  Force F{c->cfg.force, c->cfg.kf, c->cfg.ampfor};
+  const long ms = 3 * g.nkt, fs = g.nkt;
+  const size_t bytes = sizeof(cpx<double>) * 3 * g.nkt;
+  if (wn) PSG_HIP(hipMemcpyAsync(wn, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
+  UpdHeun<0, false> up{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+  PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, 9 * g.nkt * 16, 1));
+  if (wstar) PSG_HIP(hipMemcpyAsync(wstar, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
+  UpdHeun<1, false> uc{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+  PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, 9 * g.nkt * 16, 1));
+  c->time += c->cfg.dt;
+  return 0;
+}
+
+int ns3d_step_rec(Ctx *c, int umask, cpx<double> *wn, cpx<double> *wstar) {
+  return c->cfg.precision == PSG_FP32BULK ? step_rec_t<float>(c, umask, wn, wstar)
+                                          : step_rec_t<double>(c, umask, wn, wstar);
+}
+
 template <typename P> static int nonlin_t(Ctx *c, const cpx<double> *in, cpx<double> *out) {
   const Grid &g = c->g;
   Force F{c->cfg.force, c->cfg.kf, c->cfg.ampfor};
@@ -316,22 +237,30 @@ k_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, int nf, long nr
   }
 }
 
+// C2R row pass over nf fields already through the z and y inverse passes
+template <typename P> int launch_xphys(Ctx *c, const cpx<P> *B, double *out, int nf) {
+  const Grid &g = c->g;
+  long nrow = (long)g.ny * g.nz;
+  PSG_DISPATCH_N(g.nx, {
+    constexpr int TE = Row<N>::TEAMS;
+    dim3 grid((unsigned)((nrow + TE - 1) / TE), nf);
+    size_t lds = (size_t)N * TE * sizeof(cpx<P>);
+    auto k = k_xphys<P, N, TE>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, B, out, nf, nrow, g, twp<P>(c, 0));
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+template int launch_xphys<float>(Ctx *, const cpx<float> *, double *, int);
+template int launch_xphys<double>(Ctx *, const cpx<double> *, double *, int);
+
 template <typename P> static int physical_t(Ctx *c, int member, double *out) {
   const Grid &g = c->g;
   cpx<P> *A = (cpx<P> *)c->bufA, *B = (cpx<P> *)c->bufB;
   PSG_TRY(launch_zinv<P>(c, c->state + (long)member * 3 * g.nkt, 3 * g.nkt, A, 6 * g.zsz, true, 1, 1));
   PSG_TRY(launch_yinv_g<P>(c, A, B, 6));
-  long nrow = (long)g.ny * g.nz;
-  PSG_DISPATCH_N(g.nx, {
-    constexpr int TE = Row<N>::TEAMS;
-    dim3 grid((unsigned)((nrow + TE - 1) / TE), 6);
-    size_t lds = (size_t)N * TE * sizeof(cpx<P>);
-    auto k = k_xphys<P, N, TE>;
-    PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, B, out, 6, nrow, g, twp<P>(c, 0));
-  });
-  PSG_HIP(hipGetLastError());
-  return 0;
+  return launch_xphys<P>(c, B, out, 6);
 }
 
 int ns3d_physical(Ctx *c, int member, double *out_dev) {
@@ -355,11 +284,3 @@ int ns3d_init(Ctx *c) {
 
 }  // namespace psg
 
-namespace psg {
-// 3D tangent linear: provided by the tangent-linear/adjoint translation unit
-// when built; until then the Navier-Stokes 3D Newton solver uses finite differences.
-int ns3d_tl_run(Ctx *, int) {
-  set_error("3D tangent-linear stepping is not built");
-  return PSG_EINVAL;
-}
-}  // namespace psg

@@ -1,0 +1,118 @@
+// Shared pieces of the 3D Navier-Stokes right-hand side: forcing, curl,
+// and the launch helpers of the five pencil passes.  Included by the stepper
+// and by the tangent-linear/adjoint translation unit.
+#pragma once
+#include "psg_launch.hpp"
+
+namespace psg {
+
+struct Force {
+  int mode;
+  double kf, a;
+};
+
+// forcing on the curl result: 3D_NavierStokes/src/kernels.h:342-343 (DNS),
+// src_gmres/kernels.h:234-238 (GMRES), Kolmogorov f=(sin kf y,0,0) as
+// 3D_Boussinesq/src/kernels.h:382-384
+__device__ __forceinline__ void add_forcing(const Force &F, double kx, double ky, double kz, double &xiX,
+                                            double &etY, double &ztX, double &ztY) {
+  if (F.mode == PSG_FORCE_DNS) {
+    if (kx == 0.0 && kz == F.kf && ky == F.kf) xiX += F.a;
+    if (kx == 0.0 && kz == -F.kf && ky == F.kf) xiX -= F.a;
+  } else if (F.mode == PSG_FORCE_GMRES) {
+    if (kx == 0.0 && kz == F.kf && ky == F.kf) { etY += F.a; ztY -= F.a; }
+    if (kx == 0.0 && kz == -F.kf && ky == F.kf) { etY += F.a; ztY += F.a; }
+  } else if (F.mode == PSG_FORCE_KOLMO) {
+    if (kx == 0.0 && kz == 0.0 && ky == F.kf) ztX += F.a;
+  }
+}
+
+// curl of the transformed product (the last part of convol, kernels.h:326-334)
+#define PSG_CURL                                                      \
+  const double UKX = N[0].x, UKY = N[0].y, VKX = N[1].x, VKY = N[1].y; \
+  const double WKX = N[2].x, WKY = N[2].y;                            \
+  double xiX = ky * WKY - kz * VKY, xiY = kz * VKX - ky * WKX;        \
+  double etX = kz * UKY - kx * WKY, etY = kx * WKX - kz * UKX;        \
+  double ztX = kx * VKY - ky * UKY, ztY = ky * UKX - kx * VKX;
+
+
+// ------------------------------------------------------------------ launches
+template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
+template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
+
+template <typename P>
+static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, long out_ms, bool bs, int umask,
+                       int nm) {
+  const Grid &g = c->g;
+  PSG_DISPATCH_N(g.nz, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
+    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    if (bs) {
+      auto k = k_zinv<P, N, T, true>;
+      PSG_HIP(lds_attr(k, lds));
+      hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
+                         twp<P>(c, 2), c->mask, umask);
+    } else {
+      auto k = k_zinv<P, N, T, false>;
+      PSG_HIP(lds_attr(k, lds));
+      hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
+                         twp<P>(c, 2), c->mask, umask);
+    }
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+
+template <typename P, int FI, int FO, class PROD, int E = EPT, bool FRONT = true>
+static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
+  const Grid &g = c->g;
+  PSG_DISPATCH_N(g.nx, {
+    constexpr int TPX = N / E;
+    constexpr int TE = (256 / TPX) < 1 ? 1 : 256 / TPX;
+    long nrows = nrow_m * nm;
+    dim3 grid((unsigned)((nrows + TE - 1) / TE));
+    size_t lds = (size_t)padded_len<N>() * TE * sizeof(cpx<P>);
+    auto k = k_xpass<P, N, TE, FI, FO, PROD, E, FRONT>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(TE * TPX), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0), prod);
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename P, int FO, class UPD> static int launch_zfwd(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
+  const Grid &g = c->g;
+  PSG_DISPATCH_N(g.nz, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
+    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    auto k = k_zfwd<P, N, T, FO, UPD>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+#define PSG_TRY(x)          \
+  do {                      \
+    int r_ = (x);           \
+    if (r_) return r_;      \
+  } while (0)
+
+// bytes of the five passes, per member, for the profiler (algorithmic:
+// every buffer element read or written once)
+inline void pass_bytes(const Ctx *c, int fi, int fo, size_t b, long out[5]) {
+  const Grid &g = c->g;
+  long Z = (long)g.iktx * g.ikty * g.nz, Y = (long)g.iktx * g.ny * g.nz;
+  out[0] = 3 * g.nkt * 16 + fi * Z * (long)b;
+  out[1] = fi * (Z + Y) * (long)b;
+  out[2] = (fi + fo) * Y * (long)b;
+  out[3] = fo * (Y + Z) * (long)b;
+  out[4] = fo * Z * (long)b;
+}
+
+
+}  // namespace psg

@@ -187,23 +187,33 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
   extern __shared__ char smem_raw[];
   cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
   constexpr int TP = N / EPT;
+  // up to 4 fields every input is requested before the first transform; the
+  // 6-field tangent-linear/adjoint updates load each field before its transform
+  constexpr bool FRONT = FO <= 4;
+  constexpr int NV = FRONT ? FO : 1;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
   const bool valid = ikx < g.iktx;
   cpx<double> res[FO][EPT];
-  cpx<P> vin[FO][EPT];
-#pragma unroll
-  for (int f = 0; f < FO; f++) {
+  cpx<P> vin[NV][EPT];
+  auto load = [&](int f, cpx<P>(&v)[EPT]) {
     const cpx<P> *src = in + ((long)m * FO + f) * g.zsz + (long)iky * g.xp + ikx;
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++)
-      vin[f][mm] = valid ? src[(long)(j + TP * mm) * g.ikty * g.xp] : cpx<P>{0, 0};
+    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[(long)(j + TP * mm) * g.ikty * g.xp] : cpx<P>{0, 0};
+  };
+  if constexpr (FRONT) {
+#pragma unroll
+    for (int f = 0; f < FO; f++) load(f, vin[f]);
   }
 #pragma unroll
   for (int f = 0; f < FO; f++) {
     cpx<P> v[EPT];
+    if constexpr (FRONT) {
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) v[mm] = vin[f][mm];
+      for (int mm = 0; mm < EPT; mm++) v[mm] = vin[f][mm];
+    } else {
+      load(f, v);
+    }
     fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
@@ -226,66 +236,83 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
 }
 
 // ---------------------------------------------------------------- pass 3 (x)
-// Row teams: TEAMS rows per block, N/8 threads per row.  Fields are paired
+// Row teams: TEAMS rows per block, N/E threads per row.  Fields are paired
 // into complex transforms z = a + i b.  PROD::apply(double in[FI], double out[FO]).
 // A team that fits in one wavefront synchronises with wavefront fences only;
 // its LDS row is padded (p -> p + p/8) so radix-8 writes are bank-conflict free.
-template <typename P, int N, int TEAMS, int FI, int FO, class PROD>
-__global__ void __launch_bounds__(TEAMS *(N / EPT))
-k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
-        const cpx<P> *__restrict__ tw, PROD prod) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
-  constexpr int TPX = N / EPT;
-  constexpr bool WAVE = TPX <= 64;
-  constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2;
-  const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
-  const long row = (long)blockIdx.x * TEAMS + team;
-  const bool valid = row < nrows;
-  const long m = valid ? row / nrow_m : 0, rr = valid ? row % nrow_m : 0;
-  cpx<P> *tl = lds + team * padded_len<N>();
-  const int ix = g.iktx;
-  // every input of the row is requested before the first transform
-  cpx<P> vin[PI][EPT];
+// FRONT: every input of the row is requested before the first transform
+// (latency hiding for the 6-field products); without it each pair is loaded
+// just before its transform, which keeps the 9- and 12-field products of the
+// tangent-linear and adjoint passes within the register file.
+template <typename P, int N, int E> struct XRow {
+  // pair p of a row: z = a + i b from the half spectra of fields fa, fb
+  __device__ __forceinline__ static void load(cpx<P> (&v)[E], const cpx<P> *ra, const cpx<P> *rb, bool hasb,
+                                              bool valid, int j, int ix) {
+    constexpr int TPX = N / E;
 #pragma unroll
-  for (int p = 0; p < PI; p++) {
-    const int fa = 2 * p, fb = 2 * p + 1;
-    const cpx<P> *ra = in + ((long)m * FI + fa) * g.ysz + rr * g.xp;
-    const cpx<P> *rb = in + ((long)m * FI + fb) * g.ysz + rr * g.xp;
-#pragma unroll
-    for (int mm = 0; mm < EPT; mm++) {
+    for (int mm = 0; mm < E; mm++) {
       const int k = j + TPX * mm;
       cpx<P> a{0, 0}, b{0, 0};
       cpx<P> z{0, 0};
       if (valid) {
         if (k < ix) {
           a = ra[k];
-          if (fb < FI) b = rb[k];
+          if (hasb) b = rb[k];
           if (k == 0) { a.y = 0; b.y = 0; }
           z = cpx<P>{a.x - b.y, a.y + b.x};
         } else if (k > N - ix) {
           a = ra[N - k];
-          if (fb < FI) b = rb[N - k];
+          if (hasb) b = rb[N - k];
           z = cpx<P>{a.x + b.y, b.x - a.y};
         }
       }
-      vin[p][mm] = z;
+      v[mm] = z;
     }
   }
-  P phys[2 * PI][EPT];
+};
+
+template <typename P, int N, int TEAMS, int FI, int FO, class PROD, int E = EPT, bool FRONT = true>
+__global__ void __launch_bounds__(TEAMS *(N / E))
+k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
+        const cpx<P> *__restrict__ tw, PROD prod) {
+  extern __shared__ char smem_raw[];
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  constexpr int TPX = N / E;
+  constexpr bool WAVE = TPX <= 64;
+  constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2;
+  constexpr int NV = FRONT ? PI : 1;
+  const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
+  const long row = (long)blockIdx.x * TEAMS + team;
+  const bool valid = row < nrows;
+  const long m = valid ? row / nrow_m : 0, rr = valid ? row % nrow_m : 0;
+  cpx<P> *tl = lds + team * padded_len<N>();
+  const int ix = g.iktx;
+  auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + rr * g.xp; };
+  cpx<P> vin[NV][E];
+  if constexpr (FRONT) {
+#pragma unroll
+    for (int p = 0; p < PI; p++)
+      XRow<P, N, E>::load(vin[p], rowp(2 * p), rowp(2 * p + 1 < FI ? 2 * p + 1 : 2 * p), 2 * p + 1 < FI, valid, j,
+                          ix);
+  }
+  P phys[2 * PI][E];
 #pragma unroll
   for (int p = 0; p < PI; p++) {
     const int fa = 2 * p, fb = 2 * p + 1;
-    cpx<P> v[EPT];
+    cpx<P> v[E];
+    if constexpr (FRONT) {
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) v[mm] = vin[p][mm];
-    fft_pencil<P, N, EPT, true, WAVE, true>(v, tl, j, 1, 0, tw);
+      for (int mm = 0; mm < E; mm++) v[mm] = vin[p][mm];
+    } else {
+      XRow<P, N, E>::load(v, rowp(fa), rowp(fb < FI ? fb : fa), fb < FI, valid, j, ix);
+    }
+    fft_pencil<P, N, E, true, WAVE, true>(v, tl, j, 1, 0, tw);
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) { phys[fa][mm] = v[mm].x; phys[fb][mm] = v[mm].y; }
+    for (int mm = 0; mm < E; mm++) { phys[fa][mm] = v[mm].x; phys[fb][mm] = v[mm].y; }
   }
-  P res[2 * PO][EPT];
+  P res[2 * PO][E];
 #pragma unroll
-  for (int mm = 0; mm < EPT; mm++) {
+  for (int mm = 0; mm < E; mm++) {
     P pin[2 * PI], pout[2 * PO];
 #pragma unroll
     for (int f = 0; f < 2 * PI; f++) pin[f] = phys[f][mm];
@@ -297,13 +324,13 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   }
 #pragma unroll
   for (int q = 0; q < PO; q++) {
-    cpx<P> v[EPT];
+    cpx<P> v[E];
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) v[mm] = cpx<P>{res[2 * q][mm], res[2 * q + 1][mm]};
-    fft_pencil<P, N, EPT, false, WAVE, true>(v, tl, j, 1, 0, tw);
+    for (int mm = 0; mm < E; mm++) v[mm] = cpx<P>{res[2 * q][mm], res[2 * q + 1][mm]};
+    fft_pencil<P, N, E, false, WAVE, true>(v, tl, j, 1, 0, tw);
     team_sync<WAVE>();
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) tl[lpos<true>(j + TPX * mm)] = v[mm];
+    for (int mm = 0; mm < E; mm++) tl[lpos<true>(j + TPX * mm)] = v[mm];
     team_sync<WAVE>();
     if (valid) {
       cpx<P> *oa = out + ((long)m * FO + 2 * q) * g.ysz + rr * g.xp;

@@ -43,7 +43,7 @@ def test_legacy_ns3d_gmres_matches_oracle(gpu, monkeypatch):
     dt, v2, kf = 0.002, 1 / 40.0, 4
     nu, nuzn, nu1 = O.nutab(g, v2, dt)
     st = O.ic_ns3d_ref(g, 837920, 1.0)
-    x, e, z = (np.ascontiguousarray(st[i]) for i in range(3))
+    x, e, z = (st[i].copy() for i in range(3))   # copies: the call writes in place
     T, T0, amp, DT, V2 = C.c_double(0), C.c_double(1.5), C.c_double(-0.25 * kf), C.c_double(dt), C.c_double(v2)
     KF, NST = C.c_int(kf), C.c_int(30)
     lib = load("ns3d_gmres")
@@ -70,7 +70,7 @@ def test_legacy_ns3d_dns_outputs(gpu, tmp_path, monkeypatch):
     dt, v2, kf = 0.005, 0.01, 4.0
     nu, nuzn, nu1 = O.nutab(g, v2, dt)
     st = O.ic_ns3d_ref(g, 4242, 1.0)
-    x, e, z = (np.ascontiguousarray(st[i]) for i in range(3))
+    x, e, z = (st[i].copy() for i in range(3))   # copies: the call writes in place
     nr = n ** 3
     xr, er, zr = np.zeros(nr), np.zeros(nr), np.zeros(nr)
     d = lambda v: C.c_double(v)
@@ -122,7 +122,7 @@ def test_legacy_euler_matches_oracle(gpu, tmp_path, monkeypatch):
     monkeypatch.chdir(tmp_path)
     g, kx, ky, kz, L, ikf, ikn = tables3(n)
     st = O.ic_solenoidal(g, 77, 0.5)
-    x, e, z = (np.ascontiguousarray(st[i]) for i in range(3))
+    x, e, z = (st[i].copy() for i in range(3))   # copies: the call writes in place
     nr = n ** 3
     xr, er, zr = np.zeros(nr), np.zeros(nr), np.zeros(nr)
     d = lambda v: C.c_double(v)
@@ -134,3 +134,90 @@ def test_legacy_euler_matches_oracle(gpu, tmp_path, monkeypatch):
                                  r(i(4)), r(i(100)), r(i(0)))
     want = O.NS3D(n, dt=0.005, flavour=1, force=0).run(st, 12)
     assert rel(np.stack([x, e, z]), want) < 1e-11
+
+
+def bous_args(g, n, st, T, T0, Tstep, amp, dt, kf, v2, ri, sc, theta, L, kx, ky, kz, ikf, ikn,
+              nout=1, noutv=1, stats=0, rec=0, adapt=0, rank=0):
+    """The 37-argument Boussinesq list (3D_Boussinesq/src/cuda.cu:21-58), by address."""
+    d = lambda v: C.c_double(v)
+    i = lambda v: C.c_int(v)
+    keep = [d(Tstep), d(amp), d(dt), d(1.0), d(kf), d(v2), d(ri), d(sc), d(theta), d(1.0), d(0.0),
+            i(g.iktx), i(g.ikty), i(g.iktz), i(g.ktz), i(g.nkt), i(n), i(n), i(n), i(nout), i(noutv), i(stats),
+            i(rec), i(adapt), i(rank)]
+    a = [arr(st[0]), arr(st[1]), arr(st[2]), arr(st[3]), arr(kx), arr(ky), arr(kz), r(T), r(T0)]
+    a += [r(v) for v in keep[:11]] + [arr(ikf), arr(ikn), arr(L)] + [r(v) for v in keep[11:]]
+    return a, keep
+
+
+def test_legacy_bous_gmres_and_timederiv(gpu):
+    n = 32
+    g, kx, ky, kz, L, ikf, ikn = tables3(n)
+    v2, sc, ri, theta, dt, kf, Tstep = 1 / 100.0, 1.0, 1.0, np.pi / 2, 0.004, 4.0, 0.1
+    st0 = O.ic_solenoidal(g, 61, 0.5, 0.1)
+    st = [st0[i].copy() for i in range(4)]
+    T, T0 = C.c_double(0.0), C.c_double(0.0)
+    lib = load("bous_gmres")
+    a, keep = bous_args(g, n, st, T, T0, Tstep, -kf / 2, dt, kf, v2, ri, sc, theta, L, kx, ky, kz, ikf, ikn)
+    lib.timestep_cuda_(*a)
+    # the driver's loop (src_gmres/cuda.cu:188-262) replayed on the host
+    tme, nst = 0.0, 0
+    while Tstep - tme > dt:
+        tme += dt
+        nst += 1
+    orc = O.BOUS(n, v2=v2, sc=sc, ri=ri, theta=theta, dt=dt)
+    want = orc.run(st0, nst)
+    if tme != Tstep:
+        want = O.BOUS(n, v2=v2, sc=sc, ri=ri, theta=theta, dt=Tstep - tme).run(want, 1)
+    assert rel(np.stack(st), want) < 1e-11
+    assert abs(T.value - nst * dt) < 1e-14
+    # timederiv_cuda_ overwrites the state with dz/dt
+    st2 = [st0[i].copy() for i in range(4)]
+    a, keep = bous_args(g, n, st2, T, T0, Tstep, -kf / 2, dt, kf, v2, ri, sc, theta, L, kx, ky, kz, ikf, ikn)
+    lib.timederiv_cuda_(*a)
+    assert rel(np.stack(st2), orc.rhs(st0)) < 1e-13
+
+
+def test_legacy_bous_dns_steps_and_dumps(gpu, tmp_path, monkeypatch):
+    n = 32
+    monkeypatch.chdir(tmp_path)
+    g, kx, ky, kz, L, ikf, ikn = tables3(n)
+    v2, sc, ri, theta, dt, kf, Tstep = 1 / 100.0, 1.0, 1.0, np.pi / 2, 0.004, 4.0, 0.06
+    st0 = O.ic_solenoidal(g, 62, 0.5, 0.1)
+    st = [st0[i].copy() for i in range(4)]
+    T, T0 = C.c_double(0.0), C.c_double(0.0)
+    a, keep = bous_args(g, n, st, T, T0, Tstep, -kf / 2, dt, kf, v2, ri, sc, theta, L, kx, ky, kz, ikf, ikn,
+                        nout=5, noutv=10, stats=1)
+    load("bous_dns").timestep_cuda_(*a)
+    # while (tme <= Tstep) with dt = Tstep/(Tstep/DELT) (init.h:24-29, cuda.cu:68-99)
+    nstop = Tstep / dt
+    d = Tstep / nstop if nstop * dt != Tstep else dt
+    tme, nst = 0.0, 0
+    while tme <= Tstep:
+        tme += d
+        nst += 1
+    orc = O.BOUS(n, v2=v2, sc=sc, ri=ri, theta=theta, dt=d)
+    assert rel(np.stack(st), orc.run(st0, nst)) < 1e-11
+    assert abs(T.value - tme) < 1e-14
+    # first dump: velocity and density of the initial state, x fastest
+    ns = O.NS3D(n)
+    phys = ns.physical(st0[:3])
+    zero = np.zeros_like(st0[0])
+    rho = ns.physical(np.stack([st0[3], zero, zero]))[0]
+    for name, want in (("U", phys[3]), ("V", phys[4]), ("W", phys[5]), ("rho", rho)):
+        got = np.fromfile("JOB0/%s001.raw" % name)
+        assert rel(got, want.ravel()) < 1e-12, name
+    assert os.path.exists("JOB0/U002.raw")
+
+
+def test_bous_physical_fields(gpu):
+    import psgpu_amd as P
+    n = 32
+    g = O.grid3(n, n, n)
+    st = O.ic_solenoidal(g, 63, 0.5, 0.1)
+    with P.Stepper(flavour="bous", n=n, v2=1 / 100.0, ri=1.0) as s:
+        s.set_state(st)
+        got = s.physical()
+    ns = O.NS3D(n)
+    zero = np.zeros_like(st[0])
+    assert rel(got[:6], ns.physical(st[:3])) < 1e-13
+    assert rel(got[6], ns.physical(np.stack([st[3], zero, zero]))[0]) < 1e-13

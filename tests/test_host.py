@@ -21,7 +21,7 @@ def _declared(header):
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header", ["psg.h", "psg_host.h"])
+@pytest.mark.parametrize("header", ["psg.h", "psg_host.h", "psg_newton.h", "psg_adjoint.h"])
 def test_library_exports_declared_symbols(header):
     L = P.lib()
     missing = [n for n in _declared(header) if not hasattr(L, n)]
@@ -75,7 +75,8 @@ def test_create_fails_loudly_without_device():
         P.Stepper(flavour="ns3d", n=32)
 
 
-@pytest.mark.parametrize("flavour", ["ns3d_dns", "ns3d_gmres", "euler", "ns2d_gmres", "ns2d_dns"])
+@pytest.mark.parametrize("flavour", ["ns3d_dns", "ns3d_gmres", "euler", "ns2d_gmres", "ns2d_dns", "bous_dns",
+                                     "bous_gmres"])
 def test_legacy_libraries_export_reference_symbols(flavour):
     import ctypes
     path = os.path.join(ROOT, "psgpu_amd", "_lib", "libpsg_legacy_%s.so" % flavour)
@@ -85,3 +86,5 @@ def test_legacy_libraries_export_reference_symbols(flavour):
     for sym in ("timestep_cuda_", "set_gpu_", "timestep_cuda_%s" % flavour):
         assert hasattr(lib, sym), sym
     assert declared == []   # the legacy header declares no psg_* symbols
+    if flavour == "bous_gmres":
+        assert hasattr(lib, "timederiv_cuda_")
