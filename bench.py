@@ -46,9 +46,12 @@ def model_bytes_step(n, b):
     return 2 * rhs + stage
 
 
-def time_form(P, st0, n, precision, steps, warmup, profile):
+def time_form(P, st0, n, precision, steps, warmup, profile, tune=True):
     s = P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov", precision=precision)
     s.set_state(st0)
+    tuned = s.autotune(steps=4) if tune else None
+    if tuned:
+        log("bench: autotune %s:\n%s" % (precision, tuned.rstrip()))
     s.run(warmup)
     s.sync()
     if profile:
@@ -59,8 +62,9 @@ def time_form(P, st0, n, precision, steps, warmup, profile):
     dt = time.perf_counter() - t0
     prof = s.profile_report() if profile else []
     e, z, _ = s.diag()
+    variant = s.get_variant()
     s.close()
-    return dt, prof, e
+    return dt, prof, e, variant
 
 
 def cpu_baseline(n, steps):
@@ -86,6 +90,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default kernel shapes (no autotune)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1", file=sys.stderr)
@@ -95,16 +100,16 @@ def main():
     g = P.hgrid("ns3d", n, n, n)
     st0 = P.ic_solenoidal(g, 20240601, 0.5)
     log("bench: ns3d %d^3 fp64, warmup %d, timed %d steps" % (n, a.warmup, a.steps))
-    t64, prof64, e64 = time_form(P, st0, n, "fp64", a.steps, a.warmup, True)
+    t64, prof64, e64, var64 = time_form(P, st0, n, "fp64", a.steps, a.warmup, True, not a.no_tune)
     sps64 = a.steps / t64
     log("bench: fp64 %.2f steps/s" % sps64)
     fp32 = None
     if not a.no_fp32:
         log("bench: ns3d %d^3 fp32-bulk" % n)
-        t32, prof32, e32 = time_form(P, st0, n, "fp32", a.steps, a.warmup, True)
+        t32, prof32, e32, var32 = time_form(P, st0, n, "fp32", a.steps, a.warmup, True, not a.no_tune)
         sps32 = a.steps / t32
         fp32 = {"steps_per_s": sps32, "ms_per_step": 1e3 * t32 / a.steps, "speedup_vs_fp64": sps32 / sps64,
-                "energy_rel_err_vs_fp64_gpu": abs(e32 - e64) / e64, "steps": a.steps,
+                "energy_rel_err_vs_fp64_gpu": abs(e32 - e64) / e64, "steps": a.steps, "xpass_variant": var32,
                 "kernels": prof32}
         log("bench: fp32-bulk %.2f steps/s (x%.2f)" % (sps32, sps32 / sps64))
     # dominant kernel of the FP64 step
@@ -126,7 +131,7 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "ns3d_heun_cn_%d3_kolmogorov" % n, "grid": [n, n, n], "nu": 0.01, "dt": 0.002,
                       "forcing": "sin(4y) x", "ic": "solenoidal E(k)~k^4exp(-2(k/4)^2), E=0.5, seed 20240601",
-                      "members": 1},
+                      "members": 1, "xpass_variant": var64},
            "roofline": roof, "model": model, "fp32_bulk": fp32, "kernels_fp64": prof64, "cpu_baseline": cpu}
     print(json.dumps(out))
 

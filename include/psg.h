@@ -110,6 +110,22 @@ const char *psg_last_error(void);
 /* number of visible devices (0 when none) */
 int psg_device_count(void);
 
+/* ---------- kernel shapes and autotuning ----------
+ * Shape variants of a pass (launch geometry, points per thread, register
+ * staging); results are identical up to FP rounding of the transform order.
+ * x-pass of the 3D Navier-Stokes and Euler right-hand side, variants 0..4:
+ *   0  8 points/thread, inputs front-loaded, product kept for all outputs
+ *   1  8 points/thread, inputs front-loaded, product re-evaluated per output pair
+ *   2  8 points/thread, inputs loaded per pair, product per output pair
+ *   3  4 points/thread, inputs front-loaded, product kept
+ *   4  4 points/thread, inputs loaded per pair, product per output pair
+ * psg_autotune times `steps` steps of every variant on the context's own state
+ * (restored afterwards), keeps the fastest and writes a text report. */
+#define PSG_PASS_XPASS 0
+int psg_set_variant(psg_ctx *ctx, int pass, int variant);
+int psg_get_variant(psg_ctx *ctx, int pass);
+int psg_autotune(psg_ctx *ctx, int steps, char *report, int report_len);
+
 /* ---------- timing hooks for measurement ----------
  * Kernel names of one step in launch order and their average duration (ms)
  * measured with HIP events on the context stream over the last psg_run call

@@ -93,6 +93,9 @@ def lib():
     L.psg_stream.argtypes = [_P]
     L.psg_stream.restype = _P
     L.psg_profile_enable.argtypes = [_P, _I]
+    L.psg_set_variant.argtypes = [_P, _I, _I]
+    L.psg_get_variant.argtypes = [_P, _I]
+    L.psg_autotune.argtypes = [_P, _I, C.c_char_p, _I]
     L.psg_profile_get.argtypes = [_P, _I, C.POINTER(C.c_char_p), C.POINTER(_D), C.POINTER(_L), C.POINTER(_I)]
     hg = C.POINTER(HGrid)
     L.psgh_grid3.argtypes = [hg, _I, _I, _I, _D]
@@ -355,6 +358,18 @@ class Stepper:
         res = {f: getattr(r, f) for f, _ in NewtonResult._fields_}
         res["status_name"] = NEWTON_STATUS.get(r.status, "?")
         return z, res
+
+    def set_variant(self, variant, pass_=0):
+        _check(lib().psg_set_variant(self.h, pass_, variant), "psg_set_variant")
+
+    def get_variant(self, pass_=0):
+        return lib().psg_get_variant(self.h, pass_)
+
+    def autotune(self, steps=5):
+        """Time every kernel-shape variant on this context's state, keep the fastest; returns the report."""
+        buf = C.create_string_buffer(2048)
+        _check(lib().psg_autotune(self.h, steps, buf, 2048), "psg_autotune")
+        return buf.value.decode()
 
     def profile(self, on=True):
         _check(lib().psg_profile_enable(self.h, 1 if on else 0), "psg_profile_enable")

@@ -527,6 +527,75 @@ int psg_physical(psg_ctx *c, int m, double *host_out) {
 
 void *psg_stream(psg_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+int psg_set_variant(psg_ctx *c, int pass, int variant) {
+  if (!c) return PSG_EINVAL;
+  if (pass != PSG_PASS_XPASS || variant < 0 || variant >= PSG_XPASS_VARIANTS) {
+    set_error("psg_set_variant: unknown pass %d or variant %d", pass, variant);
+    return PSG_EINVAL;
+  }
+  c->xvar = variant;
+  return 0;
+}
+
+int psg_get_variant(psg_ctx *c, int pass) {
+  if (!c || pass != PSG_PASS_XPASS) return PSG_EINVAL;
+  return c->xvar;
+}
+
+int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
+  if (!c) return PSG_EINVAL;
+  std::string rep;
+  if (c->g.dim != 3 || c->cfg.flavour == PSG_BOUS3D) {
+    rep = "no shape variants for this flavour";
+  } else {
+    if (steps < 1) steps = 1;
+    const size_t sb = sizeof(cpx<double>) * c->nf * c->g.nkt * c->batch;
+    void *save = nullptr;
+    PSG_HIP(hipMalloc(&save, sb));
+    PSG_HIP(hipMemcpyAsync(save, c->state, sb, hipMemcpyDeviceToDevice, c->stream));
+    const double t0 = c->time;
+    const bool prof = c->prof;
+    c->prof = false;
+    hipEvent_t e0, e1;
+    PSG_HIP(hipEventCreate(&e0));
+    PSG_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int bestv = c->xvar, rc = 0;
+    for (int v = 0; v < PSG_XPASS_VARIANTS && rc == 0; v++) {
+      c->xvar = v;
+      if ((rc = ns3d_run(c, 1))) break;
+      hipEventRecord(e0, c->stream);
+      if ((rc = ns3d_run(c, steps))) break;
+      hipEventRecord(e1, c->stream);
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      const double per = ms / steps;
+      char line[96];
+      snprintf(line, sizeof line, "xpass variant %d: %.4f ms/step\n", v, per);
+      rep += line;
+      if (per < best) { best = per; bestv = v; }
+      hipMemcpyAsync(c->state, save, sb, hipMemcpyDeviceToDevice, c->stream);
+    }
+    hipStreamSynchronize(c->stream);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(save);
+    c->time = t0;
+    c->prof = prof;
+    c->xvar = bestv;
+    if (rc) return rc;
+    char line[64];
+    snprintf(line, sizeof line, "chosen xpass variant %d\n", bestv);
+    rep += line;
+  }
+  if (report && report_len > 0) {
+    strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return 0;
+}
+
 int psg_profile_enable(psg_ctx *c, int on) {
   if (!c) return PSG_EINVAL;
   hipStreamSynchronize(c->stream);

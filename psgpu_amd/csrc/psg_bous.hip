@@ -21,8 +21,7 @@ template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
             const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask, int nout) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
@@ -212,7 +211,7 @@ static int launch_zinv_b(Ctx *c, const cpx<double> *st, long st_ms, int umask, i
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_zinv_bous<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, A, 9 * g.zsz, g,
@@ -234,7 +233,7 @@ static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, in
     constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)padded_len<N>() * TE * sizeof(cpx<P>);
+    size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
@@ -249,7 +248,7 @@ template <typename P, class UPD> static int launch_zfwd_b(Ctx *c, const cpx<P> *
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_zfwd<P, N, T, 4, UPD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);

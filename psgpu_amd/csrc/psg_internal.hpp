@@ -25,6 +25,7 @@ struct Grid {
 };
 
 enum { PREC64 = 0, PREC32 = 1 };
+constexpr int PSG_XPASS_VARIANTS = 5;
 
 struct ProfEntry {
   std::string name;
@@ -60,6 +61,8 @@ struct Ctx {
   // combined base + perturbation transforms
   cpx<double> *tl_state = nullptr, *tl_stage = nullptr, *tl_rh = nullptr;
   void *tl_bufA = nullptr, *tl_bufB = nullptr, *tl_bufC = nullptr;
+  // kernel shape variants (psg_autotune / psg_set_variant)
+  int xvar = 0;    // Navier-Stokes x-pass: 0..PSG_XPASS_VARIANTS-1
   // diagnostics scratch
   double *dred = nullptr;
   size_t dred_n = 0;

@@ -82,7 +82,7 @@ template <typename P> static int launch_yinv2(Ctx *c, const cpx<double> *st, cpx
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, 1, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k2_yinv<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, g.nkt, out, 3 * g.ysz, g,
@@ -97,7 +97,7 @@ template <typename P, class UPD> static int launch_yfwd2(Ctx *c, const cpx<P> *i
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, 1, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k2_yfwd<P, N, T, UPD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, 2 * g.ysz, g, twp<P>(c, 1), upd);
@@ -113,7 +113,7 @@ static int launch_xpass2(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int
     constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)padded_len<N>() * TE * sizeof(cpx<P>);
+    size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
@@ -185,8 +185,7 @@ int ns2d_nonlin(Ctx *c, int, const cpx<double> *in_dev, cpx<double> *out_dev) {
 template <typename P, int N, int TE>
 __global__ void __launch_bounds__(TE *(N / EPT))
 k2_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, long nrow, Grid g, const cpx<P> *__restrict__ tw) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TPX = N / EPT;
   const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
   const long row = (long)blockIdx.x * TE + team;
@@ -223,7 +222,7 @@ template <typename P> static int physical2_t(Ctx *c, int member, double *out) {
   PSG_DISPATCH_N(g.nx, {
     constexpr int TE = Row<N>::TEAMS;
     dim3 grid((unsigned)((nrow + TE - 1) / TE), 3);
-    size_t lds = (size_t)N * TE * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * TE + N) * sizeof(cpx<P>);
     auto k = k2_xphys<P, N, TE>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, B, out, nrow, g, twp<P>(c, 0));
@@ -288,7 +287,7 @@ template <typename P, class UPD> static int rhs2_tl(Ctx *c, UPD upd, int nm) {
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, 1, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k2_yinv<P, N, T, 2>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, c->stage, g.nkt, B, 6 * g.ysz, g,
@@ -299,7 +298,7 @@ template <typename P, class UPD> static int rhs2_tl(Ctx *c, UPD upd, int nm) {
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, 1, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k2_yfwd<P, N, T, UPD, 4>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, Cb, 4 * g.ysz, g, twp<P>(c, 1), upd);

@@ -90,7 +90,7 @@ template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, i
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, g.nz, nmf);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_yinv<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1));
@@ -104,7 +104,7 @@ template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, i
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, g.nz, nmf);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_yfwd<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1));
@@ -117,6 +117,19 @@ template int launch_yinv_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int);
 template int launch_yinv_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int);
 template int launch_yfwd_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int);
 template int launch_yfwd_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int);
+
+// x-pass shapes of the Navier-Stokes product, chosen per context (psg_autotune):
+// points per thread, input front-loading, per-pair product re-evaluation
+template <typename P> static int xpass_ns(Ctx *c, const cpx<P> *B, cpx<P> *Cb, int nm) {
+  const long nr = (long)c->g.ny * c->g.nz;
+  switch (c->xvar) {
+    case 1: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true>(c, B, Cb, nr, nm, ProdNS<P>{});
+    case 2: return launch_xpass<P, 6, 3, ProdNS<P>, 8, false, true>(c, B, Cb, nr, nm, ProdNS<P>{});
+    case 3: return launch_xpass<P, 6, 3, ProdNS<P>, 4, true, false>(c, B, Cb, nr, nm, ProdNS<P>{});
+    case 4: return launch_xpass<P, 6, 3, ProdNS<P>, 4, false, true>(c, B, Cb, nr, nm, ProdNS<P>{});
+    default: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, false>(c, B, Cb, nr, nm, ProdNS<P>{});
+  }
+}
 
 // One right-hand side evaluation for all members, ending in functor upd.
 template <typename P, class UPD>
@@ -132,7 +145,7 @@ static int rhs3(Ctx *c, const cpx<double> *st, long st_ms, int umask, UPD upd, l
   PSG_TRY(launch_yinv_g<P>(c, A, B, 6 * nm));
   prof_end(c);
   prof_begin(c, "xpass", pb[2] * nm);
-  PSG_TRY((launch_xpass<P, 6, 3>(c, B, Cb, (long)g.ny * g.nz, nm, ProdNS<P>{})));
+  PSG_TRY(xpass_ns<P>(c, B, Cb, nm));
   prof_end(c);
   prof_begin(c, "yfwd", pb[3] * nm);
   PSG_TRY(launch_yfwd_g<P>(c, Cb, A, 3 * nm));
@@ -211,8 +224,7 @@ template <typename P, int N, int TE>
 __global__ void __launch_bounds__(TE *(N / EPT))
 k_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, int nf, long nrow, Grid g,
         const cpx<P> *__restrict__ tw) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TPX = N / EPT;
   const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
   const long row = (long)blockIdx.x * TE + team;
@@ -244,7 +256,7 @@ template <typename P> int launch_xphys(Ctx *c, const cpx<P> *B, double *out, int
   PSG_DISPATCH_N(g.nx, {
     constexpr int TE = Row<N>::TEAMS;
     dim3 grid((unsigned)((nrow + TE - 1) / TE), nf);
-    size_t lds = (size_t)N * TE * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * TE + N) * sizeof(cpx<P>);
     auto k = k_xphys<P, N, TE>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, B, out, nf, nrow, g, twp<P>(c, 0));

@@ -47,7 +47,7 @@ static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, l
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     if (bs) {
       auto k = k_zinv<P, N, T, true>;
       PSG_HIP(lds_attr(k, lds));
@@ -65,7 +65,7 @@ static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, l
 }
 
 
-template <typename P, int FI, int FO, class PROD, int E = EPT, bool FRONT = true>
+template <typename P, int FI, int FO, class PROD, int E = EPT, bool FRONT = true, bool LAZY = false>
 static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nx, {
@@ -73,8 +73,8 @@ static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int 
     constexpr int TE = (256 / TPX) < 1 ? 1 : 256 / TPX;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)padded_len<N>() * TE * sizeof(cpx<P>);
-    auto k = k_xpass<P, N, TE, FI, FO, PROD, E, FRONT>;
+    size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);
+    auto k = k_xpass<P, N, TE, FI, FO, PROD, E, FRONT, LAZY>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(TE * TPX), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0), prod);
   });
@@ -87,7 +87,7 @@ template <typename P, int FO, class UPD> static int launch_zfwd(Ctx *c, const cp
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_zfwd<P, N, T, FO, UPD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);

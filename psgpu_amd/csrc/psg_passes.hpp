@@ -26,6 +26,22 @@ namespace psg {
 
 constexpr int EPT = 8;  // points per thread in every pencil transform
 
+// Every transform kernel keeps the N-entry twiddle table of its pencil length
+// at the start of its dynamic LDS: the block copies it from HBM once, and the
+// Stockham stages then read twiddles at LDS latency instead of through the
+// vector memory pipeline.  The kernel's data region starts N entries later
+// (launches add N*sizeof(cpx<P>) to the LDS size).
+template <typename P, int N>
+__device__ __forceinline__ const cpx<P> *stage_twiddles(const cpx<P> *__restrict__ tw, cpx<P> *lds_tw) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) lds_tw[i] = tw[i];
+  __syncthreads();
+  return lds_tw;
+}
+#define PSG_LDS_TWIDDLES(P, N)                                          \
+  extern __shared__ char smem_raw[];                                   \
+  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw) + (N);            \
+  tw = stage_twiddles<P, N>(tw, reinterpret_cast<cpx<P> *>(smem_raw))
+
 template <typename P> __device__ __forceinline__ cpx<P> to(cpx<double> a) { return {(P)a.x, (P)a.y}; }
 __device__ __forceinline__ cpx<double> up(cpx<double> a) { return a; }
 __device__ __forceinline__ cpx<double> up(cpx<float> a) { return {(double)a.x, (double)a.y}; }
@@ -80,8 +96,7 @@ template <typename P, int N, int T, bool BS>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
        const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
@@ -132,8 +147,7 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
 template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, z = blockIdx.y, mf = blockIdx.z;
@@ -158,8 +172,7 @@ k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
 template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, z = blockIdx.y, mf = blockIdx.z;
@@ -184,8 +197,7 @@ k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
 template <typename P, int N, int T, int FO, class UPD>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   // up to 4 fields every input is requested before the first transform; the
   // 6-field tangent-linear/adjoint updates load each field before its transform
@@ -271,12 +283,14 @@ template <typename P, int N, int E> struct XRow {
   }
 };
 
-template <typename P, int N, int TEAMS, int FI, int FO, class PROD, int E = EPT, bool FRONT = true>
+// LAZY: the product is re-evaluated per output pair instead of being kept
+// for all outputs, which frees 2*FO*E registers for occupancy.
+template <typename P, int N, int TEAMS, int FI, int FO, class PROD, int E = EPT, bool FRONT = true,
+          bool LAZY = false>
 __global__ void __launch_bounds__(TEAMS *(N / E))
 k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
         const cpx<P> *__restrict__ tw, PROD prod) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TPX = N / E;
   constexpr bool WAVE = TPX <= 64;
   constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2;
@@ -310,23 +324,39 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
 #pragma unroll
     for (int mm = 0; mm < E; mm++) { phys[fa][mm] = v[mm].x; phys[fb][mm] = v[mm].y; }
   }
-  P res[2 * PO][E];
-#pragma unroll
-  for (int mm = 0; mm < E; mm++) {
-    P pin[2 * PI], pout[2 * PO];
+  constexpr int NR = LAZY ? 1 : 2 * PO;
+  P res[NR][E];
+  auto product = [&](int mm, P (&pout)[2 * PO]) {
+    P pin[2 * PI];
 #pragma unroll
     for (int f = 0; f < 2 * PI; f++) pin[f] = phys[f][mm];
 #pragma unroll
     for (int f = 0; f < 2 * PO; f++) pout[f] = 0;
     prod(pin, pout);
+  };
+  if constexpr (!LAZY) {
 #pragma unroll
-    for (int f = 0; f < 2 * PO; f++) res[f][mm] = pout[f];
+    for (int mm = 0; mm < E; mm++) {
+      P pout[2 * PO];
+      product(mm, pout);
+#pragma unroll
+      for (int f = 0; f < 2 * PO; f++) res[f][mm] = pout[f];
+    }
   }
+  (void)res;
 #pragma unroll
   for (int q = 0; q < PO; q++) {
     cpx<P> v[E];
 #pragma unroll
-    for (int mm = 0; mm < E; mm++) v[mm] = cpx<P>{res[2 * q][mm], res[2 * q + 1][mm]};
+    for (int mm = 0; mm < E; mm++) {
+      if constexpr (LAZY) {
+        P pout[2 * PO];
+        product(mm, pout);
+        v[mm] = cpx<P>{pout[2 * q], pout[2 * q + 1]};
+      } else {
+        v[mm] = cpx<P>{res[2 * q][mm], res[2 * q + 1][mm]};
+      }
+    }
     fft_pencil<P, N, E, false, WAVE, true>(v, tl, j, 1, 0, tw);
     team_sync<WAVE>();
 #pragma unroll
@@ -388,8 +418,7 @@ __global__ void __launch_bounds__(T *(N / EPT))
 k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out, long out_ms, Grid g,
         const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask,
         const cpx<double> *__restrict__ st2 = nullptr) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, m = blockIdx.z;
@@ -448,8 +477,7 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
 template <typename P, int N, int T, class UPD, int NF = 2>
 __global__ void __launch_bounds__(T *(N / EPT))
 k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, m = blockIdx.z;

@@ -73,8 +73,7 @@ template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv_curl(const cpx<double> *__restrict__ st, long fs, cpx<P> *__restrict__ out, Grid g,
             const cpx<P> *__restrict__ tw, const double *__restrict__ tab, double fac) {
-  extern __shared__ char smem_raw[];
-  cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw);
+  PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const int ikx = blockIdx.x * T + col, iky = blockIdx.y;
@@ -234,7 +233,7 @@ static int rhs_adj(Ctx *c, const cpx<double> *w, const cpx<double> *v, const dou
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, 1);
-    size_t lds = (size_t)N * T * sizeof(cpx<P>);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_zinv_curl<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, v, g.nkt, A + 6 * g.zsz, g, twp<P>(c, 2),

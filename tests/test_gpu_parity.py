@@ -208,3 +208,30 @@ def test_bous_aspect_batch(gpu):
         s.run(10)
         for m in range(2):
             assert rel(s.get_state(m), orc.run(sts[m], 10)) < 1e-11
+
+
+@pytest.mark.parametrize("variant", range(5))
+def test_ns3d_xpass_variants_match_oracle(gpu, variant):
+    n = 32
+    st = O.ic_solenoidal(O.grid3(n, n, n), 24)
+    want = O.NS3D(n, v2=0.01, dt=0.002, force=3).run(st, 20)
+    with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, precision="fp64") as s:
+        s.set_variant(variant)
+        assert s.get_variant() == variant
+        s.set_state(st)
+        s.run(20)
+        assert rel(s.get_state(), want) < 1e-11
+
+
+def test_autotune_keeps_state_and_picks_a_variant(gpu):
+    n = 64
+    st = O.ic_solenoidal(O.grid3(n, n, n), 25)
+    with P.Stepper(flavour="ns3d", n=n) as s:
+        s.set_state(st)
+        rep = s.autotune(steps=2)
+        assert "chosen xpass variant" in rep
+        assert 0 <= s.get_variant() < 5
+        assert np.array_equal(s.get_state(), st)
+        s.run(10)
+        got = s.get_state()
+    assert rel(got, O.NS3D(n, v2=0.01, dt=0.002, force=3).run(st, 10)) < 1e-11
