@@ -66,6 +66,16 @@ _lib = None
 _P, _D, _I, _L = C.c_void_p, C.c_double, C.c_int, C.c_long
 
 
+class KV(C.Structure):
+    """psgh_kv: labels and values of one parameter file (include/psg_host.h)."""
+    _fields_ = [("n", C.c_int), ("key", (C.c_char * 40) * 48), ("kind", C.c_char * 48), ("ival", C.c_long * 48),
+                ("fval", C.c_double * 48), ("sval", (C.c_char * 257) * 48)]
+
+
+PARAMS_LAYOUTS = {"ns3d_dns": 0, "ns3d_gmres": 1, "euler": 2, "bous_dns": 3, "bous_gmres": 4, "ns2d_dns": 5,
+                  "ns2d_gmres": 6, "extra_ns2d": 7, "extra_ns3d": 8, "extra_bous": 9}
+
+
 def lib():
     """Load libpsg.so (raises PsgError if it has not been built)."""
     global _lib
@@ -112,6 +122,12 @@ def lib():
     L.psgh_ic_ns2d_ref.argtypes = [hg, _I, _D, _P]
     L.psgh_ic_solenoidal.argtypes = [hg, C.c_uint64, _D, _D, _P, _P, _P, _P]
     L.psgh_ic_ns2d_cfg.argtypes = [hg, C.c_uint64, _D, _P]
+    L.psgh_params_read.argtypes = [C.c_char_p, _I, C.POINTER(KV), C.c_char_p, _I]
+    L.psgh_zk_header.argtypes = [_P, _I, _I, _I, _I]
+    L.psgh_zk_record.argtypes = [_P, _P, _I, _P, _L]
+    L.psgh_rec_read.argtypes = [_P, _P, _L]
+    L.psgh_rec_read.restype = _L
+    L.psgh_raw_write.argtypes = [C.c_char_p, _P, _L]
     L.psgh_fortran_f.argtypes = [C.c_char_p, _I, _I]
     L.psgh_fortran_f.restype = _D
     L.psg_tangent_steps.argtypes = [_P, _P, _P, _I, _P, _P]
@@ -196,6 +212,23 @@ def ic_ns2d_cfg(g, seed, ampv=1.0):
     z = np.zeros((1, g.nkt), np.complex128)
     lib().psgh_ic_ns2d_cfg(C.byref(g), seed, ampv, _ptr(z[0]))
     return z
+
+
+def read_params(path, layout):
+    """Read a reference parameter file with the reader of `layout` (PARAMS_LAYOUTS key);
+    returns {label: value} in file order.  Raises ValueError on a label mismatch."""
+    kv = KV()
+    msg = C.create_string_buffer(256)
+    rc = lib().psgh_params_read(str(path).encode(), PARAMS_LAYOUTS.get(layout, layout), C.byref(kv), msg, 256)
+    if rc != 0:
+        raise ValueError(msg.value.decode())
+    out = {}
+    for i in range(kv.n):
+        k = kv.key[i].value.decode().rstrip(" =").rstrip()
+        t = kv.kind[i:i + 1].decode()
+        out[k] = {"I": lambda: kv.ival[i], "L": lambda: bool(kv.ival[i]), "F": lambda: kv.fval[i],
+                  "A": lambda: kv.sval[i].value.decode()}[t]()
+    return out
 
 
 # ------------------------------------------------------------------ stepper
