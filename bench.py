@@ -24,10 +24,13 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+METRIC = "3D NS 256\u00b3 timesteps/s FP64 vs FP32-bulk, %HBM roofline, Newton s/orbit"
 
 
 def log(*a):
@@ -46,7 +49,9 @@ def model_bytes_step(n, b):
     return 2 * rhs + stage
 
 
-def time_form(P, st0, n, precision, steps, warmup, profile, tune=True):
+def time_form(P, st0, n, precision, steps, warmup, tune=True, prof_steps=20):
+    """Timed steps (captured step graphs, no per-kernel events), then a separate
+    profiled run whose HIP events give each kernel's mean launch time."""
     s = P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov", precision=precision)
     s.set_state(st0)
     tuned = s.autotune(steps=4) if tune else None
@@ -54,17 +59,130 @@ def time_form(P, st0, n, precision, steps, warmup, profile, tune=True):
         log("bench: autotune %s:\n%s" % (precision, tuned.rstrip()))
     s.run(warmup)
     s.sync()
-    if profile:
-        s.profile(True)
     t0 = time.perf_counter()
     s.run(steps)
     s.sync()
     dt = time.perf_counter() - t0
-    prof = s.profile_report() if profile else []
     e, z, _ = s.diag()
+    prof = []
+    if prof_steps:
+        s.profile(True)
+        s.run(prof_steps)
+        s.sync()
+        prof = s.profile_report()
+        s.profile(False)
     variant = s.get_variant()
     s.close()
     return dt, prof, e, variant
+
+
+def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-6):
+    """Gradient of J = ||Phi_T(u0) - u0||^2 at 128^3, 200 steps, checkpoints every 20,
+    against central differences along seeded solenoidal directions (FD with the
+    FP64 stepper); FP64 and FP32-bulk adjoints."""
+    g = P.hgrid("ns3d", n, n, n)
+    L = P.tables(g)[3].astype(float)
+    z0 = P.ic_solenoidal(g, 20240603, 0.5)
+    out = {"grid": [n, n, n], "steps": nsteps, "dt": 0.002, "ckpt": ckpt, "directions": ndir, "eps": eps}
+    with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov") as s:
+        def J(x):
+            s.set_state(x)
+            s.run(nsteps)
+            r = (s.get_state() - x) * L[None, :]
+            return float(np.sum(np.abs(r) ** 2))
+        s.residual_grad(z0, nsteps, ckpt)       # warm
+        s.sync()
+        t0 = time.perf_counter()
+        J0, g64, _ = s.residual_grad(z0, nsteps, ckpt)
+        out["fp64_grad_seconds"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        J(z0)
+        out["forward_seconds"] = time.perf_counter() - t0
+        errs64, errs32, fds = [], [], []
+        with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov", precision="fp32") as s32:
+            s32.residual_grad(z0, nsteps, ckpt)
+            t0 = time.perf_counter()
+            _, g32, _ = s32.residual_grad(z0, nsteps, ckpt)
+            out["fp32_grad_seconds"] = time.perf_counter() - t0
+        for k in range(ndir):
+            v = P.ic_solenoidal(g, 777000 + k, 0.5)
+            v /= np.sqrt(np.sum(np.abs(v) ** 2))
+            fd = (J(z0 + eps * v) - J(z0 - eps * v)) / (2 * eps)
+            ip = lambda a: float(np.sum((np.conj(a) * v).real * L[None, :]))
+            fds.append(fd)
+            errs64.append(abs(ip(g64) - fd) / abs(fd))
+            errs32.append(abs(ip(g32) - fd) / abs(fd))
+    out.update({"J": J0, "fp64_max_rel_err_vs_fd": max(errs64), "fp32_max_rel_err_vs_fd": max(errs32),
+                "fp64_median_rel_err_vs_fd": float(np.median(errs64))})
+    return out
+
+
+def newton_case(P, n=512, t_spin=100.0, t_window=60.0, dt_sample=0.25, nguess=2, max_newton=30,
+                tmin=5.0, tmax=20.0):
+    """2D Kolmogorov flow n^2, Re=40, sin(4y) forcing, dt=0.005: spin up a seeded
+    random-phase state, take the closest shift-minimised near-recurrences with
+    T in [tmin, tmax] as guesses, and run Newton-GMRES-hookstep (Krylov 40,
+    residual 1e-10) on each with finite-difference Jv and with tangent-linear Jv."""
+    from psgpu_amd import recurrence as R
+    g = P.hgrid("ns2d", n, n)
+    L = P.tables(g)[3]
+    z0 = P.ic_ns2d_cfg(g, 20240607, 1.0)
+    out = {"grid": [n, n], "re": 40.0, "dt": 0.005, "t_spin": t_spin, "window": t_window, "krylov": 40,
+           "netol": 1e-10, "max_newton": max_newton}
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / 40.0, dt=0.005, kf=4.0, ampfor=-2.0) as s:
+        t0 = time.perf_counter()
+        times, snaps = R.record(s, z0, t_spin, t_window, dt_sample)
+        out["record_seconds"] = time.perf_counter() - t0
+        out["dns_steps_per_s"] = (t_spin + t_window) / 0.005 / out["record_seconds"]
+        hits = R.best_recurrences(times, snaps, g.iktx, L, tmin, tmax, top=nguess)
+        out["guesses"] = hits
+        runs = []
+        for h in hits:
+            for jv in (0, 1):
+                z, res = s.newton(snaps[h["i"]][None], T=h["T"], sx=h["shift"], jv=jv, netol=1e-10,
+                                  max_gmres=40, max_newton=max_newton, dt0=0.005)
+                res["jv"] = "tangent-linear" if jv else "finite-difference"
+                res["guess"] = h["i"]
+                runs.append(res)
+                log("bench: newton guess t=%.2f T=%.3f %s: %s ner=%.2e in %.1f s" %
+                    (h["t"], h["T"], res["jv"], res["status_name"], res["ner"], res["seconds"]))
+        out["runs"] = runs
+        for jv in ("finite-difference", "tangent-linear"):
+            rs = [r for r in runs if r["jv"] == jv]
+            conv = [r for r in rs if r["status_name"] == "CONVERGED"]
+            out[jv] = {"converged": len(conv), "tried": len(rs), "seconds_total": sum(r["seconds"] for r in rs),
+                       "s_per_orbit": (sum(r["seconds"] for r in conv) / len(conv)) if conv else None}
+    return out
+
+
+def bous_ensemble(P, n=512, members=4, steps=6, warmup=2):
+    """3D Boussinesq n^3, N=Ri=1, Pr=1, nu=kappa=1/400, Kolmogorov forcing, RK4 with
+    per-stage CN, dt=0.002, theta=90 (radians, as the reference passes it): `members`
+    seeded ensemble members stepped together, captured step graphs, FP64 and FP32-bulk."""
+    g = P.hgrid("bous", n, n, n)
+    out = {"grid": [n, n, n], "members": members, "steps": steps, "dt": 0.002, "nu": 1 / 400.0, "ri": 1.0,
+           "sc": 1.0, "theta": 90.0}
+    sts = [P.ic_solenoidal(g, 20240605 + m, 0.5, 0.1) for m in range(members)]
+    for prec in ("fp64", "fp32"):
+        with P.Stepper(flavour="bous", n=n, v2=1 / 400.0, sc=1.0, ri=1.0, theta=90.0, dt=0.002, batch=members,
+                       force="kolmogorov", precision=prec) as s:
+            s.set_graph(2)
+            for m in range(members):
+                s.set_state(sts[m], m)
+            s.run(warmup)
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(steps)
+            s.sync()
+            dt = time.perf_counter() - t0
+            e = [s.diag(m)[0] for m in range(members)]
+        out[prec] = {"ensemble_steps_per_s": steps / dt, "member_steps_per_s": members * steps / dt,
+                     "ms_per_step": 1e3 * dt / steps, "energy": e}
+        log("bench: bous %d^3 x%d %s %.2f ensemble steps/s" % (n, members, prec, steps / dt))
+    out["fp32_speedup"] = out["fp32"]["ensemble_steps_per_s"] / out["fp64"]["ensemble_steps_per_s"]
+    out["fp32_energy_rel_err_vs_fp64"] = max(abs(a - b) / b for a, b in zip(out["fp32"]["energy"],
+                                                                             out["fp64"]["energy"]))
+    return out
 
 
 def cpu_baseline(n, steps):
@@ -91,6 +209,11 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel shapes (no autotune)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^3 adjoint-gradient case")
+    ap.add_argument("--no-bous", action="store_true", help="skip the 512^3 Boussinesq ensemble case")
+    ap.add_argument("--newton", type=int, default=0, metavar="NGUESS",
+                    help="run the 2D 512^2 Newton case (TL vs FD Jv) on NGUESS recurrence guesses")
+    ap.add_argument("--newton-n", type=int, default=512)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1", file=sys.stderr)
@@ -100,13 +223,13 @@ def main():
     g = P.hgrid("ns3d", n, n, n)
     st0 = P.ic_solenoidal(g, 20240601, 0.5)
     log("bench: ns3d %d^3 fp64, warmup %d, timed %d steps" % (n, a.warmup, a.steps))
-    t64, prof64, e64, var64 = time_form(P, st0, n, "fp64", a.steps, a.warmup, True, not a.no_tune)
+    t64, prof64, e64, var64 = time_form(P, st0, n, "fp64", a.steps, a.warmup, not a.no_tune)
     sps64 = a.steps / t64
     log("bench: fp64 %.2f steps/s" % sps64)
     fp32 = None
     if not a.no_fp32:
         log("bench: ns3d %d^3 fp32-bulk" % n)
-        t32, prof32, e32, var32 = time_form(P, st0, n, "fp32", a.steps, a.warmup, True, not a.no_tune)
+        t32, prof32, e32, var32 = time_form(P, st0, n, "fp32", a.steps, a.warmup, not a.no_tune)
         sps32 = a.steps / t32
         fp32 = {"steps_per_s": sps32, "ms_per_step": 1e3 * t32 / a.steps, "speedup_vs_fp64": sps32 / sps64,
                 "energy_rel_err_vs_fp64_gpu": abs(e32 - e64) / e64, "steps": a.steps, "xpass_variant": var32,
@@ -122,17 +245,32 @@ def main():
                 "avg_launch_ms": dom["avg_ms"]}
     mb = model_bytes_step(n, 16)
     model = {"gb_per_step": mb / 1e9, "achieved_gbs": mb * sps64 / 1e9, "frac": mb * sps64 / 1e9 / PEAK_HBM_GBS}
+    adj = None
+    if not a.no_adjoint:
+        log("bench: adjoint gradient 128^3, 200 steps")
+        adj = adjoint_case(P)
+        log("bench: adjoint fp64 %.3f s, max rel err vs FD %.2e" % (adj["fp64_grad_seconds"],
+                                                                     adj["fp64_max_rel_err_vs_fd"]))
+    bous = None
+    if not a.no_bous:
+        log("bench: boussinesq 512^3 ensemble")
+        bous = bous_ensemble(P)
+    newton = None
+    if a.newton:
+        log("bench: newton 2D %d^2, %d guesses" % (a.newton_n, a.newton))
+        newton = newton_case(P, n=a.newton_n, nguess=a.newton)
     cpu = None
     if not a.no_cpu:
         log("bench: cpu baseline (%d steps)" % a.cpu_steps)
         cpu = cpu_baseline(n, a.cpu_steps)
-    out = {"metric": "3D NS %d^3 timesteps/s (FP64 plain; FP32-bulk beside)" % n, "value": sps64,
+    out = {"metric": METRIC, "value": sps64,
            "unit": "steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * t64 / a.steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "ns3d_heun_cn_%d3_kolmogorov" % n, "grid": [n, n, n], "nu": 0.01, "dt": 0.002,
                       "forcing": "sin(4y) x", "ic": "solenoidal E(k)~k^4exp(-2(k/4)^2), E=0.5, seed 20240601",
                       "members": 1, "xpass_variant": var64},
-           "roofline": roof, "model": model, "fp32_bulk": fp32, "kernels_fp64": prof64, "cpu_baseline": cpu}
+           "roofline": roof, "model": model, "fp32_bulk": fp32, "kernels_fp64": prof64,
+           "adjoint_128": adj, "bous_ensemble_512": bous, "newton_2d": newton, "cpu_baseline": cpu}
     print(json.dumps(out))
 
 

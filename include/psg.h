@@ -122,6 +122,12 @@ int psg_device_count(void);
  * psg_autotune times `steps` steps of every variant on the context's own state
  * (restored afterwards), keeps the fastest and writes a text report. */
 #define PSG_PASS_XPASS 0
+/* Step graphs: psg_run launches its first step directly, then replays a
+ * captured HIP graph of `steps` steps (default 16) as often as fits, then
+ * launches the remainder directly; results are bitwise those of direct
+ * launches.  steps = 0 disables capture.  Graphs are rebuilt after psg_set_dt
+ * or a variant change, and bypassed while profiling is on. */
+int psg_set_graph(psg_ctx *ctx, int steps);
 int psg_set_variant(psg_ctx *ctx, int pass, int variant);
 int psg_get_variant(psg_ctx *ctx, int pass);
 int psg_autotune(psg_ctx *ctx, int steps, char *report, int report_len);

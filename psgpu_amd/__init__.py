@@ -104,6 +104,7 @@ def lib():
     L.psg_stream.restype = _P
     L.psg_profile_enable.argtypes = [_P, _I]
     L.psg_set_variant.argtypes = [_P, _I, _I]
+    L.psg_set_graph.argtypes = [_P, _I]
     L.psg_get_variant.argtypes = [_P, _I]
     L.psg_autotune.argtypes = [_P, _I, C.c_char_p, _I]
     L.psg_profile_get.argtypes = [_P, _I, C.POINTER(C.c_char_p), C.POINTER(_D), C.POINTER(_L), C.POINTER(_I)]
@@ -391,6 +392,10 @@ class Stepper:
         res = {f: getattr(r, f) for f, _ in NewtonResult._fields_}
         res["status_name"] = NEWTON_STATUS.get(r.status, "?")
         return z, res
+
+    def set_graph(self, steps):
+        """Steps per captured HIP graph in run() (0: direct launches only)."""
+        _check(lib().psg_set_graph(self.h, steps), "psg_set_graph")
 
     def set_variant(self, variant, pass_=0):
         _check(lib().psg_set_variant(self.h, pass_, variant), "psg_set_variant")

@@ -213,6 +213,8 @@ static int upload_twiddles(Ctx *c) {
 
 static void free_ctx(Ctx *c) {
   if (!c) return;
+  for (auto &e : c->graph_exec)
+    if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
   void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
                   c->tl_rh, c->tl_bufA, c->tl_bufB, c->tl_bufC, c->bufA, c->bufB, c->bufC,
                   c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred};
@@ -241,6 +243,58 @@ int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out) {
                      in, out, c->nu, c->mask, c->g.nkt, c->nf);
   PSG_HIP(hipGetLastError());
   return 0;
+}
+
+static int run_eager(Ctx *c, int kind, int n, bool first) {
+  if (n <= 0) return 0;
+  if (kind == 1) return c->g.dim == 2 ? ns2d_tl_run(c, n, first) : ns3d_tl_run(c, n, first);
+  if (c->g.dim == 2) return ns2d_run(c, n, first);
+  return c->cfg.flavour == PSG_BOUS3D ? bous_run(c, n, first) : ns3d_run(c, n, first);
+}
+
+// Capture graph_steps steps (all launches, no memsets or host work) into an
+// executable graph for `kind`; a failed capture switches the context to
+// eager launches for good.
+static int build_graph(Ctx *c, int kind) {
+  if (c->graph_exec[kind]) {
+    (void)hipGraphExecDestroy(c->graph_exec[kind]);
+    c->graph_exec[kind] = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1;
+  const int r = run_eager(c, kind, c->graph_steps, false);
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (r || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return -1;
+  }
+  const hipError_t ei = hipGraphInstantiate(&c->graph_exec[kind], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    c->graph_exec[kind] = nullptr;
+    (void)hipGetLastError();
+    return -1;
+  }
+  c->graph_built[kind] = c->graph_gen;
+  return 0;
+}
+
+int run_steps(Ctx *c, int kind, int nsteps) {
+  if (nsteps <= 0) return 0;
+  const int G = c->graph_steps;
+  if (G <= 0 || c->prof || c->graph_failed || nsteps < G + 1) return run_eager(c, kind, nsteps, true);
+  // the first step of a call is eager: it may mask the velocity / copy the stage input
+  if (int r = run_eager(c, kind, 1, true)) return r;
+  int left = nsteps - 1;
+  if (c->graph_built[kind] != c->graph_gen || !c->graph_exec[kind]) {
+    if (build_graph(c, kind)) {
+      c->graph_failed = true;
+      return run_eager(c, kind, left, false);
+    }
+  }
+  for (; left >= G; left -= G) PSG_HIP(hipGraphLaunch(c->graph_exec[kind], c->stream));
+  return run_eager(c, kind, left, false);
 }
 
 int tl_alloc(Ctx *c) {
@@ -273,7 +327,7 @@ int tl_flow_map(Ctx *c, const cpx<double> *z0, const cpx<double> *dz0, cpx<doubl
   const size_t bytes = sizeof(cpx<double>) * c->nf * c->g.nkt;
   PSG_HIP(hipMemcpyAsync(c->state, z0, bytes, hipMemcpyDeviceToDevice, c->stream));
   PSG_HIP(hipMemcpyAsync(c->tl_state, dz0, bytes, hipMemcpyDeviceToDevice, c->stream));
-  auto run = [&](int n) { return c->g.dim == 2 ? ns2d_tl_run(c, n) : ns3d_tl_run(c, n); };
+  auto run = [&](int n) { return run_steps(c, 1, n); };
   int nstop = (int)(T / dt0), r = 0;
   if (policy == 0) {
     if (nstop < 1) nstop = 1;
@@ -437,7 +491,8 @@ int psg_get_state_device(psg_ctx *c, int m, void *dev) {
 int psg_run(psg_ctx *c, int nsteps) {
   if (!c || nsteps < 0) { set_error("invalid run arguments"); return PSG_EINVAL; }
   if (nsteps == 0) return 0;
-  int r = c->g.dim == 2 ? ns2d_run(c, nsteps) : (c->cfg.flavour == PSG_BOUS3D ? bous_run(c, nsteps) : ns3d_run(c, nsteps));
+  int r = run_steps(c, 0, nsteps);
+  if (!r) c->time += nsteps * c->cfg.dt;
   if (c->prof) prof_collect(c);
   return r;
 }
@@ -446,6 +501,7 @@ int psg_set_dt(psg_ctx *c, double dt) {
   if (!c || !(dt > 0.0)) { set_error("invalid dt"); return PSG_EINVAL; }
   PSG_HIP(hipStreamSynchronize(c->stream));
   c->cfg.dt = dt;
+  c->graph_gen++;   // captured launches hold dt by value
   return upload_tables(c);
 }
 
@@ -534,6 +590,15 @@ int psg_set_variant(psg_ctx *c, int pass, int variant) {
     return PSG_EINVAL;
   }
   c->xvar = variant;
+  c->graph_gen++;
+  return 0;
+}
+
+int psg_set_graph(psg_ctx *c, int steps) {
+  if (!c || steps < 0) { set_error("psg_set_graph: bad argument"); return PSG_EINVAL; }
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  c->graph_steps = steps;
+  c->graph_gen++;
   return 0;
 }
 
@@ -563,9 +628,10 @@ int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
     int bestv = c->xvar, rc = 0;
     for (int v = 0; v < PSG_XPASS_VARIANTS && rc == 0; v++) {
       c->xvar = v;
-      if ((rc = ns3d_run(c, 1))) break;
+      c->graph_gen++;
+      if ((rc = run_steps(c, 0, 1))) break;
       hipEventRecord(e0, c->stream);
-      if ((rc = ns3d_run(c, steps))) break;
+      if ((rc = run_steps(c, 0, steps))) break;
       hipEventRecord(e1, c->stream);
       hipEventSynchronize(e1);
       float ms = 0.f;
@@ -584,6 +650,7 @@ int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
     c->time = t0;
     c->prof = prof;
     c->xvar = bestv;
+    c->graph_gen++;
     if (rc) return rc;
     char line[64];
     snprintf(line, sizeof line, "chosen xpass variant %d\n", bestv);

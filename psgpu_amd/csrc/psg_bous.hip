@@ -285,29 +285,31 @@ static BousPar bpar(const Ctx *c) {
                  c->cfg.ampfor};
 }
 
-template <typename P> static int runb_t(Ctx *c, int nsteps) {
+template <typename P> static int runb_t(Ctx *c, int nsteps, bool first) {
   const Grid &g = c->g;
   const long tot = 4 * g.nkt * c->batch;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)std::min<long>(4096, (tot + 255) / 256)), dim3(256), 0, c->stream,
-                     c->state, c->stage, tot);
-  PSG_HIP(hipGetLastError());
+  if (first) {
+    // stage input = state (setVelocity); later steps leave it equal to the state themselves
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)std::min<long>(4096, (tot + 255) / 256)), dim3(256), 0, c->stream,
+                       c->state, c->stage, tot);
+    PSG_HIP(hipGetLastError());
+  }
   const BousPar p = bpar(c);
   for (int nt = 0; nt < nsteps; nt++) {
     UpdBous<0> u0{c->state, c->stage, c->rh, g.nkt, c->mask, p};
-    PSG_TRYB(rhs_b<P>(c, c->stage, nt == 0 ? 1 : 0, u0, c->batch));
+    PSG_TRYB(rhs_b<P>(c, c->stage, (first && nt == 0) ? 1 : 0, u0, c->batch));
     UpdBous<1> u1{c->state, c->stage, c->rh, g.nkt, c->mask, p};
     PSG_TRYB(rhs_b<P>(c, c->stage, 0, u1, c->batch));
     UpdBous<2> u2{c->state, c->stage, c->rh, g.nkt, c->mask, p};
     PSG_TRYB(rhs_b<P>(c, c->stage, 0, u2, c->batch));
     UpdBous<3> u3{c->state, c->stage, c->rh, g.nkt, c->mask, p};
     PSG_TRYB(rhs_b<P>(c, c->stage, 0, u3, c->batch));
-    c->time += c->cfg.dt;
   }
   return 0;
 }
 
-int bous_run(Ctx *c, int nsteps) {
-  return c->cfg.precision == PSG_FP32BULK ? runb_t<float>(c, nsteps) : runb_t<double>(c, nsteps);
+int bous_run(Ctx *c, int nsteps, bool first) {
+  return c->cfg.precision == PSG_FP32BULK ? runb_t<float>(c, nsteps, first) : runb_t<double>(c, nsteps, first);
 }
 
 template <typename P> static int rhsb_t(Ctx *c, const cpx<double> *in, cpx<double> *out) {

@@ -61,6 +61,13 @@ struct Ctx {
   // combined base + perturbation transforms
   cpx<double> *tl_state = nullptr, *tl_stage = nullptr, *tl_rh = nullptr;
   void *tl_bufA = nullptr, *tl_bufB = nullptr, *tl_bufC = nullptr;
+  // captured step graphs: [0] plain stepper, [1] tangent linear; rebuilt when
+  // graph_gen changes (dt, variants, graph length)
+  int graph_steps = 16;            // steps per captured graph, 0 = eager launches only
+  unsigned graph_gen = 1;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  unsigned graph_built[2] = {0, 0};
+  bool graph_failed = false;       // a capture failed once: eager from then on
   // kernel shape variants (psg_autotune / psg_set_variant)
   int xvar = 0;    // Navier-Stokes x-pass: 0..PSG_XPASS_VARIANTS-1
   // diagnostics scratch
@@ -91,21 +98,24 @@ void prof_end(Ctx *c);
 
 // per-flavour drivers
 int ns3d_init(Ctx *c);
-int ns3d_run(Ctx *c, int nsteps);
+int ns3d_run(Ctx *c, int nsteps, bool first);
 int ns3d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
 int ns3d_physical(Ctx *c, int member, double *out_dev);
 int ns2d_init(Ctx *c);
-int ns2d_run(Ctx *c, int nsteps);
+int ns2d_run(Ctx *c, int nsteps, bool first);
 int ns2d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
 int ns2d_physical(Ctx *c, int member, double *out_dev);
 int diag_member(Ctx *c, int member, double *e, double *z, double *spec);
 int bous_init(Ctx *c);
-int bous_run(Ctx *c, int nsteps);
+int bous_run(Ctx *c, int nsteps, bool first);
 int bous_timederiv(Ctx *c, const cpx<double> *in_dev, cpx<double> *out_dev);
 int bous_physical(Ctx *c, int member, double *out_dev);
 int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out);
-int ns2d_tl_run(Ctx *c, int nsteps);
-int ns3d_tl_run(Ctx *c, int nsteps);
+int ns2d_tl_run(Ctx *c, int nsteps, bool first);
+int ns3d_tl_run(Ctx *c, int nsteps, bool first);
+// nsteps of the plain (kind 0) or tangent-linear (kind 1) stepper: an eager
+// first step, then replays of a captured graph of graph_steps steps
+int run_steps(Ctx *c, int kind, int nsteps);
 int ns3d_step_rec(Ctx *c, int umask, cpx<double> *wn, cpx<double> *wstar);
 int ns3d_adjoint(Ctx *c, const cpx<double> *z0, const cpx<double> *lamT, int nsteps, int ckpt, cpx<double> *zT,
                  cpx<double> *lam0);

@@ -1,5 +1,7 @@
 // Launch helpers: compile-time shapes per pencil length and runtime dispatch.
 #pragma once
+#include <mutex>
+#include <unordered_map>
 #include "psg_passes.hpp"
 
 namespace psg {
@@ -22,11 +24,22 @@ template <typename P> inline const cpx<P> *twp(const Ctx *c, int axis);
 template <> inline const cpx<double> *twp<double>(const Ctx *c, int axis) { return c->tw64[axis]; }
 template <> inline const cpx<float> *twp<float>(const Ctx *c, int axis) { return c->tw32[axis]; }
 
-// raise the dynamic-LDS limit of a kernel once when it needs more than 64 KiB
-template <typename F> inline hipError_t lds_attr(F *kernel, size_t bytes) {
+// raise the dynamic-LDS limit of a kernel when it needs more than 64 KiB; each
+// kernel is raised once per process (the call is then skipped, so it never
+// runs inside a stream capture after the kernel's first direct launch)
+inline hipError_t lds_attr_raw(const void *kernel, size_t bytes) {
   if (bytes <= 65536) return hipSuccess;
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  static std::mutex mu;
+  static std::unordered_map<const void *, size_t> done;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = done.find(kernel);
+  if (it != done.end() && it->second >= bytes) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) done[kernel] = bytes;
+  return e;
+}
+template <typename F> inline hipError_t lds_attr(F *kernel, size_t bytes) {
+  return lds_attr_raw(reinterpret_cast<const void *>(kernel), bytes);
 }
 
 #define PSG_CASE_N(NV, ...) \

@@ -146,12 +146,15 @@ template <typename P, class UPD> static int rhs2(Ctx *c, const cpx<double> *z0, 
   return 0;
 }
 
-template <typename P> static int run2_t(Ctx *c, int nsteps) {
+template <typename P> static int run2_t(Ctx *c, int nsteps, bool first) {
   const Grid &g = c->g;
   const long total = g.nkt * c->batch;
-  hipLaunchKernelGGL(k2_setz0, dim3((unsigned)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096)),
-                     dim3(256), 0, c->stream, c->state, c->stage, c->mask, g.nkt, total);
-  PSG_HIP(hipGetLastError());
+  if (first) {
+    // stage input Z0 = masked state; later steps leave Z0 = state themselves
+    hipLaunchKernelGGL(k2_setz0, dim3((unsigned)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096)),
+                       dim3(256), 0, c->stream, c->state, c->stage, c->mask, g.nkt, total);
+    PSG_HIP(hipGetLastError());
+  }
   const double famp = c->cfg.force == PSG_FORCE_NONE ? 0.0 : c->cfg.ampfor;
   for (int nt = 0; nt < nsteps; nt++) {
     Upd2D<0> u0{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
@@ -162,13 +165,12 @@ template <typename P> static int run2_t(Ctx *c, int nsteps) {
     PSG_TRY2(rhs2<P>(c, c->stage, u2, c->batch));
     Upd2D<3> u3{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
     PSG_TRY2(rhs2<P>(c, c->stage, u3, c->batch));
-    c->time += c->cfg.dt;
   }
   return 0;
 }
 
-int ns2d_run(Ctx *c, int nsteps) {
-  return c->cfg.precision == PSG_FP32BULK ? run2_t<float>(c, nsteps) : run2_t<double>(c, nsteps);
+int ns2d_run(Ctx *c, int nsteps, bool first) {
+  return c->cfg.precision == PSG_FP32BULK ? run2_t<float>(c, nsteps, first) : run2_t<double>(c, nsteps, first);
 }
 
 template <typename P> static int nonlin2_t(Ctx *c, const cpx<double> *in, cpx<double> *out) {
@@ -307,13 +309,16 @@ template <typename P, class UPD> static int rhs2_tl(Ctx *c, UPD upd, int nm) {
   return 0;
 }
 
-template <typename P> static int run2tl_t(Ctx *c, int nsteps) {
+template <typename P> static int run2tl_t(Ctx *c, int nsteps, bool first) {
   const Grid &g = c->g;
   const long total = g.nkt * c->batch;
   const unsigned gb = (unsigned)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
-  hipLaunchKernelGGL(k2_setz0, dim3(gb), dim3(256), 0, c->stream, c->state, c->stage, c->mask, g.nkt, total);
-  hipLaunchKernelGGL(k2_setz0, dim3(gb), dim3(256), 0, c->stream, c->tl_state, c->tl_stage, c->mask, g.nkt, total);
-  PSG_HIP(hipGetLastError());
+  if (first) {
+    hipLaunchKernelGGL(k2_setz0, dim3(gb), dim3(256), 0, c->stream, c->state, c->stage, c->mask, g.nkt, total);
+    hipLaunchKernelGGL(k2_setz0, dim3(gb), dim3(256), 0, c->stream, c->tl_state, c->tl_stage, c->mask, g.nkt,
+                       total);
+    PSG_HIP(hipGetLastError());
+  }
   const double famp = c->cfg.force == PSG_FORCE_NONE ? 0.0 : c->cfg.ampfor;
   for (int nt = 0; nt < nsteps; nt++) {
 #define PSG_TL_STAGE(S)                                                                                    \
@@ -324,13 +329,12 @@ template <typename P> static int run2tl_t(Ctx *c, int nsteps) {
     }
     PSG_TL_STAGE(0) PSG_TL_STAGE(1) PSG_TL_STAGE(2) PSG_TL_STAGE(3)
 #undef PSG_TL_STAGE
-    c->time += c->cfg.dt;
   }
   return 0;
 }
 
-int ns2d_tl_run(Ctx *c, int nsteps) {
-  return c->cfg.precision == PSG_FP32BULK ? run2tl_t<float>(c, nsteps) : run2tl_t<double>(c, nsteps);
+int ns2d_tl_run(Ctx *c, int nsteps, bool first) {
+  return c->cfg.precision == PSG_FP32BULK ? run2tl_t<float>(c, nsteps, first) : run2tl_t<double>(c, nsteps, first);
 }
 
 int ns2d_init(Ctx *c) {

@@ -156,14 +156,14 @@ static int rhs3(Ctx *c, const cpx<double> *st, long st_ms, int umask, UPD upd, l
   return 0;
 }
 
-template <typename P> static int run_t(Ctx *c, int nsteps) {
+template <typename P> static int run_t(Ctx *c, int nsteps, bool first) {
   const Grid &g = c->g;
   const bool euler = c->cfg.flavour == PSG_EULER3D;
   Force F{c->cfg.force, c->cfg.kf, c->cfg.ampfor};
   const long ms = 3 * g.nkt, fs = g.nkt;
   const long sb_pre = 9 * g.nkt * 16, sb_cor = 9 * g.nkt * 16;  // state + RH traffic
   for (int nt = 0; nt < nsteps; nt++) {
-    const int umask = nt == 0 ? 1 : 0;
+    const int umask = (first && nt == 0) ? 1 : 0;
     if (euler) {
       UpdHeun<0, true> up{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
       PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb_pre, c->batch));
@@ -175,13 +175,12 @@ template <typename P> static int run_t(Ctx *c, int nsteps) {
       UpdHeun<1, false> uc{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
       PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb_cor, c->batch));
     }
-    c->time += c->cfg.dt;
   }
   return 0;
 }
 
-int ns3d_run(Ctx *c, int nsteps) {
-  return c->cfg.precision == PSG_FP32BULK ? run_t<float>(c, nsteps) : run_t<double>(c, nsteps);
+int ns3d_run(Ctx *c, int nsteps, bool first) {
+  return c->cfg.precision == PSG_FP32BULK ? run_t<float>(c, nsteps, first) : run_t<double>(c, nsteps, first);
 }
 
 // One Navier-Stokes step of member 0 that records w^n (before the step) and
@@ -198,7 +197,6 @@ template <typename P> static int step_rec_t(Ctx *c, int umask, cpx<double> *wn, 
   if (wstar) PSG_HIP(hipMemcpyAsync(wstar, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
   UpdHeun<1, false> uc{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
   PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, 9 * g.nkt * 16, 1));
-  c->time += c->cfg.dt;
   return 0;
 }
 

@@ -198,27 +198,26 @@ template <typename P, class UPD> static int rhs_tl(Ctx *c, int umask, UPD upd, i
   return 0;
 }
 
-template <typename P> static int tl_run_t(Ctx *c, int nsteps) {
+template <typename P> static int tl_run_t(Ctx *c, int nsteps, bool first) {
   const Grid &g = c->g;
   Force F{c->cfg.force, c->cfg.kf, c->cfg.ampfor};
   const long ms = 3 * g.nkt, fs = g.nkt;
   for (int nt = 0; nt < nsteps; nt++) {
     UpdTL<0> up{c->state, c->rh, c->tl_state, c->tl_rh, ms, fs, c->nuzn, c->nu1, c->mask, F};
-    PSG_TRY(rhs_tl<P>(c, nt == 0 ? 1 : 0, up, c->batch));
+    PSG_TRY(rhs_tl<P>(c, (first && nt == 0) ? 1 : 0, up, c->batch));
     UpdTL<1> uc{c->state, c->rh, c->tl_state, c->tl_rh, ms, fs, c->nuzn, c->nu1, c->mask, F};
     PSG_TRY(rhs_tl<P>(c, 0, uc, c->batch));
-    c->time += c->cfg.dt;
   }
   return 0;
 }
 
-int ns3d_tl_run(Ctx *c, int nsteps) {
+int ns3d_tl_run(Ctx *c, int nsteps, bool first) {
   if (c->cfg.flavour != PSG_NS3D) {
     set_error("3D tangent linear: Navier-Stokes flavour only");
     return PSG_EINVAL;
   }
   if (int r = tl_alloc(c)) return r;
-  return c->cfg.precision == PSG_FP32BULK ? tl_run_t<float>(c, nsteps) : tl_run_t<double>(c, nsteps);
+  return c->cfg.precision == PSG_FP32BULK ? tl_run_t<float>(c, nsteps, first) : tl_run_t<double>(c, nsteps, first);
 }
 
 // one application of C'(w)^T to the zinv input v (scaled by fac*tab), ending in upd
@@ -390,7 +389,7 @@ int psg_tangent_steps(psg_ctx *c, const double *z0, const double *dz0, int nstep
   const size_t sb = sizeof(cpx<double>) * 3 * c->g.nkt;
   PSG_HIP(hipMemcpyAsync(c->state, z0, sb, hipMemcpyHostToDevice, c->stream));
   PSG_HIP(hipMemcpyAsync(c->tl_state, dz0, sb, hipMemcpyHostToDevice, c->stream));
-  if (int r = ns3d_tl_run(c, nsteps)) return r;
+  if (int r = run_steps(c, 1, nsteps)) return r;
   if (zT) PSG_HIP(hipMemcpyAsync(zT, c->state, sb, hipMemcpyDeviceToHost, c->stream));
   PSG_HIP(hipMemcpyAsync(dzT, c->tl_state, sb, hipMemcpyDeviceToHost, c->stream));
   PSG_HIP(hipStreamSynchronize(c->stream));

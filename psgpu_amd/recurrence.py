@@ -1,0 +1,78 @@
+"""Near-recurrence search for Newton guesses in 2D Kolmogorov flow.
+
+The reference looks back over a stored history of DNS states for
+min over x-shifts s of ||S_s z(t+T) - z(t)|| / ||z(t)|| below a threshold, and
+writes such (z, T, s) triples as Newton guesses
+(2D_NavierStokes/src_RK4/functions.h:105-309, recurrence_check / computeZdash;
+3D_NavierStokes/src/functions.h:213-389).  This module does the same over
+snapshots taken from the GPU stepper: the shift that best aligns two states is
+found for all shifts at once, because the shifted inner product is a Fourier
+series in s over the kx index,
+
+    sum_L Re(conj(a) e^{-i kx s} b) = Re sum_kx e^{-i kx s} c(kx),
+    c(kx) = sum_ky conj(a(kx,ky)) b(kx,ky) L(kx,ky),
+
+evaluated on M equally spaced shifts by one FFT of length M per pair.
+The shift convention S_s z = exp(-i kx s) z is the Newton solver's
+(include/psg_newton.h), so a hit (t, T, s) is a guess for S_s Phi_T(z) = z.
+"""
+import numpy as np
+
+
+def record(stepper, z0, t_spin, t_window, dt_sample):
+    """Spin a 2D stepper up from z0 for t_spin, then keep a snapshot every
+    dt_sample over t_window.  Returns (times, snapshots[n, nkt])."""
+    dt = stepper.cfg.dt
+    stepper.set_state(np.asarray(z0).reshape(1, -1))
+    nspin = int(round(t_spin / dt))
+    if nspin:
+        stepper.run(nspin)
+    every = max(1, int(round(dt_sample / dt)))
+    nsnap = int(round(t_window / dt_sample)) + 1
+    snaps = np.empty((nsnap, stepper.nkt), np.complex128)
+    times = np.empty(nsnap)
+    for i in range(nsnap):
+        if i:
+            stepper.run(every)
+        snaps[i] = stepper.get_state()[0]
+        times[i] = t_spin + i * every * dt
+    return times, snaps
+
+
+def shifted_overlap(a, B, iktx, L, M=512):
+    """max over M shifts s_j = 2 pi j / M of sum_L Re(conj(a) S_s b) for each row b of B;
+    returns (overlap[nb], shift[nb])."""
+    ikty = a.size // iktx
+    w = (np.conj(a) * L).reshape(ikty, iktx)
+    c = np.einsum("yx,nyx->nx", w, B.reshape(len(B), ikty, iktx))
+    f = np.fft.fft(c, n=M, axis=1).real          # sum_kx c(kx) e^{-2 pi i kx j / M}
+    j = np.argmax(f, axis=1)
+    return f[np.arange(len(B)), j], 2 * np.pi * j / M
+
+
+def best_recurrences(times, snaps, iktx, L, tmin, tmax, top=8, M=512, separation=None):
+    """Pairs (t_i, T = t_j - t_i in [tmin, tmax]) ordered by the shift-minimised
+    relative distance ||S_s z_j - z_i|| / ||z_i||; at most one hit per `separation`
+    time units of t_i (default tmin) so that the guesses are distinct.
+    Returns a list of dicts {i, t, T, shift, dist}."""
+    L = np.asarray(L, float)
+    norms = np.sqrt(np.sum(np.abs(snaps) ** 2 * L[None, :], axis=1))
+    hits = []
+    for i in range(len(times)):
+        js = np.nonzero((times - times[i] >= tmin) & (times - times[i] <= tmax))[0]
+        if js.size == 0:
+            continue
+        ov, sh = shifted_overlap(snaps[i], snaps[js], iktx, L, M)
+        d2 = norms[i] ** 2 + norms[js] ** 2 - 2 * ov
+        k = int(np.argmin(d2))
+        hits.append({"i": i, "t": float(times[i]), "T": float(times[js[k]] - times[i]), "shift": float(sh[k]),
+                     "dist": float(np.sqrt(max(d2[k], 0.0)) / norms[i])})
+    hits.sort(key=lambda h: h["dist"])
+    sep = tmin if separation is None else separation
+    out = []
+    for h in hits:
+        if all(abs(h["t"] - o["t"]) >= sep for o in out):
+            out.append(h)
+        if len(out) == top:
+            break
+    return out

@@ -235,3 +235,49 @@ def test_autotune_keeps_state_and_picks_a_variant(gpu):
         s.run(10)
         got = s.get_state()
     assert rel(got, O.NS3D(n, v2=0.01, dt=0.002, force=3).run(st, 10)) < 1e-11
+
+
+@pytest.mark.parametrize("flavour,kw", [
+    ("ns3d", dict(n=32, v2=0.01, dt=0.002, force="kolmogorov")),
+    ("euler", dict(n=32, dt=0.004, force="none")),
+    ("ns2d", dict(n=64, v2=1 / 40.0, dt=0.005, force="kolmogorov", kf=4.0, ampfor=-2.0)),
+    ("bous", dict(n=32, v2=0.01, dt=0.004, ri=1.0, force="kolmogorov")),
+])
+def test_graph_replays_match_direct_launches(gpu, flavour, kw):
+    # captured step graphs replayed several times, with direct launches,
+    # host copies, a dt change and synchronisation in between, give bitwise
+    # the states of direct launches
+    if flavour == "ns2d":
+        st = O.ic_ns2d_cfg(O.grid2(64, 64), 6)[None]
+    elif flavour == "bous":
+        st = O.ic_solenoidal(O.grid3(32, 32, 32), 7, 0.5, 0.1)
+    else:
+        st = O.ic_solenoidal(O.grid3(32, 32, 32), 7)
+    with P.Stepper(flavour=flavour, **kw) as a, P.Stepper(flavour=flavour, **kw) as b:
+        a.set_graph(0)
+        b.set_graph(4)
+        a.set_state(st)
+        b.set_state(st)
+        for call, n in enumerate([13, 9, 4, 21]):
+            if call == 2:
+                a.set_dt(kw["dt"] * 0.5)
+                b.set_dt(kw["dt"] * 0.5)
+            a.run(n)
+            b.run(n)
+            sa, sb = a.get_state(), b.get_state()
+            assert np.array_equal(sa, sb), (flavour, call)
+            assert np.isfinite(sa).all()
+
+
+def test_graph_tangent_linear_matches_direct(gpu):
+    g = O.grid2(64, 64)
+    z = O.ic_ns2d_cfg(g, 8)
+    d = O.ic_ns2d_cfg(g, 9)
+    with P.Stepper(flavour="ns2d", n=64, v2=1 / 40.0, dt=0.01, kf=4.0, ampfor=-2.0) as a, \
+            P.Stepper(flavour="ns2d", n=64, v2=1 / 40.0, dt=0.01, kf=4.0, ampfor=-2.0) as b:
+        a.set_graph(0)
+        b.set_graph(5)
+        for T in (0.37, 0.51):
+            za, da = a.tangent(z[None], d[None], T, 0.01, 0)
+            zb, db = b.tangent(z[None], d[None], T, 0.01, 0)
+            assert np.array_equal(za, zb) and np.array_equal(da, db)
