@@ -6,17 +6,26 @@ per field), nu = 1/100, dt = 0.002, Kolmogorov forcing f = (sin 4y, 0, 0),
 seeded random solenoidal initial velocity with E(k) ~ k^4 exp(-2(k/4)^2),
 E = 0.5 (synthetic data).  The state is resident in HBM before timing starts.
 
-`value` is the plain FP64 form's steps/s.  The FP32-bulk form (FFTs and
-products in FP32, state/update/diagnostics FP64) is timed on the same input
-and reported under "fp32_bulk" with its speed-up and its energy error
-against the FP64 GPU run.  "roofline" is the dominant kernel's achieved HBM
-bandwidth (algorithmic bytes per launch / mean launch time from HIP events on
-the context stream during the timed steps); "model" gives the whole-step
-fraction of the 8 TB/s roofline with the 5-pass pruned-pencil byte model.
-"cpu_baseline" is the FP64 C oracle (test infrastructure) timed on this
-host's cores on a bounded sample of the same workload.
+`value` is the plain FP64 form's steps/s (captured step graphs, kernel shapes
+autotuned first).  The FP32-bulk form (FFTs and products in FP32,
+state/update/diagnostics FP64) is timed on the same input and reported under
+"fp32_bulk" with its speed-up and its energy error against the FP64 GPU run.
+"roofline" is the dominant kernel's achieved HBM bandwidth: algorithmic bytes
+per launch / mean launch time from HIP events on the context stream over a
+separate profiled run of 20 steps.  "model" is the whole-step fraction of the
+8 TB/s roofline under the 5-pass pruned-pencil byte model.  "cpu_baseline" is
+the FP64 C oracle (test infrastructure) timed on this host's cores on a
+bounded sample of the same workload.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-fp32]
+Further cases on the same line: "adjoint_128" (gradient of ||Phi_T(u0)-u0||^2
+at 128^3 over 200 steps with checkpoints every 20, error against central
+differences along 16 seeded solenoidal directions, FP64 and FP32-bulk),
+"bous_ensemble_512" (3D Boussinesq 512^3, 4 members stepped together, FP64 and
+FP32-bulk) and, with --newton K, "newton_2d" (2D Kolmogorov 512^2 Newton-GMRES-
+hookstep from K recurrence guesses, finite-difference vs tangent-linear Jv).
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-fp32] [--no-tune]
+                    [--no-adjoint] [--no-bous] [--newton K]
 """
 import argparse
 import json
@@ -98,7 +107,8 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-6):
         t0 = time.perf_counter()
         J(z0)
         out["forward_seconds"] = time.perf_counter() - t0
-        errs64, errs32, fds = [], [], []
+        errs64, errs32, fds, nerr64 = [], [], [], []
+        gnorm = float(np.sqrt(np.sum(np.abs(g64) ** 2 * L[None, :])))
         with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov", precision="fp32") as s32:
             s32.residual_grad(z0, nsteps, ckpt)
             t0 = time.perf_counter()
@@ -112,8 +122,12 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-6):
             fds.append(fd)
             errs64.append(abs(ip(g64) - fd) / abs(fd))
             errs32.append(abs(ip(g32) - fd) / abs(fd))
-    out.update({"J": J0, "fp64_max_rel_err_vs_fd": max(errs64), "fp32_max_rel_err_vs_fd": max(errs32),
-                "fp64_median_rel_err_vs_fd": float(np.median(errs64))})
+            nerr64.append(abs(ip(g64) - fd) / gnorm)
+    # per direction |<g,v> - fd| / |fd|; and the same error over ||g|| (|v| = 1), which does not
+    # inflate when a direction happens to be nearly orthogonal to the gradient
+    out.update({"J": J0, "grad_norm": gnorm, "fp64_max_rel_err_vs_fd": max(errs64),
+                "fp64_median_rel_err_vs_fd": float(np.median(errs64)), "fp32_max_rel_err_vs_fd": max(errs32),
+                "fp64_max_err_over_grad_norm": max(nerr64), "fd_values": fds})
     return out
 
 

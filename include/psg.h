@@ -113,12 +113,19 @@ int psg_device_count(void);
 /* ---------- kernel shapes and autotuning ----------
  * Shape variants of a pass (launch geometry, points per thread, register
  * staging); results are identical up to FP rounding of the transform order.
- * x-pass of the 3D Navier-Stokes and Euler right-hand side, variants 0..4:
+ * x-pass of the 3D Navier-Stokes and Euler right-hand side, variants 0..9
+ * (rows per block: as many as fit in 256 threads unless stated):
  *   0  8 points/thread, inputs front-loaded, product kept for all outputs
  *   1  8 points/thread, inputs front-loaded, product re-evaluated per output pair
  *   2  8 points/thread, inputs loaded per pair, product per output pair
  *   3  4 points/thread, inputs front-loaded, product kept
  *   4  4 points/thread, inputs loaded per pair, product per output pair
+ *   5  as 1 with 2 rows per block
+ *   6  as 1 with 4 rows per block
+ *   7  4 points/thread, front-loaded, product per pair, 1 row per block
+ *   8  16 points/thread (radix-16), inputs per pair, product per pair (FP32-bulk;
+ *      in the FP64 form 8 and 9 run variant 1)
+ *   9  as 8 with 4 rows per block
  * psg_autotune times `steps` steps of every variant on the context's own state
  * (restored afterwards), keeps the fastest and writes a text report. */
 #define PSG_PASS_XPASS 0

@@ -24,7 +24,8 @@ k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__r
   PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
-  const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
+  const TileId tid = xcd_tile();
+  const int ikx = tid.x * T + col, iky = tid.y, m = tid.z;
   const bool valid = ikx < g.iktx;
   const int ky = iky - g.kty;
   const double kx = (double)ikx * g.alpha, kyd = (double)ky;

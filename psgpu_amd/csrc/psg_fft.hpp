@@ -71,10 +71,46 @@ template <typename T, bool INV> __device__ __forceinline__ void dft8(cpx<T> *v) 
   v[3] = add(e3, o3); v[7] = sub(e3, o3);
 }
 
+// 16-point DFT as 4 x 4: n = n1 + 4 n2, k = 4 k1 + k2,
+// X[4k1+k2] = sum_n1 W4^{n1 k1} W16^{n1 k2} sum_n2 W4^{n2 k2} v[n1+4n2]
+template <typename T> struct K16 {
+  static constexpr T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173;
+  static constexpr T r2 = (T)0.70710678118654752440;
+};
+template <typename T, bool INV> __device__ __forceinline__ void dft16(cpx<T> *v) {
+#pragma unroll
+  for (int n1 = 0; n1 < 4; n1++) dft4<T, INV>(v + n1, 4);   // v[n1+4k2] = Y[n1][k2]
+  // W16^m = (cos 2 pi m/16, -sin 2 pi m/16) for m = n1*k2; inverse uses the conjugate
+  const T c1 = K16<T>::c1, s1 = K16<T>::s1, r2 = K16<T>::r2;
+  const cpx<T> w1{c1, -s1}, w2{r2, -r2}, w3{s1, -c1}, w6{-r2, -r2}, w9{-c1, s1};
+  auto tw = [](cpx<T> a, cpx<T> w) { return INV ? mulc(a, w) : mul(a, w); };
+  v[5] = tw(v[5], w1);    // n1=1,k2=1
+  v[9] = tw(v[9], w2);    // n1=1,k2=2
+  v[13] = tw(v[13], w3);  // n1=1,k2=3
+  v[6] = tw(v[6], w2);    // n1=2,k2=1
+  v[10] = rot90<T, INV>(v[10]);  // n1=2,k2=2: W16^4 = -i (forward)
+  v[14] = tw(v[14], w6);  // n1=2,k2=3
+  v[7] = tw(v[7], w3);    // n1=3,k2=1
+  v[11] = tw(v[11], w6);  // n1=3,k2=2
+  v[15] = tw(v[15], w9);  // n1=3,k2=3
+#pragma unroll
+  for (int k2 = 0; k2 < 4; k2++) dft4<T, INV>(v + 4 * k2, 1);  // v[4k2+k1] = X[4k1+k2]
+  // 4x4 transpose to natural order
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = a + 1; b < 4; b++) {
+      cpx<T> t = v[4 * a + b];
+      v[4 * a + b] = v[4 * b + a];
+      v[4 * b + a] = t;
+    }
+}
+
 template <typename T, bool INV, int R> __device__ __forceinline__ void dftR(cpx<T> *v) {
   if constexpr (R == 2) dft2<T, INV>(v[0], v[1]);
   else if constexpr (R == 4) dft4<T, INV>(v, 1);
   else if constexpr (R == 8) dft8<T, INV>(v);
+  else if constexpr (R == 16) dft16<T, INV>(v);
   else static_assert(R == 1, "unsupported radix");
 }
 

@@ -10,6 +10,7 @@
 // The reference stores the velocity UK,VK,WK after every update kernel; here
 // it is recomputed from the vorticity inside the first transform pass, which
 // is the same Biot-Savart expression and saves 6 NKT complex of traffic.
+#include <type_traits>
 #include "psg_ns3d.hpp"
 
 namespace psg {
@@ -127,6 +128,18 @@ template <typename P> static int xpass_ns(Ctx *c, const cpx<P> *B, cpx<P> *Cb, i
     case 2: return launch_xpass<P, 6, 3, ProdNS<P>, 8, false, true>(c, B, Cb, nr, nm, ProdNS<P>{});
     case 3: return launch_xpass<P, 6, 3, ProdNS<P>, 4, true, false>(c, B, Cb, nr, nm, ProdNS<P>{});
     case 4: return launch_xpass<P, 6, 3, ProdNS<P>, 4, false, true>(c, B, Cb, nr, nm, ProdNS<P>{});
+    case 5: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 2>(c, B, Cb, nr, nm, ProdNS<P>{});
+    case 6: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 4>(c, B, Cb, nr, nm, ProdNS<P>{});
+    case 7: return launch_xpass<P, 6, 3, ProdNS<P>, 4, true, true, 1>(c, B, Cb, nr, nm, ProdNS<P>{});
+    case 8:
+    case 9:
+      // 16 points per thread (radix-16 stages, one LDS exchange per 256-point
+      // transform): FP32 only, the FP64 registers would not fit
+      if constexpr (std::is_same<P, float>::value) {
+        if (c->xvar == 8) return launch_xpass<P, 6, 3, ProdNS<P>, 16, false, true>(c, B, Cb, nr, nm, ProdNS<P>{});
+        return launch_xpass<P, 6, 3, ProdNS<P>, 16, false, true, 4>(c, B, Cb, nr, nm, ProdNS<P>{});
+      }
+      return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true>(c, B, Cb, nr, nm, ProdNS<P>{});
     default: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, false>(c, B, Cb, nr, nm, ProdNS<P>{});
   }
 }

@@ -65,12 +65,15 @@ static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, l
 }
 
 
-template <typename P, int FI, int FO, class PROD, int E = EPT, bool FRONT = true, bool LAZY = false>
+// TEAMS_MAX caps the rows per block (0: as many as fit in 256 threads)
+template <typename P, int FI, int FO, class PROD, int E = EPT, bool FRONT = true, bool LAZY = false,
+          int TEAMS_MAX = 0>
 static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nx, {
     constexpr int TPX = N / E;
-    constexpr int TE = (256 / TPX) < 1 ? 1 : 256 / TPX;
+    constexpr int TE0 = (256 / TPX) < 1 ? 1 : 256 / TPX;
+    constexpr int TE = (TEAMS_MAX > 0 && TEAMS_MAX < TE0) ? TEAMS_MAX : TE0;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
     size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);

@@ -37,6 +37,24 @@ __device__ __forceinline__ const cpx<P> *stage_twiddles(const cpx<P> *__restrict
   __syncthreads();
   return lds_tw;
 }
+// XCD-aware tile order.  The dispatcher hands workgroups to the 8 XCDs round
+// robin (workgroup b runs on XCD b % 8, x fastest, then y, then z), and each
+// XCD has its own L2.  Column tiles of neighbouring kx ranges share cache lines
+// of the NKT-ordered state and tables (and of FP32 pass buffers), so the
+// logical tile order gives XCD x one contiguous range of tiles: shared lines
+// are then fetched once per XCD range boundary instead of once per tile pair.
+struct TileId { int x, y, z; };
+__device__ __forceinline__ TileId xcd_tile() {
+  constexpr unsigned X = 8;
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned nb = gx * gy * gridDim.z;
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned per = nb / X, tail = nb - per * X;
+  const unsigned x = b % X, q = b / X;
+  const unsigned l = x < tail ? x * (per + 1) + q : tail * (per + 1) + (x - tail) * per + q;
+  return TileId{(int)(l % gx), (int)((l / gx) % gy), (int)(l / (gx * gy))};
+}
+
 #define PSG_LDS_TWIDDLES(P, N)                                          \
   extern __shared__ char smem_raw[];                                   \
   cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw) + (N);            \
@@ -99,7 +117,8 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
   PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
-  const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
+  const TileId tid = xcd_tile();
+  const int ikx = tid.x * T + col, iky = tid.y, m = tid.z;
   const bool valid = ikx < g.iktx;
   const int ky = iky - g.kty;
   const double kx = (double)ikx * g.alpha, kyd = (double)ky;
@@ -150,7 +169,8 @@ k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
   PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
-  const int ikx = blockIdx.x * T + col, z = blockIdx.y, mf = blockIdx.z;
+  const TileId tid = xcd_tile();
+  const int ikx = tid.x * T + col, z = tid.y, mf = tid.z;
   const bool valid = ikx < g.iktx;
   const cpx<P> *src = in + (long)mf * g.zsz + (long)z * g.ikty * g.xp + ikx;
   cpx<P> v[EPT];
@@ -175,7 +195,8 @@ k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
   PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
-  const int ikx = blockIdx.x * T + col, z = blockIdx.y, mf = blockIdx.z;
+  const TileId tid = xcd_tile();
+  const int ikx = tid.x * T + col, z = tid.y, mf = tid.z;
   const bool valid = ikx < g.iktx;
   const cpx<P> *src = in + (long)mf * g.ysz + (long)z * g.ny * g.xp + ikx;
   cpx<P> v[EPT];
@@ -204,7 +225,8 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
   constexpr bool FRONT = FO <= 4;
   constexpr int NV = FRONT ? FO : 1;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
-  const int ikx = blockIdx.x * T + col, iky = blockIdx.y, m = blockIdx.z;
+  const TileId tid = xcd_tile();
+  const int ikx = tid.x * T + col, iky = tid.y, m = tid.z;
   const bool valid = ikx < g.iktx;
   cpx<double> res[FO][EPT];
   cpx<P> vin[NV][EPT];
@@ -421,7 +443,8 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
   PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
-  const int ikx = blockIdx.x * T + col, m = blockIdx.z;
+  const TileId tid = xcd_tile();
+  const int ikx = tid.x * T + col, m = tid.z;
   const bool valid = ikx < g.iktx;
   const double kx = (double)ikx * g.alpha;
 #pragma unroll 1
@@ -480,7 +503,8 @@ k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restr
   PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
-  const int ikx = blockIdx.x * T + col, m = blockIdx.z;
+  const TileId tid = xcd_tile();
+  const int ikx = tid.x * T + col, m = tid.z;
   const bool valid = ikx < g.iktx;
   cpx<double> res[NF][EPT];
 #pragma unroll

@@ -76,7 +76,8 @@ k_zinv_curl(const cpx<double> *__restrict__ st, long fs, cpx<P> *__restrict__ ou
   PSG_LDS_TWIDDLES(P, N);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
-  const int ikx = blockIdx.x * T + col, iky = blockIdx.y;
+  const TileId tid = xcd_tile();
+  const int ikx = tid.x * T + col, iky = tid.y;
   const bool valid = ikx < g.iktx;
   const int ky = iky - g.kty;
   const double kx = (double)ikx * g.alpha, kyd = (double)ky;

@@ -210,7 +210,7 @@ def test_bous_aspect_batch(gpu):
             assert rel(s.get_state(m), orc.run(sts[m], 10)) < 1e-11
 
 
-@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("variant", range(10))
 def test_ns3d_xpass_variants_match_oracle(gpu, variant):
     n = 32
     st = O.ic_solenoidal(O.grid3(n, n, n), 24)
@@ -230,7 +230,7 @@ def test_autotune_keeps_state_and_picks_a_variant(gpu):
         s.set_state(st)
         rep = s.autotune(steps=2)
         assert "chosen xpass variant" in rep
-        assert 0 <= s.get_variant() < 5
+        assert 0 <= s.get_variant() < 10
         assert np.array_equal(s.get_state(), st)
         s.run(10)
         got = s.get_state()
@@ -281,3 +281,20 @@ def test_graph_tangent_linear_matches_direct(gpu):
             za, da = a.tangent(z[None], d[None], T, 0.01, 0)
             zb, db = b.tangent(z[None], d[None], T, 0.01, 0)
             assert np.array_equal(za, zb) and np.array_equal(da, db)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 5, 8, 9])
+def test_fp32_xpass_variants_error_bound(gpu, variant):
+    # FP32-bulk shapes (8, 9: radix-16 transforms) against the FP64 oracle
+    n = 64
+    st = O.ic_solenoidal(O.grid3(n, n, n), 26)
+    orc = O.NS3D(n, v2=0.01, dt=0.002, force=3)
+    want = orc.run(st, 30)
+    with P.Stepper(flavour="ns3d", n=n, precision="fp32") as s:
+        s.set_variant(variant)
+        s.set_state(st)
+        s.run(30)
+        got = s.get_state()
+        e = s.diag()[0]
+    assert abs(e - orc.diag(want)[0]) / orc.diag(want)[0] < 1e-6
+    assert rel(got, want) < 2e-5
