@@ -371,7 +371,7 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
   int NERcount = 0;
   int status = PSG_NEWTON_FAILED, INE = 0;
   long gm_total = 0, hook_total = 0;
-  if (o->verbose) printf("newton: start T=%.10g sx=%.6g NER=%.3e\n", T, sh[0], NER);
+  if (o->verbose) { fprintf(stderr, "newton: start T=%.10g sx=%.6g NER=%.3e\n", T, sh[0], NER); fflush(stderr); }
   if (NER <= o->netol) status = PSG_NEWTON_CONVERGED;
   for (INE = 1; INE <= o->max_newton && status != PSG_NEWTON_CONVERGED; INE++) {
     const double NORMb = NORMFN;
@@ -459,7 +459,7 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
         r2 += t * t;
       }
       GMR = sqrt(r2) / NORMb;
-      if (o->verbose > 1) printf("  gmres %d  residual %.3e\n", k, GMR);
+      if (o->verbose > 1) { fprintf(stderr, "  gmres %d  residual %.3e\n", k, GMR); fflush(stderr); }
       if (GMR <= o->gmtol) break;
     }
     if (k > K) k = K;   // Krylov cap reached: hookstep in the space built so far
@@ -503,9 +503,11 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
       V.copy(ZNSH, FNplus);
       V.axpby(-1.0, Zplus, 1.0, FNplus);
       NORMFNplus = norm(FNplus);
-      if (o->verbose > 1)
-        printf("  hook %d mu=%.3e Delta=%.3e |F+|=%.3e pred=%.3e |F|=%.3e\n", IHOOK, mu, Delta, NORMFNplus,
-               NORMFNpred, NORMFN);
+      if (o->verbose > 1) {
+        fprintf(stderr, "  hook %d mu=%.3e Delta=%.3e |F+|=%.3e pred=%.3e |F|=%.3e\n", IHOOK, mu, Delta,
+                NORMFNplus, NORMFNpred, NORMFN);
+        fflush(stderr);
+      }
       if (NORMFNplus != NORMFNplus || NORMFNpred != NORMFNpred) {
         Delta /= 2.0;
         halve = true;
@@ -559,7 +561,7 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
     V.copy(FNplus, FN);
     NORMFN = NORMFNplus;
     NER = NORMFN / NORMZ;
-    if (o->verbose) printf("newton %d: NER=%.3e T=%.10g sx=%.6g gmres=%d hook=%d\n", INE, NER, T, sh[0], k, IHOOK);
+    if (o->verbose) { fprintf(stderr, "newton %d: NER=%.3e T=%.10g sx=%.6g gmres=%d hook=%d\n", INE, NER, T, sh[0], k, IHOOK); fflush(stderr); }
     const double relch = fabs((NER - NERold) / NER);
     if (relch < 0.001) {
       if (++NERcount > o->stall_count) { status = PSG_NEWTON_STALLED; break; }

@@ -52,7 +52,9 @@ def shifted_overlap(a, B, iktx, L, M=512):
 
 def best_recurrences(times, snaps, iktx, L, tmin, tmax, top=8, M=512, separation=None):
     """Pairs (t_i, T = t_j - t_i in [tmin, tmax]) ordered by the shift-minimised
-    relative distance ||S_s z_j - z_i|| / ||z_i||; at most one hit per `separation`
+    relative distance ||S_s z_j - z_i|| / ||z_i||.  For each t_i only interior
+    local minima of the distance over T count (a minimum at the end of the T
+    window reflects slow drift, not a return); at most one hit per `separation`
     time units of t_i (default tmin) so that the guesses are distinct.
     Returns a list of dicts {i, t, T, shift, dist}."""
     L = np.asarray(L, float)
@@ -60,11 +62,14 @@ def best_recurrences(times, snaps, iktx, L, tmin, tmax, top=8, M=512, separation
     hits = []
     for i in range(len(times)):
         js = np.nonzero((times - times[i] >= tmin) & (times - times[i] <= tmax))[0]
-        if js.size == 0:
+        if js.size < 3:
             continue
         ov, sh = shifted_overlap(snaps[i], snaps[js], iktx, L, M)
         d2 = norms[i] ** 2 + norms[js] ** 2 - 2 * ov
-        k = int(np.argmin(d2))
+        interior = np.nonzero((d2[1:-1] < d2[:-2]) & (d2[1:-1] <= d2[2:]))[0] + 1
+        if interior.size == 0:
+            continue
+        k = int(interior[np.argmin(d2[interior])])
         hits.append({"i": i, "t": float(times[i]), "T": float(times[js[k]] - times[i]), "shift": float(sh[k]),
                      "dist": float(np.sqrt(max(d2[k], 0.0)) / norms[i])})
     hits.sort(key=lambda h: h["dist"])
