@@ -54,7 +54,9 @@ def model_bytes_step(n, b):
     Z, Y = iktx * ikty * n, iktx * n * n
     S, Fi, Fo = 3, 6, 3
     rhs = b * (S * nkt + 2 * Fi * Z + 2 * Fi * Y + 2 * Fo * Y + 2 * Fo * Z)
-    stage = 6 * S * nkt * 16          # state/RH traffic stays FP64
+    # Heun state/RH traffic (FP64): predictor reads w^n, writes w* and RH; corrector reads w*, RH,
+    # writes w^{n+1}
+    stage = 6 * S * nkt * 16
     return 2 * rhs + stage
 
 
@@ -279,6 +281,10 @@ def main():
                 "avg_launch_ms": dom["avg_ms"]}
     mb = model_bytes_step(n, 16)
     model = {"gb_per_step": mb / 1e9, "achieved_gbs": mb * sps64 / 1e9, "frac": mb * sps64 / 1e9 / PEAK_HBM_GBS}
+    if fp32:
+        mb32 = model_bytes_step(n, 8)
+        fp32["model"] = {"gb_per_step": mb32 / 1e9, "achieved_gbs": mb32 * fp32["steps_per_s"] / 1e9,
+                         "frac": mb32 * fp32["steps_per_s"] / 1e9 / PEAK_HBM_GBS}
     adj = None
     if not a.no_adjoint:
         log("bench: adjoint gradient 128^3, 200 steps")

@@ -77,80 +77,49 @@ __global__ void k2_setz0(const cpx<double> *__restrict__ z, cpx<double> *__restr
   }
 }
 
-// 2D launch shapes.  The standard column tile (T kx columns of N/8 threads)
-// gives only ceil(IKTX/T) blocks per member, too few to fill 256 CUs for
-// a single 2D field; "narrow" shapes use 2-column tiles and one x-pass row per
-// block when the standard tiling would leave most CUs idle.  At these sizes
-// the pass buffers are L2/MALL-resident, so the wider tiles' coalescing
-// matters less than parallelism.
-static bool narrow2(const Ctx *c, int nm) { return (long)((c->g.iktx + 7) / 8) * nm < 256; }
-
-template <typename P, int N, int T, int NSRC>
-static int yinv2_shape(Ctx *c, const cpx<double> *st, const cpx<double> *st2, cpx<P> *out, long out_ms, int nm) {
+template <typename P> static int launch_yinv2(Ctx *c, const cpx<double> *st, cpx<P> *out, int nm) {
   const Grid &g = c->g;
-  dim3 grid((g.iktx + T - 1) / T, 1, nm);
-  size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
-  auto k = k2_yinv<P, N, T, NSRC>;
-  PSG_HIP(lds_attr(k, lds));
-  hipLaunchKernelGGL(k, grid, dim3(T * (N / EPT)), lds, c->stream, st, g.nkt, out, out_ms, g, twp<P>(c, 1),
-                     c->mask, st2);
+  PSG_DISPATCH_N(g.ny, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
+    auto k = k2_yinv<P, N, T>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, g.nkt, out, 3 * g.ysz, g,
+                       twp<P>(c, 1), c->mask, (const cpx<double> *)nullptr);
+  });
   PSG_HIP(hipGetLastError());
   return 0;
 }
 
-template <typename P, int N, int T, class UPD, int NF>
-static int yfwd2_shape(Ctx *c, const cpx<P> *in, long in_ms, UPD upd, int nm) {
+template <typename P, class UPD> static int launch_yfwd2(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
   const Grid &g = c->g;
-  dim3 grid((g.iktx + T - 1) / T, 1, nm);
-  size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
-  auto k = k2_yfwd<P, N, T, UPD, NF>;
-  PSG_HIP(lds_attr(k, lds));
-  hipLaunchKernelGGL(k, grid, dim3(T * (N / EPT)), lds, c->stream, in, in_ms, g, twp<P>(c, 1), upd);
-  PSG_HIP(hipGetLastError());
-  return 0;
-}
-
-template <typename P, int NSRC>
-static int launch_yinv2(Ctx *c, const cpx<double> *st, const cpx<double> *st2, cpx<P> *out, int nm) {
-  const bool narrow = narrow2(c, nm);
-  PSG_DISPATCH_N(c->g.ny, {
-    return narrow ? yinv2_shape<P, N, 2, NSRC>(c, st, st2, out, 3 * NSRC * c->g.ysz, nm)
-                  : yinv2_shape<P, N, Col<N>::T, NSRC>(c, st, st2, out, 3 * NSRC * c->g.ysz, nm);
+  PSG_DISPATCH_N(g.ny, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
+    auto k = k2_yfwd<P, N, T, UPD>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, 2 * g.ysz, g, twp<P>(c, 1), upd);
   });
-  return 0;
-}
-
-template <typename P, int NF, class UPD> static int launch_yfwd2(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
-  const bool narrow = narrow2(c, nm);
-  PSG_DISPATCH_N(c->g.ny, {
-    return narrow ? yfwd2_shape<P, N, 2, UPD, NF>(c, in, NF * c->g.ysz, upd, nm)
-                  : yfwd2_shape<P, N, Col<N>::T, UPD, NF>(c, in, NF * c->g.ysz, upd, nm);
-  });
-  return 0;
-}
-
-template <typename P, int N, int TE, int FI, int FO, class PROD>
-static int xpass2_shape(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
-  const Grid &g = c->g;
-  long nrows = nrow_m * nm;
-  dim3 grid((unsigned)((nrows + TE - 1) / TE));
-  size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);
-  auto k = k_xpass<P, N, TE, FI, FO, PROD>;
-  PSG_HIP(lds_attr(k, lds));
-  hipLaunchKernelGGL(k, grid, dim3(TE * (N / EPT)), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0), prod);
   PSG_HIP(hipGetLastError());
   return 0;
 }
 
 template <typename P, int FI, int FO, class PROD>
 static int launch_xpass2(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
-  const bool narrow = narrow2(c, nm);
-  PSG_DISPATCH_N(c->g.nx, {
+  const Grid &g = c->g;
+  PSG_DISPATCH_N(g.nx, {
     constexpr int TE = Row<N>::TEAMS;
-    constexpr int TN = (N / EPT) >= 64 ? 1 : 64 / (N / EPT);   // one wavefront per block
-    return narrow ? xpass2_shape<P, N, TN, FI, FO, PROD>(c, in, out, nrow_m, nm, prod)
-                  : xpass2_shape<P, N, TE, FI, FO, PROD>(c, in, out, nrow_m, nm, prod);
+    long nrows = nrow_m * nm;
+    dim3 grid((unsigned)((nrows + TE - 1) / TE));
+    size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);
+    auto k = k_xpass<P, N, TE, FI, FO, PROD>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
+                       prod);
   });
+  PSG_HIP(hipGetLastError());
   return 0;
 }
 
@@ -166,13 +135,13 @@ template <typename P, class UPD> static int rhs2(Ctx *c, const cpx<double> *z0, 
   const long Y = (long)g.iktx * g.ny;
   const size_t b = sizeof(cpx<P>);
   prof_begin(c, "yinv2", (long)(g.nkt * 16 + 3 * Y * b) * nm);
-  PSG_TRY2((launch_yinv2<P, 1>(c, z0, nullptr, B, nm)));
+  PSG_TRY2(launch_yinv2<P>(c, z0, B, nm));
   prof_end(c);
   prof_begin(c, "xpass2", (long)(5 * Y * b) * nm);
   PSG_TRY2((launch_xpass2<P, 3, 2>(c, B, Cb, g.ny, nm, Prod2D<P>{})));
   prof_end(c);
   prof_begin(c, "yfwd2_update", (long)(2 * Y * b + 4 * g.nkt * 16) * nm);
-  PSG_TRY2((launch_yfwd2<P, 2>(c, Cb, upd, nm)));
+  PSG_TRY2((launch_yfwd2<P>(c, Cb, upd, nm)));
   prof_end(c);
   return 0;
 }
@@ -250,7 +219,7 @@ template <typename P> static int physical2_t(Ctx *c, int member, double *out) {
   const long total = g.nkt;
   hipLaunchKernelGGL(k2_setz0, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream,
                      c->state + (long)member * g.nkt, c->tmp, c->mask, g.nkt, total);
-  PSG_TRY2((launch_yinv2<P, 1>(c, c->tmp, nullptr, B, 1)));
+  PSG_TRY2(launch_yinv2<P>(c, c->tmp, B, 1));
   long nrow = g.ny;
   PSG_DISPATCH_N(g.nx, {
     constexpr int TE = Row<N>::TEAMS;
@@ -317,9 +286,26 @@ template <int STAGE> struct Upd2DTL {
 template <typename P, class UPD> static int rhs2_tl(Ctx *c, UPD upd, int nm) {
   const Grid &g = c->g;
   cpx<P> *B = (cpx<P> *)c->tl_bufB, *Cb = (cpx<P> *)c->tl_bufC;
-  PSG_TRY2((launch_yinv2<P, 2>(c, c->stage, c->tl_stage, B, nm)));
+  PSG_DISPATCH_N(g.ny, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
+    auto k = k2_yinv<P, N, T, 2>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, c->stage, g.nkt, B, 6 * g.ysz, g,
+                       twp<P>(c, 1), c->mask, c->tl_stage);
+  });
+  PSG_HIP(hipGetLastError());
   PSG_TRY2((launch_xpass2<P, 6, 4>(c, B, Cb, g.ny, nm, Prod2DTL<P>{})));
-  PSG_TRY2((launch_yfwd2<P, 4>(c, Cb, upd, nm)));
+  PSG_DISPATCH_N(g.ny, {
+    constexpr int T = Col<N>::T;
+    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
+    auto k = k2_yfwd<P, N, T, UPD, 4>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, Cb, 4 * g.ysz, g, twp<P>(c, 1), upd);
+  });
+  PSG_HIP(hipGetLastError());
   return 0;
 }
 
