@@ -103,7 +103,8 @@ static void prof_collect(Ctx *c) {
 // ------------------------------------------------------------------ diag
 __global__ void k_diag(const cpx<double> *__restrict__ st, long fs, int nf, const int8_t *__restrict__ L, Grid g,
                        int nbins, double *__restrict__ part) {
-  extern __shared__ double sh[];  // [2*blockDim] + nbins
+  PSG_DYN_SMEM(sh_raw);  // [2*blockDim] + nbins doubles
+  double *sh = reinterpret_cast<double *>(sh_raw);
   double *bins = sh + 2 * blockDim.x;
   for (int i = threadIdx.x; i < nbins; i += blockDim.x) bins[i] = 0.0;
   __syncthreads();

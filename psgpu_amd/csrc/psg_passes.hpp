@@ -55,8 +55,12 @@ __device__ __forceinline__ TileId xcd_tile() {
   return TileId{(int)(l % gx), (int)((l / gx) % gy), (int)(l / (gx * gy))};
 }
 
+// dynamic LDS of a kernel (a CPU build for bounds checking supplies its own)
+#ifndef PSG_DYN_SMEM
+#define PSG_DYN_SMEM(name) extern __shared__ char name[]
+#endif
 #define PSG_LDS_TWIDDLES(P, N)                                          \
-  extern __shared__ char smem_raw[];                                   \
+  PSG_DYN_SMEM(smem_raw);                                              \
   cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw) + (N);            \
   tw = stage_twiddles<P, N>(tw, reinterpret_cast<cpx<P> *>(smem_raw))
 

@@ -1,0 +1,226 @@
+// Runs the product C-ABI, built for the CPU against tests/emul/hip/hip_runtime.h
+// with AddressSanitizer, over every flavour, precision, x-pass shape variant,
+// graph path, tangent-linear/adjoint and Newton entry point at small sizes,
+// and checks the results against the FP64 oracle.  Exit status 0 = all passed.
+// TEST INFRASTRUCTURE ONLY.
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <string.h>
+#include <complex>
+#include <vector>
+#include "oracle.h"
+#include "psg.h"
+#include "psg_adjoint.h"
+#include "psg_newton.h"
+
+typedef std::vector<double> V;
+static int g_fail = 0, g_pass = 0;
+
+static void check(bool ok, const char *what, double v) {
+  printf("%s %-58s %.3e\n", ok ? "PASS" : "FAIL", what, v);
+  fflush(stdout);
+  if (ok) g_pass++;
+  else g_fail++;
+}
+static double rel(const V &a, const V &b) {
+  double m = 0, n = 0;
+  for (size_t i = 0; i < a.size(); i++) { m = fmax(m, fabs(a[i] - b[i])); n = fmax(n, fabs(b[i])); }
+  return m / (n > 0 ? n : 1);
+}
+static psg_config cfg3(int flav, int nx, int ny, int nz, double alpha, int prec, int batch, double v2, double dt,
+                       int force) {
+  psg_config c;
+  memset(&c, 0, sizeof c);
+  c.flavour = flav; c.precision = prec; c.batch = batch; c.nx = nx; c.ny = ny; c.nz = nz; c.alpha = alpha;
+  c.v2 = v2; c.dt = dt; c.force = force; c.kf = 4.0;
+  c.ampfor = force == PSG_FORCE_DNS ? -4.0 : (force == PSG_FORCE_GMRES ? -1.0 : (force == PSG_FORCE_KOLMO ? -2.0 : 0));
+  c.ri = 1.0; c.sc = 1.0; c.theta = M_PI / 2;
+  return c;
+}
+static V ic3(int nx, int ny, int nz, double alpha, uint64_t seed, int nf) {
+  or_grid g;
+  or_grid3(&g, nx, ny, nz, alpha);
+  V s(2 * nf * g.nkt, 0.0);
+  orc *p = (orc *)s.data();
+  or_ic_solenoidal(&g, seed, 0.5, 0.1, p, p + g.nkt, p + 2 * g.nkt, nf == 4 ? p + 3 * g.nkt : nullptr);
+  return s;
+}
+
+static void ns3d_case(const char *name, int flav, int nx, int ny, int nz, double alpha, int prec, int variant,
+                      int graph, int steps, int force, double tol) {
+  psg_config c = cfg3(flav, nx, ny, nz, alpha, prec, 1, flav == PSG_EULER3D ? 0.0 : 0.02, 0.004, force);
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, name, 1); return; }
+  psg_set_variant(ctx, PSG_PASS_XPASS, variant);
+  psg_set_graph(ctx, graph);
+  V st = ic3(nx, ny, nz, alpha, 11, 3), got(st.size());
+  psg_set_state(ctx, 0, st.data());
+  psg_run(ctx, steps);
+  psg_get_state(ctx, 0, got.data());
+  double e, z, sp[64];
+  psg_diag(ctx, 0, &e, &z, sp);
+  psg_destroy(ctx);
+  or3_ctx *o = or3_create(nx, ny, nz, alpha, flav == PSG_EULER3D ? 1 : 0, c.v2, c.dt, force, c.kf, c.ampfor);
+  const long nkt = (long)st.size() / 6;
+  V want = st;
+  orc *w = (orc *)want.data();
+  or3_run(o, w, w + nkt, w + 2 * nkt, steps);
+  or3_destroy(o);
+  check(rel(got, want) < tol, name, rel(got, want));
+}
+
+static void bous_case(const char *name, int nx, int ny, int nz, double alpha, int prec, int batch, int graph) {
+  psg_config c = cfg3(PSG_BOUS3D, nx, ny, nz, alpha, prec, batch, 0.01, 0.004, PSG_FORCE_KOLMO);
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, name, 1); return; }
+  psg_set_graph(ctx, graph);
+  std::vector<V> st;
+  for (int m = 0; m < batch; m++) { st.push_back(ic3(nx, ny, nz, alpha, 40 + m, 4)); psg_set_state(ctx, m, st[m].data()); }
+  psg_run(ctx, 4);
+  const long nkt = (long)st[0].size() / 8;
+  orb_ctx *o = orb_create(nx, ny, nz, alpha, c.v2, c.sc, c.ri, c.theta, c.dt, c.kf, c.ampfor);
+  double worst = 0;
+  for (int m = 0; m < batch; m++) {
+    V got(st[m].size()), want = st[m];
+    psg_get_state(ctx, m, got.data());
+    orc *w = (orc *)want.data();
+    orb_run(o, w, w + nkt, w + 2 * nkt, w + 3 * nkt, 4);
+    worst = fmax(worst, rel(got, want));
+  }
+  // dz/dt and physical fields of member 0 through the same context
+  V d(st[0].size()), dw(st[0].size());
+  psg_timederiv(ctx, 0, st[0].data(), d.data());
+  orc *s0 = (orc *)st[0].data(), *o0 = (orc *)dw.data();
+  orb_rhs(o, s0, s0 + nkt, s0 + 2 * nkt, s0 + 3 * nkt, o0, o0 + nkt, o0 + 2 * nkt, o0 + 3 * nkt);
+  V phys((size_t)7 * nx * ny * nz);
+  psg_physical(ctx, 0, phys.data());
+  orb_destroy(o);
+  psg_destroy(ctx);
+  check(worst < (prec ? 1e-4 : 1e-11), name, worst);
+  check(rel(d, dw) < (prec ? 1e-4 : 1e-12), "  bous timederiv", rel(d, dw));
+}
+
+static void ns2d_case(const char *name, int n, int prec, int batch, int graph, int steps) {
+  psg_config c = cfg3(PSG_NS2D, n, n, 1, 1.0, prec, batch, 1 / 40.0, 0.005, PSG_FORCE_KOLMO);
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, name, 1); return; }
+  psg_set_graph(ctx, graph);
+  or_grid g;
+  or_grid2(&g, n, n, 1.0);
+  std::vector<V> st;
+  for (int m = 0; m < batch; m++) {
+    st.push_back(V(2 * g.nkt));
+    or_ic_ns2d_cfg(&g, 5 + m, 1.0, (orc *)st[m].data());
+    psg_set_state(ctx, m, st[m].data());
+  }
+  psg_run(ctx, steps);
+  or2_ctx *o = or2_create(n, n, 1.0, c.v2, c.dt, 4, -2.0);
+  double worst = 0;
+  for (int m = 0; m < batch; m++) {
+    V got(st[m].size()), want = st[m];
+    psg_get_state(ctx, m, got.data());
+    or2_run(o, (orc *)want.data(), steps);
+    worst = fmax(worst, rel(got, want));
+  }
+  V nl(st[0].size()), nlw(st[0].size()), phys((size_t)3 * n * n);
+  psg_nonlin(ctx, 0, st[0].data(), nl.data());
+  or2_nonlin(o, (orc *)st[0].data(), (orc *)nlw.data());
+  psg_physical(ctx, 0, phys.data());
+  or2_destroy(o);
+  psg_destroy(ctx);
+  check(worst < (prec ? 1e-4 : 1e-11), name, worst);
+  check(rel(nl, nlw) < (prec ? 1e-4 : 1e-12), "  2d nonlin", rel(nl, nlw));
+}
+
+static double ip(const V &a, const V &b, const std::vector<int> &L, long nkt) {
+  double s = 0;
+  for (size_t i = 0; i < a.size(); i += 2) {
+    const long ik = (long)(i / 2) % nkt;
+    if (L[ik]) s += a[i] * b[i] + a[i + 1] * b[i + 1];
+  }
+  return s;
+}
+
+static void tladj_case(int n, int prec) {
+  psg_config c = cfg3(PSG_NS3D, n, n, n, 1.0, prec, 1, 0.02, 0.004, PSG_FORCE_KOLMO);
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, "tl/adjoint create", 1); return; }
+  or_grid g;
+  or_grid3(&g, n, n, n, 1.0);
+  std::vector<int> L(g.nkt);
+  or_tables(&g, nullptr, nullptr, nullptr, L.data());
+  V z = ic3(n, n, n, 1.0, 3, 3), v = ic3(n, n, n, 1.0, 4, 3), w = ic3(n, n, n, 1.0, 5, 3);
+  V zT(z.size()), jv(z.size()), jtw(z.size());
+  const int nst = 7;
+  psg_tangent_steps(ctx, z.data(), v.data(), nst, zT.data(), jv.data());
+  psg_adjoint(ctx, z.data(), w.data(), nst, 3, nullptr, jtw.data());
+  const double a = ip(jv, w, L, g.nkt), b = ip(v, jtw, L, g.nkt);
+  check(fabs(a - b) <= (prec ? 1e-4 : 1e-11) * fabs(a), prec ? "tl/adjoint dot product fp32" : "tl/adjoint dot product fp64",
+        fabs(a - b) / fabs(a));
+  double J;
+  V grad(z.size());
+  psg_residual_grad(ctx, z.data(), nst, 2, 0.1, 0.0, 0.0, &J, grad.data(), nullptr);
+  check(J > 0 && std::isfinite(J), "  residual grad finite", J);
+  // 3D Newton-tangent route and one Newton iteration with TL Jv (bounds only)
+  V z1(z.size()), dz1(z.size());
+  psg_tangent(ctx, z.data(), v.data(), 5 * c.dt, c.dt, PSG_TIME_LASTSTEP, z1.data(), dz1.data());
+  check(rel(dz1, dz1) == 0 && std::isfinite(dz1[0]), "  psg_tangent 3d", 0);
+  psg_destroy(ctx);
+}
+
+static void newton2d_case(int jv) {
+  const int n = 32;
+  psg_config c = cfg3(PSG_NS2D, n, n, 1, 1.0, 0, 1, 1 / 5.0, 0.01, PSG_FORCE_KOLMO);
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, "newton create", 1); return; }
+  or_grid g;
+  or_grid2(&g, n, n, 1.0);
+  std::vector<double> kx(g.nkt), ky(g.nkt);
+  std::vector<int> L(g.nkt);
+  or_tables(&g, kx.data(), ky.data(), nullptr, L.data());
+  V z(2 * g.nkt, 0.0), d(2 * g.nkt);
+  or_ic_ns2d_cfg(&g, 9, 0.05, (orc *)d.data());
+  for (long i = 0; i < g.nkt; i++) {
+    if (kx[i] == 0 && ky[i] == 4) z[2 * i] = -5.0 / 8;
+    z[2 * i] += d[2 * i]; z[2 * i + 1] += d[2 * i + 1];
+  }
+  psg_newton_opts o;
+  psg_newton_defaults(PSG_NS2D, &o);
+  o.nshift = 0; o.fixed_period = 1; o.jv = jv; o.dt0 = 0.01; o.max_gmres = 12; o.max_newton = 8;
+  psg_newton_result r;
+  double T = 0.2, sx = 0, sz = 0;
+  psg_newton_solve(ctx, z.data(), &T, &sx, &sz, &o, &r);
+  psg_destroy(ctx);
+  check(r.status == PSG_NEWTON_CONVERGED, jv ? "newton 2d laminar (TL Jv)" : "newton 2d laminar (FD Jv)", r.ner);
+}
+
+int main(int argc, char **argv) {
+  omp_set_dynamic(0);
+  omp_set_max_active_levels(1);
+  const bool quick = argc > 1 && strcmp(argv[1], "quick") == 0;
+  // 3D Navier-Stokes: forcings, precisions, every x-pass variant, graph path, aspect ratio
+  for (int v = 0; v < 10; v++) {
+    char nm[96];
+    snprintf(nm, sizeof nm, "ns3d 16^3 fp64 xpass variant %d", v);
+    ns3d_case(nm, PSG_NS3D, 16, 16, 16, 1.0, PSG_FP64, v, 0, 2, PSG_FORCE_KOLMO, 1e-11);
+    if (quick) continue;
+    snprintf(nm, sizeof nm, "ns3d 32^3 fp32 xpass variant %d", v);
+    ns3d_case(nm, PSG_NS3D, 32, 32, 32, 1.0, PSG_FP32BULK, v, 0, 2, PSG_FORCE_KOLMO, 1e-4);
+  }
+  ns3d_case("ns3d 32^3 fp64 graph 2 (dns forcing)", PSG_NS3D, 32, 32, 32, 1.0, PSG_FP64, 1, 2, 6, PSG_FORCE_DNS, 1e-11);
+  ns3d_case("ns3d 32x16x16 a=0.5 fp64 graph (gmres forcing)", PSG_NS3D, 32, 16, 16, 0.5, PSG_FP64, 0, 2, 5,
+            PSG_FORCE_GMRES, 1e-11);
+  ns3d_case("euler 16^3 fp64 graph", PSG_EULER3D, 16, 16, 16, 1.0, PSG_FP64, 0, 2, 5, PSG_FORCE_NONE, 1e-11);
+  bous_case("bous 16^3 fp64 batch 2 graph", 16, 16, 16, 1.0, PSG_FP64, 2, 2);
+  if (!quick) bous_case("bous 32x16x16 a=0.5 fp32", 32, 16, 16, 0.5, PSG_FP32BULK, 1, 0);
+  ns2d_case("ns2d 32^2 fp64 graph", 32, PSG_FP64, 1, 2, 5);
+  ns2d_case("ns2d 64^2 fp64 batch 3", 64, PSG_FP64, 3, 0, 3);
+  if (!quick) ns2d_case("ns2d 64^2 fp32 graph", 64, PSG_FP32BULK, 1, 2, 5);
+  tladj_case(16, PSG_FP64);
+  if (!quick) tladj_case(16, PSG_FP32BULK);
+  newton2d_case(0);
+  newton2d_case(1);
+  printf("%d passed, %d failed\n", g_pass, g_fail);
+  return g_fail ? 1 : 0;
+}
