@@ -265,3 +265,76 @@ def test_bous_projection_and_rk4_order():
     for nst in (8, 16):
         errs.append(np.abs(O.BOUS(16, v2=0.02, dt=T / nst).run(st, nst) - ref).max())
     assert errs[0] / errs[1] > 3.0                      # at least second order (RK4 stages, CN viscous part)
+
+
+def test_nonlinear_terms_conserve_quadratic_invariants():
+    # The 2/3-truncated advection term is conservative: 3D Navier-Stokes / Euler
+    # keep the energy sum_L |w|^2/|k|^2; 2D keeps energy and enstrophy.  A wrong
+    # sign or index in one component of the product, Biot-Savart or curl breaks it.
+    g = O.grid3(16, 16, 16)
+    kx, ky, kz, L = O.tables(g)
+    k2 = np.maximum(kx ** 2 + ky ** 2 + kz ** 2, 1e-3)
+    s = O.NS3D(16, force=0)
+    for seed in range(3):
+        st = O.ic_solenoidal(g, 200 + seed, 0.5)
+        n = s.nonlin(st)
+        de = np.sum((np.conj(st) * n).real * L / k2)
+        scale = np.sum(np.abs(st) * np.abs(n) * L / k2)
+        assert abs(de) < 1e-13 * scale, (de, scale)
+    g2 = O.grid2(32, 32)
+    kx2, ky2, _, L2 = O.tables(g2)
+    k22 = np.maximum(kx2 ** 2 + ky2 ** 2, 1e-3)
+    s2 = O.NS2D(32, famp=0.0)
+    for seed in range(3):
+        z = O.ic_ns2d_cfg(g2, 300 + seed)
+        n = s2.nonlin(z)
+        scale = np.sum(np.abs(z) * np.abs(n) * L2)
+        assert abs(np.sum((np.conj(z) * n).real * L2)) < 1e-13 * scale           # enstrophy
+        assert abs(np.sum((np.conj(z) * n).real * L2 / k22)) < 1e-13 * scale     # energy
+
+
+def test_boussinesq_buoyancy_exchange_conserves_total_energy():
+    # With diffusion and forcing off, kinetic energy sum|w|^2/|k|^2 and Ri sum|rho|^2
+    # exchange through the buoyancy terms only: -dKE/dt / d(sum|rho|^2)/dt = Ri for
+    # any state and any gravity angle theta (radians, as the reference passes it).
+    g = O.grid3(16, 16, 16)
+    kx, ky, kz, L = O.tables(g)
+    k2 = np.maximum(kx ** 2 + ky ** 2 + kz ** 2, 1e-3)
+    for theta in (np.pi / 2, 0.3, 90.0):
+        b = O.BOUS(16, v2=1e-14, sc=1.0, ri=1.7, theta=theta, dt=0.01, ampfor=0.0)
+        for seed in range(3):
+            st = O.ic_solenoidal(g, 100 + seed, 0.5, 0.3)
+            d = b.rhs(st)
+            dke = 2 * np.sum((np.conj(st[:3]) * d[:3]).real * L / k2)
+            dpe = 2 * np.sum((np.conj(st[3]) * d[3]).real * L)
+            assert abs(-dke / dpe - 1.7) < 1e-8, (theta, seed, -dke / dpe)
+
+
+def test_nonlinear_term_2d_matches_numpy():
+    # independent restatement with numpy FFTs: dw/dt = -(d/dx(u w) + d/dy(v w)),
+    # u = d psi/dy, v = -d psi/dx, psi = w/|k|^2, forcing -2 at (0, +-4)
+    n = 32
+    s = O.NS2D(n, famp=-2.0)
+    g = s.g
+    z = O.ic_ns2d_cfg(g, 17)
+    kx, ky, _, L = O.tables(g)
+    F = np.zeros((n, n), complex)
+    for i in np.where(L == 1)[0]:
+        a, b = int(kx[i]), int(ky[i])
+        F[b % n, a % n] += z[i]
+        F[(-b) % n, (-a) % n] += np.conj(z[i])
+    ky_, kx_ = np.meshgrid(np.fft.fftfreq(n, 1 / n), np.fft.fftfreq(n, 1 / n), indexing="ij")
+    k2 = np.maximum(kx_ ** 2 + ky_ ** 2, 1e-3)
+    psi = F / k2
+    u = np.fft.ifft2(1j * ky_ * psi).real * n * n
+    v = np.fft.ifft2(-1j * kx_ * psi).real * n * n
+    w = np.fft.ifft2(F).real * n * n
+    N = -(1j * kx_ * np.fft.fft2(u * w) + 1j * ky_ * np.fft.fft2(v * w)) / (n * n)
+    want = np.zeros_like(z)
+    for i in np.where(L == 1)[0]:
+        a, b = int(kx[i]), int(ky[i])
+        want[i] = N[b % n, a % n]
+        if a == 0 and abs(b) == 4:
+            want[i] += -2.0
+    got = s.nonlin(z)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
