@@ -134,60 +134,50 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-6):
 
 
 def newton_case(P, n=512, t_spin=100.0, t_window=200.0, dt_sample=0.25, nguess=8, max_newton=60,
-                tmin=5.0, tmax=20.0, threads=4):
+                tmin=5.0, tmax=20.0):
     """2D Kolmogorov flow n^2, Re=40, sin(4y) forcing, dt=0.005: spin up a seeded
     random-phase state, take the closest shift-minimised near-recurrences with
     T in [tmin, tmax] as guesses, and run Newton-GMRES-hookstep (Krylov 40,
     residual 1e-10) on all guesses with finite-difference Jv, then on the same
-    guesses with tangent-linear Jv.  Guesses run concurrently, one context and
-    HIP stream each, `threads` at a time; s/orbit = batch wall time / converged."""
-    from concurrent.futures import ThreadPoolExecutor
+    guesses with tangent-linear Jv, one guess after the other on one context;
+    s/orbit = total time / converged."""
     from psgpu_amd import recurrence as R
     g = P.hgrid("ns2d", n, n)
     L = P.tables(g)[3]
     z0 = P.ic_ns2d_cfg(g, 20240607, 1.0)
     out = {"grid": [n, n], "re": 40.0, "dt": 0.005, "t_spin": t_spin, "window": t_window, "krylov": 40,
-           "netol": 1e-10, "max_newton": max_newton, "concurrent": threads}
+           "netol": 1e-10, "max_newton": max_newton}
     kw = dict(flavour="ns2d", n=n, v2=1 / 40.0, dt=0.005, kf=4.0, ampfor=-2.0)
     with P.Stepper(**kw) as s:
         t0 = time.perf_counter()
         times, snaps = R.record(s, z0, t_spin, t_window, dt_sample)
         out["record_seconds"] = time.perf_counter() - t0
         out["dns_steps_per_s"] = (t_spin + t_window) / 0.005 / out["record_seconds"]
-    t0 = time.perf_counter()
-    hits = R.best_recurrences(times, snaps, g.iktx, L, tmin, tmax, top=nguess)
-    out["search_seconds"] = time.perf_counter() - t0
-    out["guesses"] = hits
-    log("bench: newton guesses " + ", ".join("t=%.2f T=%.3f s=%.3f d=%.3f" % (h["t"], h["T"], h["shift"],
-                                                                              h["dist"]) for h in hits))
-    guesses = [snaps[h["i"]].copy() for h in hits]
-    del snaps
-
-    def solve(args):
-        k, jv = args
-        h = hits[k]
-        with P.Stepper(**kw) as st:
-            z, res = st.newton(guesses[k][None], T=h["T"], sx=h["shift"], jv=jv, netol=1e-10, max_gmres=40,
-                               max_newton=max_newton, dt0=0.005, verbose=1)
-        res["jv"] = "tangent-linear" if jv else "finite-difference"
-        res["guess"] = k
-        log("bench: newton guess %d (T=%.3f) %s: %s ner=%.2e T=%.6f in %.1f s, %d iterations" %
-            (k, h["T"], res["jv"], res["status_name"], res["ner"], res["T"], res["seconds"], res["newton_iters"]))
-        return res
-
-    runs = []
-    for jv in (0, 1):
-        name = "tangent-linear" if jv else "finite-difference"
         t0 = time.perf_counter()
-        with ThreadPoolExecutor(max_workers=threads) as ex:
-            rs = list(ex.map(solve, [(k, jv) for k in range(len(hits))]))
-        wall = time.perf_counter() - t0
-        runs += rs
-        conv = [r for r in rs if r["status_name"] == "CONVERGED"]
-        out[name] = {"converged": len(conv), "tried": len(rs), "wall_seconds": wall,
-                     "s_per_orbit": wall / len(conv) if conv else None,
-                     "newton_iters": [r["newton_iters"] for r in rs], "status": [r["status_name"] for r in rs]}
-    out["runs"] = runs
+        hits = R.best_recurrences(times, snaps, g.iktx, L, tmin, tmax, top=nguess)
+        out["search_seconds"] = time.perf_counter() - t0
+        out["guesses"] = hits
+        log("bench: newton guesses " + ", ".join("t=%.2f T=%.3f s=%.3f d=%.3f" % (h["t"], h["T"], h["shift"],
+                                                                                  h["dist"]) for h in hits))
+        runs = []
+        for jv in (0, 1):
+            name = "tangent-linear" if jv else "finite-difference"
+            rs = []
+            for k, h in enumerate(hits):
+                z, res = s.newton(snaps[h["i"]][None], T=h["T"], sx=h["shift"], jv=jv, netol=1e-10, max_gmres=40,
+                                  max_newton=max_newton, dt0=0.005, verbose=1)
+                res["jv"] = name
+                res["guess"] = k
+                log("bench: newton guess %d (T=%.3f) %s: %s ner=%.2e T=%.6f in %.1f s, %d iterations" %
+                    (k, h["T"], name, res["status_name"], res["ner"], res["T"], res["seconds"], res["newton_iters"]))
+                rs.append(res)
+            runs += rs
+            conv = [r for r in rs if r["status_name"] == "CONVERGED"]
+            total = sum(r["seconds"] for r in rs)
+            out[name] = {"converged": len(conv), "tried": len(rs), "seconds_total": total,
+                         "s_per_orbit": total / len(conv) if conv else None,
+                         "newton_iters": [r["newton_iters"] for r in rs], "status": [r["status_name"] for r in rs]}
+        out["runs"] = runs
     return out
 
 

@@ -44,6 +44,11 @@ double psgh_ranno0(void);
 /* reference random ICs: 3D_NavierStokes/src/init.F90:151-230, 2D_NavierStokes/src_RK4/init.F90:138-208 */
 void psgh_ic_ns3d_ref(const psgh_grid *g, int iseed, double ampv, double *xii, double *eta, double *zet);
 void psgh_ic_ns2d_ref(const psgh_grid *g, int iseed, double ampv, double *z);
+/* 3D_Euler/src/init.F90:137-190 (Taylor-Green pair plus a random kz = 1 shell in zet) and
+ * 3D_Boussinesq/src/init.F90:234-305 (solenoidal random shell, rho = 0) */
+void psgh_ic_euler_ref(const psgh_grid *g, int iseed, double ampv, double *xii, double *eta, double *zet);
+void psgh_ic_bous_ref(const psgh_grid *g, int iseed, double ampv, double *xii, double *eta, double *zet,
+                      double *rho);
 /* benchmark ICs (SplitMix64 + Box-Muller): solenoidal velocity with E(k) ~ k^4 exp(-2(k/4)^2),
  * sum_L|u|^2 = energy, w = i k x u; optional buoyancy with sum|rho|^2 = rho_scale^2 energy */
 void psgh_ic_solenoidal(const psgh_grid *g, uint64_t seed, double energy, double rho_scale,

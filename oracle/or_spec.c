@@ -219,6 +219,87 @@ void or_ic_ns3d_ref(const or_grid *g, int iseed, double ampv, orc *xii, orc *eta
   free(kx); free(ky); free(kz); free(L);
 }
 
+/* Reference 3D Euler IC: 3D_Euler/src/init.F90:137-190.  Loop order IKX, IKY,
+ * IKZ over every stored mode (no mask); one phase draw per mode;
+ * zet = EK + PK e^{i phase} with the Taylor-Green pair EK = +-1/4 at
+ * (kx = alpha, ky = +-1, kz = 0) and PK = AMPV on 1.5 < |k| <= 3.5, kz = 1;
+ * xii = eta = 0; no normalisation. */
+void or_ic_euler_ref(const or_grid *g, int iseed, double ampv, orc *xii, orc *eta, orc *zet) {
+  double *kx = malloc(sizeof(double) * g->nkt), *ky = malloc(sizeof(double) * g->nkt),
+         *kz = malloc(sizeof(double) * g->nkt);
+  int *L = malloc(sizeof(int) * g->nkt);
+  or_tables(g, kx, ky, kz, L);
+  or_rng_seed(iseed);
+  memset(xii, 0, sizeof(orc) * g->nkt);
+  memset(eta, 0, sizeof(orc) * g->nkt);
+  memset(zet, 0, sizeof(orc) * g->nkt);
+  for (int ikx = 0; ikx < g->iktx; ikx++)
+    for (int iky = 0; iky < g->ikty; iky++)
+      for (int ikz = 0; ikz < g->iktz; ikz++) {
+        long ik = ((long)ikz * g->ikty + iky) * g->iktx + ikx;
+        double wk = sqrt(kx[ik] * kx[ik] + ky[ik] * ky[ik] + kz[ik] * kz[ik]);
+        double pk = (wk > 1.5 && wk <= 3.5 && kz[ik] == 1.0) ? ampv : 0.0;
+        double ek = 0.0;
+        if (ikx == 1 && ky[ik] == 1.0 && kz[ik] == 0.0) ek = 0.25;
+        else if (ikx == 1 && ky[ik] == -1.0 && kz[ik] == 0.0) ek = -0.25;
+        double s, c, ph = or_ranno0();
+        sincos(ph, &s, &c);
+        zet[ik].re = ek + pk * c;
+        zet[ik].im = pk * s;
+      }
+  free(kx); free(ky); free(kz); free(L);
+}
+
+/* Reference 3D Boussinesq IC: 3D_Boussinesq/src/init.F90:234-305.  Loop order
+ * IKX, IKY, IKZ over every stored mode; three phase draws per mode (the third is
+ * drawn and unused); xii, eta = 0.1 e^{i phase} on 2.5 < |k| <= 9.5,
+ * zet = -(kx xii + ky eta)/kz (all three zero where kz = 0); the enstrophy of
+ * every stored mode, 2 sum|w|^2, scales the active modes to AMPV; rho = 0. */
+void or_ic_bous_ref(const or_grid *g, int iseed, double ampv, orc *xii, orc *eta, orc *zet, orc *rho) {
+  double *kx = malloc(sizeof(double) * g->nkt), *ky = malloc(sizeof(double) * g->nkt),
+         *kz = malloc(sizeof(double) * g->nkt);
+  int *L = malloc(sizeof(int) * g->nkt);
+  or_tables(g, kx, ky, kz, L);
+  or_rng_seed(iseed);
+  memset(rho, 0, sizeof(orc) * g->nkt);
+  double enstro = 0.0;
+  for (int ikx = 0; ikx < g->iktx; ikx++)
+    for (int iky = 0; iky < g->ikty; iky++)
+      for (int ikz = 0; ikz < g->iktz; ikz++) {
+        long ik = ((long)ikz * g->ikty + iky) * g->iktx + ikx;
+        double wk = sqrt(kx[ik] * kx[ik] + ky[ik] * ky[ik] + kz[ik] * kz[ik]);
+        double ek = (wk > 2.5 && wk <= 9.5) ? 0.1 : 0.0;
+        double s, c;
+        sincos(or_ranno0(), &s, &c);
+        xii[ik].re = ek * c; xii[ik].im = ek * s;
+        sincos(or_ranno0(), &s, &c);
+        eta[ik].re = ek * c; eta[ik].im = ek * s;
+        (void)or_ranno0();
+        if (kz[ik] != 0.0) {
+          zet[ik].re = -(kx[ik] * xii[ik].re + ky[ik] * eta[ik].re) / kz[ik];
+          zet[ik].im = -(kx[ik] * xii[ik].im + ky[ik] * eta[ik].im) / kz[ik];
+        } else {
+          xii[ik].re = xii[ik].im = eta[ik].re = eta[ik].im = zet[ik].re = zet[ik].im = 0.0;
+        }
+        double r1 = xii[ik].re * xii[ik].re + eta[ik].re * eta[ik].re + zet[ik].re * zet[ik].re;
+        double r2 = xii[ik].im * xii[ik].im + eta[ik].im * eta[ik].im + zet[ik].im * zet[ik].im;
+        enstro = enstro + r1 + r2;
+      }
+  enstro *= 2.0;
+  if (enstro != 0.0) {
+    double f = sqrt(ampv / enstro);
+    for (long ik = 0; ik < g->nkt; ik++) {
+      if (L[ik]) {
+        xii[ik].re *= f; xii[ik].im *= f; eta[ik].re *= f; eta[ik].im *= f;
+        zet[ik].re *= f; zet[ik].im *= f;
+      } else {
+        xii[ik].re = xii[ik].im = eta[ik].re = eta[ik].im = zet[ik].re = zet[ik].im = 0.0;
+      }
+    }
+  }
+  free(kx); free(ky); free(kz); free(L);
+}
+
 /* Reference 2D random IC: 2D_NavierStokes/src_RK4/init.F90:138-208.
  * EK=1/|k| on 2.5<|k|<=9.5 with kx>=0.1, VK=|k|^2 EK, Z=sqrt(VK/|k|) e^{i phase};
  * one phase draw per active mode (IKX outer, IKY inner); 2*sum|Z|^2 = AMPV. */

@@ -80,6 +80,8 @@ int    or_pm_next(int idum);             /* one Park-Miller step (Schrage) */
 /* reference random ICs (enstrophy-normalised random-phase shells) */
 void or_ic_ns3d_ref(const or_grid *g, int iseed, double ampv, orc *xii, orc *eta, orc *zet);
 void or_ic_ns2d_ref(const or_grid *g, int iseed, double ampv, orc *z);
+void or_ic_euler_ref(const or_grid *g, int iseed, double ampv, orc *xii, orc *eta, orc *zet);
+void or_ic_bous_ref(const or_grid *g, int iseed, double ampv, orc *xii, orc *eta, orc *zet, orc *rho);
 /* benchmark ICs: SplitMix64 Gaussian phases, documented spectra */
 void or_ic_solenoidal(const or_grid *g, uint64_t seed, double energy, double amp_scale,
                       orc *xii, orc *eta, orc *zet, orc *rho /* may be NULL */);

@@ -121,6 +121,8 @@ def lib():
     L.psgh_ranno0.restype = _D
     L.psgh_ic_ns3d_ref.argtypes = [hg, _I, _D, _P, _P, _P]
     L.psgh_ic_ns2d_ref.argtypes = [hg, _I, _D, _P]
+    L.psgh_ic_euler_ref.argtypes = [hg, _I, _D, _P, _P, _P]
+    L.psgh_ic_bous_ref.argtypes = [hg, _I, _D, _P, _P, _P, _P]
     L.psgh_ic_solenoidal.argtypes = [hg, C.c_uint64, _D, _D, _P, _P, _P, _P]
     L.psgh_ic_ns2d_cfg.argtypes = [hg, C.c_uint64, _D, _P]
     L.psgh_params_read.argtypes = [C.c_char_p, _I, C.POINTER(KV), C.c_char_p, _I]
@@ -192,6 +194,20 @@ def nutab(g, v2, dt):
 def ic_ns3d_ref(g, iseed, ampv=1.0):
     s = np.zeros((3, g.nkt), np.complex128)
     lib().psgh_ic_ns3d_ref(C.byref(g), iseed, ampv, _ptr(s[0]), _ptr(s[1]), _ptr(s[2]))
+    return s
+
+
+def ic_euler_ref(g, iseed, ampv=1.0):
+    """3D Euler IC (3D_Euler/src/init.F90:137-190): Taylor-Green pair plus a random kz=1 shell in zet."""
+    s = np.zeros((3, g.nkt), np.complex128)
+    lib().psgh_ic_euler_ref(C.byref(g), iseed, ampv, _ptr(s[0]), _ptr(s[1]), _ptr(s[2]))
+    return s
+
+
+def ic_bous_ref(g, iseed, ampv=1.0):
+    """3D Boussinesq IC (3D_Boussinesq/src/init.F90:234-305): (xii, eta, zet, rho), rho = 0."""
+    s = np.zeros((4, g.nkt), np.complex128)
+    lib().psgh_ic_bous_ref(C.byref(g), iseed, ampv, _ptr(s[0]), _ptr(s[1]), _ptr(s[2]), _ptr(s[3]))
     return s
 
 

@@ -214,6 +214,8 @@ static int upload_twiddles(Ctx *c) {
 
 static void free_ctx(Ctx *c) {
   if (!c) return;
+  // no launch of a graph may be in flight when its executable is destroyed
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->graph_exec)
     if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
   void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
@@ -258,6 +260,7 @@ static int run_eager(Ctx *c, int kind, int n, bool first) {
 // eager launches for good.
 static int build_graph(Ctx *c, int kind) {
   if (c->graph_exec[kind]) {
+    PSG_HIP(hipStreamSynchronize(c->stream));   // the old executable may still be running
     (void)hipGraphExecDestroy(c->graph_exec[kind]);
     c->graph_exec[kind] = nullptr;
   }

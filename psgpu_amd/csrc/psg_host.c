@@ -200,6 +200,79 @@ void psgh_ic_ns3d_ref(const psgh_grid *g, int iseed, double ampv, double *xii, d
   free(kx); free(ky); free(kz); free(L);
 }
 
+/* 3D_Euler/src/init.F90:137-190: every stored mode draws one phase;
+ * zet = EK + PK e^{i phase}, EK = +-1/4 at (kx = alpha, ky = +-1, kz = 0) (the Taylor-Green pair),
+ * PK = AMPV on 1.5 < |k| <= 3.5 with kz = 1; xii = eta = 0; no mask and no normalisation */
+void psgh_ic_euler_ref(const psgh_grid *g, int iseed, double ampv, double *xii, double *eta, double *zet) {
+  const long n = g->nkt;
+  double *kx = malloc(sizeof(double) * n), *ky = malloc(sizeof(double) * n), *kz = malloc(sizeof(double) * n);
+  psgh_tables(g, kx, ky, kz, NULL);
+  memset(xii, 0, sizeof(double) * 2 * n);
+  memset(eta, 0, sizeof(double) * 2 * n);
+  psgh_rng_seed(iseed);
+  for (int ikx = 0; ikx < g->iktx; ikx++)
+    for (int iky = 0; iky < g->ikty; iky++)
+      for (int ikz = 0; ikz < g->iktz; ikz++) {
+        const long ik = ((long)ikz * g->ikty + iky) * g->iktx + ikx;
+        const double wk = sqrt(kx[ik] * kx[ik] + ky[ik] * ky[ik] + kz[ik] * kz[ik]);
+        const double pk = (wk > 1.5 && wk <= 3.5 && kz[ik] == 1.0) ? ampv : 0.0;
+        const double ek = (ikx == 1 && kz[ik] == 0.0) ? (ky[ik] == 1.0 ? 0.25 : ky[ik] == -1.0 ? -0.25 : 0.0) : 0.0;
+        double s, co;
+        sincos(psgh_ranno0(), &s, &co);
+        zet[2 * ik] = ek + pk * co;
+        zet[2 * ik + 1] = pk * s;
+      }
+  free(kx); free(ky); free(kz);
+}
+
+/* 3D_Boussinesq/src/init.F90:234-305: every stored mode draws three phases (the third unused);
+ * xii, eta = 0.1 e^{i phase} on 2.5 < |k| <= 9.5, zet = -(kx xii + ky eta)/kz, all three zero
+ * where kz = 0; 2 sum|w|^2 over every stored mode normalised to AMPV on the mask; rho = 0 */
+void psgh_ic_bous_ref(const psgh_grid *g, int iseed, double ampv, double *xii, double *eta, double *zet,
+                      double *rho) {
+  const long n = g->nkt;
+  double *kx = malloc(sizeof(double) * n), *ky = malloc(sizeof(double) * n), *kz = malloc(sizeof(double) * n);
+  int *L = malloc(sizeof(int) * n);
+  psgh_tables(g, kx, ky, kz, L);
+  memset(rho, 0, sizeof(double) * 2 * n);
+  psgh_rng_seed(iseed);
+  double ens = 0.0;
+  for (int ikx = 0; ikx < g->iktx; ikx++)
+    for (int iky = 0; iky < g->ikty; iky++)
+      for (int ikz = 0; ikz < g->iktz; ikz++) {
+        const long ik = ((long)ikz * g->ikty + iky) * g->iktx + ikx;
+        const double wk = sqrt(kx[ik] * kx[ik] + ky[ik] * ky[ik] + kz[ik] * kz[ik]);
+        const double ek = (wk > 2.5 && wk <= 9.5) ? 0.1 : 0.0;
+        double s1, c1, s2, c2;
+        sincos(psgh_ranno0(), &s1, &c1);
+        sincos(psgh_ranno0(), &s2, &c2);
+        (void)psgh_ranno0();
+        double *w[3] = {xii + 2 * ik, eta + 2 * ik, zet + 2 * ik};
+        if (kz[ik] != 0.0) {
+          w[0][0] = ek * c1; w[0][1] = ek * s1;
+          w[1][0] = ek * c2; w[1][1] = ek * s2;
+          w[2][0] = -(kx[ik] * w[0][0] + ky[ik] * w[1][0]) / kz[ik];
+          w[2][1] = -(kx[ik] * w[0][1] + ky[ik] * w[1][1]) / kz[ik];
+        } else {
+          for (int c = 0; c < 3; c++) w[c][0] = w[c][1] = 0.0;
+        }
+        const double r1 = w[0][0] * w[0][0] + w[1][0] * w[1][0] + w[2][0] * w[2][0];
+        const double r2 = w[0][1] * w[0][1] + w[1][1] * w[1][1] + w[2][1] * w[2][1];
+        ens = ens + r1 + r2;
+      }
+  ens *= 2.0;
+  if (ens != 0.0) {
+    const double sc = sqrt(ampv / ens);
+    double *f[3] = {xii, eta, zet};
+    for (long ik = 0; ik < n; ik++)
+      for (int c = 0; c < 3; c++) {
+        f[c][2 * ik] = L[ik] ? f[c][2 * ik] * sc : 0.0;
+        f[c][2 * ik + 1] = L[ik] ? f[c][2 * ik + 1] * sc : 0.0;
+      }
+  }
+  free(kx); free(ky); free(kz); free(L);
+}
+
 void psgh_ic_ns2d_ref(const psgh_grid *g, int iseed, double ampv, double *z) {
   const long n = g->nkt;
   double *kx = malloc(sizeof(double) * n), *ky = malloc(sizeof(double) * n);

@@ -58,6 +58,8 @@ def _declare(L):
     L.or_pm_next.restype = I
     L.or_ic_ns3d_ref.argtypes = [C.POINTER(Grid), I, D, P, P, P]
     L.or_ic_ns2d_ref.argtypes = [C.POINTER(Grid), I, D, P]
+    L.or_ic_euler_ref.argtypes = [C.POINTER(Grid), I, D, P, P, P]
+    L.or_ic_bous_ref.argtypes = [C.POINTER(Grid), I, D, P, P, P, P]
     L.or_ic_solenoidal.argtypes = [C.POINTER(Grid), C.c_uint64, D, D, P, P, P, P]
     L.or_ic_ns2d_cfg.argtypes = [C.POINTER(Grid), C.c_uint64, D, P]
     L.or_gauss.argtypes = [C.c_uint64, C.c_uint64]
@@ -163,6 +165,18 @@ def ikf_ikn(g):
 def ic_ns3d_ref(g, iseed, ampv=1.0):
     s = np.zeros((3, g.nkt), np.complex128)
     lib().or_ic_ns3d_ref(C.byref(g), iseed, ampv, ptr(s[0]), ptr(s[1]), ptr(s[2]))
+    return s
+
+
+def ic_euler_ref(g, iseed, ampv=1.0):
+    s = np.zeros((3, g.nkt), np.complex128)
+    lib().or_ic_euler_ref(C.byref(g), iseed, ampv, ptr(s[0]), ptr(s[1]), ptr(s[2]))
+    return s
+
+
+def ic_bous_ref(g, iseed, ampv=1.0):
+    s = np.zeros((4, g.nkt), np.complex128)
+    lib().or_ic_bous_ref(C.byref(g), iseed, ampv, ptr(s[0]), ptr(s[1]), ptr(s[2]), ptr(s[3]))
     return s
 
 

@@ -47,6 +47,8 @@ def test_tables_match_oracle(dims):
 def test_ics_match_oracle_bitwise():
     g, go = P.hgrid("ns3d", 32, 32, 32), O.grid3(32, 32, 32)
     assert np.array_equal(P.ic_ns3d_ref(g, 837920), O.ic_ns3d_ref(go, 837920))
+    assert np.array_equal(P.ic_euler_ref(g, 837920, 0.3), O.ic_euler_ref(go, 837920, 0.3))
+    assert np.array_equal(P.ic_bous_ref(g, 4242, 2.0), O.ic_bous_ref(go, 4242, 2.0))
     assert np.array_equal(P.ic_solenoidal(g, 99), O.ic_solenoidal(go, 99))
     assert np.array_equal(P.ic_solenoidal(g, 99, 0.5, 0.1), O.ic_solenoidal(go, 99, 0.5, 0.1))
     g2, go2 = P.hgrid("ns2d", 64, 64), O.grid2(64, 64)

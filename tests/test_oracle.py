@@ -133,6 +133,61 @@ def test_ic_reference_enstrophy():
     assert np.abs(z[kx == 0]).max() == 0.0                  # kx=0 modes zeroed
 
 
+def _phases(seed, n):
+    O.lib().or_rng_seed(seed)
+    return np.array([O.lib().or_ranno0() for _ in range(n)])
+
+
+def _loop_order(g):
+    # position of each stored mode in the reference's IKX, IKY, IKZ (IKZ innermost) loop
+    ikz, iky, ikx = np.meshgrid(np.arange(g.iktz), np.arange(g.ikty), np.arange(g.iktx), indexing="ij")
+    return ((ikx * g.ikty + iky) * g.iktz + ikz).ravel()
+
+
+def test_ic_euler_reference_law():
+    g = O.grid3(16, 16, 16)
+    ampv = 0.3
+    s = O.ic_euler_ref(g, 837920, ampv)
+    kx, ky, kz, L = O.tables(g)
+    wk = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    assert np.abs(s[:2]).max() == 0.0                                       # xii = eta = 0
+    tg = (kx == 1) & (kz == 0) & (np.abs(ky) == 1)
+    assert np.array_equal(s[2, tg].real, 0.25 * ky[tg]) and np.abs(s[2, tg].imag).max() == 0.0
+    shell = (wk > 1.5) & (wk <= 3.5) & (kz == 1)
+    assert shell.sum() > 10 and np.abs(np.abs(s[2, shell]) - ampv).max() < 1e-15
+    assert np.abs(s[2, ~(shell | tg)]).max() == 0.0
+    # one phase per stored mode, in loop order: the shell phases are those draws
+    ph = _phases(837920, g.nkt)[_loop_order(g)]
+    assert np.abs(s[2, shell] - ampv * np.exp(1j * ph[shell])).max() < 1e-15
+
+
+def test_ic_boussinesq_reference_law():
+    g = O.grid3(16, 16, 16)
+    ampv = 2.0
+    s = O.ic_bous_ref(g, 4242, ampv)
+    kx, ky, kz, L = O.tables(g)
+    wk = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    assert np.abs(s[3]).max() == 0.0                                        # rho = 0
+    assert np.abs(kx * s[0] + ky * s[1] + kz * s[2]).max() < 1e-15          # k . w = 0
+    on = (wk > 2.5) & (wk <= 9.5) & (kz != 0) & (L == 1)
+    assert np.abs(s[:, ~on]).max() == 0.0
+    a = np.abs(s[0, on])
+    assert np.abs(a - a[0]).max() < 1e-15 and np.abs(np.abs(s[1, on]) - a[0]).max() < 1e-15
+    # three draws per stored mode (the third unused): xii and eta carry draws 3p and 3p+1
+    ph = _phases(4242, 3 * g.nkt)
+    p = _loop_order(g)
+    assert np.abs(s[0, on] - a[0] * np.exp(1j * ph[3 * p[on]])).max() < 1e-14
+    assert np.abs(s[1, on] - a[0] * np.exp(1j * ph[3 * p[on] + 1])).max() < 1e-14
+    # the enstrophy that sets the amplitude runs over every stored mode, the
+    # conjugate duplicates at kx = 0 included, before the mask is applied
+    act = (wk > 2.5) & (wk <= 9.5) & (kz != 0)
+    x0 = 0.1 * np.exp(1j * ph[3 * p])
+    e0 = 0.1 * np.exp(1j * ph[3 * p + 1])
+    z0 = np.where(kz != 0, -(kx * x0 + ky * e0) / np.where(kz != 0, kz, 1), 0)
+    ens = 2 * np.sum(act * (np.abs(x0) ** 2 + np.abs(e0) ** 2 + np.abs(z0) ** 2))
+    assert abs(a[0] - 0.1 * np.sqrt(ampv / ens)) < 1e-15
+
+
 def test_single_mode_cn_decay():
     # a single Fourier mode has zero nonlinear term: one Heun+CN step is (1-a)/(1+a)
     s = O.NS3D(16, v2=0.05, dt=0.01, force=0)
