@@ -21,7 +21,7 @@ template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
             const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask, int nout) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
@@ -234,7 +234,7 @@ static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, in
     constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);
+    size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),

@@ -11,10 +11,16 @@
 //
 // LDS addressing is lds[pos*stride + off]: column tiles use stride = tile
 // width and off = column (consecutive lanes hit consecutive columns), row
-// teams use stride = 1 and off = team base.
+// teams use stride = 1 and off = team base, with positions permuted by an XOR
+// swizzle (lpos) that keeps both the contiguous reads and the strided
+// Stockham writes of a wavefront on distinct LDS banks.
 //
 // Twiddles come from a per-length table W_N^k = e^{-2 pi i k/N} built on the
-// host in FP64 (glibc sincos) and rounded once for the FP32 table.
+// host in FP64 (glibc sincos) and rounded once for the FP32 table.  Kernels
+// stage them into LDS per stage and thread (tw_base): the R-1 twiddles a
+// butterfly needs are adjacent, so lanes with different k read different
+// banks instead of the bank-aligned strides r*k*N/(NS*R) of the natural
+// table.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -118,9 +124,8 @@ template <typename T, bool INV, int R> __device__ __forceinline__ void dftR(cpx<
 // Synchronisation/layout policy of a pencil team:
 //   WAVE: the team lives inside one wavefront, so LDS exchanges only need a
 //         wavefront-scope fence (no block barrier);
-//   PAD : LDS position p is stored at p + p/8, which spreads the radix-8
-//         Stockham writes (stride 8 complex) over all banks.  Used by row
-//         teams (stride 1); column tiles are conflict-free without it.
+//   SW  : row-team swizzle of LDS positions (lpos); 0 for column tiles,
+//         whose lanes already address consecutive columns.
 template <bool WAVE> __device__ __forceinline__ void team_sync() {
   if constexpr (WAVE) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -130,9 +135,32 @@ template <bool WAVE> __device__ __forceinline__ void team_sync() {
     __syncthreads();
   }
 }
-template <bool PAD> __device__ __forceinline__ int lpos(int p) { return PAD ? p + (p >> 3) : p; }
+// Row-team LDS position of point p under swizzle SW (0: none): bit T+i of p
+// is flipped by bit S+i, i < 4, SW = S | T << 4.  With S > T every flipped
+// bit is below the bit that flips it, so this permutes 0..N-1 for any
+// power-of-two N (the inverse restores bits from the top down).
+template <int SW> __device__ __forceinline__ int lpos(int p) {
+  if constexpr (SW == 0) return p;
+  else return p ^ (((p >> (SW & 15)) & 15) << (SW >> 4));
+}
+// swizzle of a row team by element size and points per thread
+template <typename T, int E> constexpr int row_swizzle() {
+  if constexpr (sizeof(T) == 8) return E == 4 ? 2 : (E == 16 ? 4 : 3);   // 16-byte points
+  else return E == 4 ? (3 | 1 << 4) : (E == 16 ? 4 : (4 | 1 << 4));      // 8-byte points
+}
 
-template <typename T, int N, int E, int NS, bool INV, bool WAVE = false, bool PAD = false>
+// Stage-major packed twiddles: stage NS (radix R, not the first) stores
+// W_N^{r k N/(NS R)}, r = 1..R-1, at tw_base + k (R-1) + r - 1.  Fewer than N
+// entries in total.
+constexpr int radix_at(int N, int E, int NS) { return (N / NS >= E) ? E : N / NS; }
+constexpr int tw_base(int N, int E, int NS) {
+  int base = 0;
+  for (int ns = radix_at(N, E, 1); ns < NS; ns *= radix_at(N, E, ns)) base += ns * (radix_at(N, E, ns) - 1);
+  return base;
+}
+constexpr int tw_count(int N, int E) { return tw_base(N, E, N); }
+
+template <typename T, int N, int E, int NS, bool INV, bool WAVE = false, int SW = 0>
 __device__ __forceinline__ void fft_stage(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
                                           const cpx<T> *__restrict__ tw) {
   constexpr int TP = N / E;
@@ -144,7 +172,7 @@ __device__ __forceinline__ void fft_stage(cpx<T> (&v)[E], cpx<T> *lds, int j, in
   if constexpr (!FIRST) {
     team_sync<WAVE>();
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = lds[lpos<PAD>(j + TP * m) * stride + off];
+    for (int m = 0; m < E; m++) v[m] = lds[lpos<SW>(j + TP * m) * stride + off];
   }
 #pragma unroll
   for (int t = 0; t < G; t++) {
@@ -156,7 +184,7 @@ __device__ __forceinline__ void fft_stage(cpx<T> (&v)[E], cpx<T> *lds, int j, in
     if constexpr (!FIRST) {
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        cpx<T> w = tw[(r * k) * (N / (NS * R))];
+        cpx<T> w = tw[tw_base(N, E, NS) + k * (R - 1) + r - 1];
         u[r] = INV ? mulc(u[r], w) : mul(u[r], w);
       }
     }
@@ -173,31 +201,31 @@ __device__ __forceinline__ void fft_stage(cpx<T> (&v)[E], cpx<T> *lds, int j, in
       const int k = b % NS;
       const int d = (b / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) lds[lpos<PAD>(d + r * NS) * stride + off] = v[t + r * G];
+      for (int r = 0; r < R; r++) lds[lpos<SW>(d + r * NS) * stride + off] = v[t + r * G];
     }
   }
 }
 
-template <typename T, int N, int E, int NS, bool INV, bool WAVE, bool PAD>
+template <typename T, int N, int E, int NS, bool INV, bool WAVE, int SW>
 __device__ __forceinline__ void fft_stages(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
                                            const cpx<T> *__restrict__ tw) {
   if constexpr (NS < N) {
     constexpr int R = (N / NS >= E) ? E : N / NS;
-    fft_stage<T, N, E, NS, INV, WAVE, PAD>(v, lds, j, stride, off, tw);
-    fft_stages<T, N, E, NS * R, INV, WAVE, PAD>(v, lds, j, stride, off, tw);
+    fft_stage<T, N, E, NS, INV, WAVE, SW>(v, lds, j, stride, off, tw);
+    fft_stages<T, N, E, NS * R, INV, WAVE, SW>(v, lds, j, stride, off, tw);
   }
 }
 
 // Full N-point transform of the pencil held by a team (see header comment).
 // Every thread of the block must call this (it contains block barriers).
-template <typename T, int N, int E, bool INV, bool WAVE = false, bool PAD = false>
+template <typename T, int N, int E, bool INV, bool WAVE = false, int SW = 0>
 __device__ __forceinline__ void fft_pencil(cpx<T> (&v)[E], cpx<T> *lds, int j, int stride, int off,
                                            const cpx<T> *__restrict__ tw) {
   static_assert((N & (N - 1)) == 0 && N >= E, "power-of-two pencil");
-  fft_stages<T, N, E, 1, INV, WAVE, PAD>(v, lds, j, stride, off, tw);
+  fft_stages<T, N, E, 1, INV, WAVE, SW>(v, lds, j, stride, off, tw);
 }
 
-// LDS complex slots per padded row of N points
-template <int N> constexpr int padded_len() { return N + N / 8; }
+// LDS complex slots per row-team row of N points
+template <int N> constexpr int row_len() { return N; }
 
 }  // namespace psg

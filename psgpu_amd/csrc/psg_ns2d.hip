@@ -113,7 +113,7 @@ static int launch_xpass2(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int
     constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);
+    size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
@@ -187,7 +187,7 @@ int ns2d_nonlin(Ctx *c, int, const cpx<double> *in_dev, cpx<double> *out_dev) {
 template <typename P, int N, int TE>
 __global__ void __launch_bounds__(TE *(N / EPT))
 k2_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, long nrow, Grid g, const cpx<P> *__restrict__ tw) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TPX = N / EPT;
   const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
   const long row = (long)blockIdx.x * TE + team;

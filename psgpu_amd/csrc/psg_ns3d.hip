@@ -235,7 +235,7 @@ template <typename P, int N, int TE>
 __global__ void __launch_bounds__(TE *(N / EPT))
 k_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, int nf, long nrow, Grid g,
         const cpx<P> *__restrict__ tw) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TPX = N / EPT;
   const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
   const long row = (long)blockIdx.x * TE + team;

@@ -76,7 +76,7 @@ static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int 
     constexpr int TE = (TEAMS_MAX > 0 && TEAMS_MAX < TE0) ? TEAMS_MAX : TE0;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)(padded_len<N>() * TE + N) * sizeof(cpx<P>);
+    size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD, E, FRONT, LAZY>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(TE * TPX), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0), prod);

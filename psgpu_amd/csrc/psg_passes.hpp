@@ -26,14 +26,25 @@ namespace psg {
 
 constexpr int EPT = 8;  // points per thread in every pencil transform
 
-// Every transform kernel keeps the N-entry twiddle table of its pencil length
-// at the start of its dynamic LDS: the block copies it from HBM once, and the
-// Stockham stages then read twiddles at LDS latency instead of through the
-// vector memory pipeline.  The kernel's data region starts N entries later
+// Every transform kernel keeps the twiddles of its pencil length at the start
+// of its dynamic LDS, repacked from the natural table W_N^k in HBM into the
+// per-stage order the Stockham stages read (tw_base in psg_fft.hpp), so
+// they are read at LDS latency and without bank conflicts.  The packed table
+// has fewer than N entries; the kernel's data region starts N entries in
 // (launches add N*sizeof(cpx<P>) to the LDS size).
-template <typename P, int N>
+template <typename P, int N, int E>
 __device__ __forceinline__ const cpx<P> *stage_twiddles(const cpx<P> *__restrict__ tw, cpx<P> *lds_tw) {
-  for (int i = threadIdx.x; i < N; i += blockDim.x) lds_tw[i] = tw[i];
+  constexpr int M = tw_count(N, E);
+  static_assert(M < N, "packed twiddles fit the reserved N entries");
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    int ns = radix_at(N, E, 1), base = 0;
+    while (i >= base + ns * (radix_at(N, E, ns) - 1)) {
+      base += ns * (radix_at(N, E, ns) - 1);
+      ns *= radix_at(N, E, ns);
+    }
+    const int R = radix_at(N, E, ns), q = i - base, k = q / (R - 1), r = q % (R - 1) + 1;
+    lds_tw[i] = tw[(r * k) * (N / (ns * R))];
+  }
   __syncthreads();
   return lds_tw;
 }
@@ -59,10 +70,10 @@ __device__ __forceinline__ TileId xcd_tile() {
 #ifndef PSG_DYN_SMEM
 #define PSG_DYN_SMEM(name) extern __shared__ char name[]
 #endif
-#define PSG_LDS_TWIDDLES(P, N)                                          \
+#define PSG_LDS_TWIDDLES(P, N, E)                                       \
   PSG_DYN_SMEM(smem_raw);                                              \
   cpx<P> *lds = reinterpret_cast<cpx<P> *>(smem_raw) + (N);            \
-  tw = stage_twiddles<P, N>(tw, reinterpret_cast<cpx<P> *>(smem_raw))
+  tw = stage_twiddles<P, N, E>(tw, reinterpret_cast<cpx<P> *>(smem_raw))
 
 template <typename P> __device__ __forceinline__ cpx<P> to(cpx<double> a) { return {(P)a.x, (P)a.y}; }
 __device__ __forceinline__ cpx<double> up(cpx<double> a) { return a; }
@@ -118,7 +129,7 @@ template <typename P, int N, int T, bool BS>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
        const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
@@ -170,7 +181,7 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
 template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
@@ -196,7 +207,7 @@ k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
 template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
@@ -222,7 +233,7 @@ k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
 template <typename P, int N, int T, int FO, class UPD>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   // up to 4 fields every input is requested before the first transform; the
   // 6-field tangent-linear/adjoint updates load each field before its transform
@@ -316,7 +327,7 @@ template <typename P, int N, int TEAMS, int FI, int FO, class PROD, int E = EPT,
 __global__ void __launch_bounds__(TEAMS *(N / E))
 k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
         const cpx<P> *__restrict__ tw, PROD prod) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, E);
   constexpr int TPX = N / E;
   constexpr bool WAVE = TPX <= 64;
   constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2;
@@ -325,7 +336,8 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   const long row = (long)blockIdx.x * TEAMS + team;
   const bool valid = row < nrows;
   const long m = valid ? row / nrow_m : 0, rr = valid ? row % nrow_m : 0;
-  cpx<P> *tl = lds + team * padded_len<N>();
+  cpx<P> *tl = lds + team * row_len<N>();
+  constexpr int SW = row_swizzle<P, E>();
   const int ix = g.iktx;
   auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + rr * g.xp; };
   cpx<P> vin[NV][E];
@@ -346,7 +358,7 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
     } else {
       XRow<P, N, E>::load(v, rowp(fa), rowp(fb < FI ? fb : fa), fb < FI, valid, j, ix);
     }
-    fft_pencil<P, N, E, true, WAVE, true>(v, tl, j, 1, 0, tw);
+    fft_pencil<P, N, E, true, WAVE, SW>(v, tl, j, 1, 0, tw);
 #pragma unroll
     for (int mm = 0; mm < E; mm++) { phys[fa][mm] = v[mm].x; phys[fb][mm] = v[mm].y; }
   }
@@ -383,16 +395,16 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
         v[mm] = cpx<P>{res[2 * q][mm], res[2 * q + 1][mm]};
       }
     }
-    fft_pencil<P, N, E, false, WAVE, true>(v, tl, j, 1, 0, tw);
+    fft_pencil<P, N, E, false, WAVE, SW>(v, tl, j, 1, 0, tw);
     team_sync<WAVE>();
 #pragma unroll
-    for (int mm = 0; mm < E; mm++) tl[lpos<true>(j + TPX * mm)] = v[mm];
+    for (int mm = 0; mm < E; mm++) tl[lpos<SW>(j + TPX * mm)] = v[mm];
     team_sync<WAVE>();
     if (valid) {
       cpx<P> *oa = out + ((long)m * FO + 2 * q) * g.ysz + rr * g.xp;
       cpx<P> *ob = out + ((long)m * FO + 2 * q + 1) * g.ysz + rr * g.xp;
       for (int k = j; k < ix; k += TPX) {
-        cpx<P> zk = tl[lpos<true>(k)], zc = tl[lpos<true>((N - k) & (N - 1))];
+        cpx<P> zk = tl[lpos<SW>(k)], zc = tl[lpos<SW>((N - k) & (N - 1))];
         const P h = (P)0.5;
         cpx<P> A{h * (zk.x + zc.x), h * (zk.y - zc.y)};
         cpx<P> B{h * (zk.y + zc.y), h * (zc.x - zk.x)};
@@ -444,7 +456,7 @@ __global__ void __launch_bounds__(T *(N / EPT))
 k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out, long out_ms, Grid g,
         const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask,
         const cpx<double> *__restrict__ st2 = nullptr) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
@@ -504,7 +516,7 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
 template <typename P, int N, int T, class UPD, int NF = 2>
 __global__ void __launch_bounds__(T *(N / EPT))
 k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();

@@ -73,7 +73,7 @@ template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv_curl(const cpx<double> *__restrict__ st, long fs, cpx<P> *__restrict__ out, Grid g,
             const cpx<P> *__restrict__ tw, const double *__restrict__ tab, double fac) {
-  PSG_LDS_TWIDDLES(P, N);
+  PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
