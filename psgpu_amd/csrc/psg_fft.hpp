@@ -26,7 +26,9 @@
 
 namespace psg {
 
-template <typename T> struct cpx { T x, y; };
+// Aligned to its full size, so that one LDS or global access moves a whole
+// point (ds_read_b128 / global_load_dwordx4 in FP64) instead of two halves.
+template <typename T> struct alignas(2 * sizeof(T)) cpx { T x, y; };
 
 template <typename T> __device__ __forceinline__ cpx<T> mk(T a, T b) { return cpx<T>{a, b}; }
 template <typename T> __device__ __forceinline__ cpx<T> add(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
@@ -143,11 +145,12 @@ template <int SW> __device__ __forceinline__ int lpos(int p) {
   if constexpr (SW == 0) return p;
   else return p ^ (((p >> (SW & 15)) & 15) << (SW >> 4));
 }
-// swizzle of a row team by element size and points per thread
-template <typename T, int E> constexpr int row_swizzle() {
-  if constexpr (sizeof(T) == 8) return E == 4 ? 2 : (E == 16 ? 4 : 3);   // 16-byte points
-  else return E == 4 ? (3 | 1 << 4) : (E == 16 ? 4 : (4 | 1 << 4));      // 8-byte points
-}
+// Swizzle of a row team by points per thread.  gfx950 serves an LDS read of
+// a wavefront in groups of 16 (128-bit) or 32 (64-bit) lanes over 64 banks
+// and a write in groups of 8 or 16 lanes over 32 banks; these swizzles keep
+// the reads, the radix-E Stockham writes and the output exchange of both
+// precisions within one pass per group.
+template <int E> constexpr int row_swizzle() { return E == 4 ? 2 : (E == 16 ? 4 : 3); }
 
 // Stage-major packed twiddles: stage NS (radix R, not the first) stores
 // W_N^{r k N/(NS R)}, r = 1..R-1, at tw_base + k (R-1) + r - 1.  Fewer than N

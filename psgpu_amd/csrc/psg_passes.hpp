@@ -68,7 +68,7 @@ __device__ __forceinline__ TileId xcd_tile() {
 
 // dynamic LDS of a kernel (a CPU build for bounds checking supplies its own)
 #ifndef PSG_DYN_SMEM
-#define PSG_DYN_SMEM(name) extern __shared__ char name[]
+#define PSG_DYN_SMEM(name) extern __shared__ __attribute__((aligned(16))) char name[]
 #endif
 #define PSG_LDS_TWIDDLES(P, N, E)                                       \
   PSG_DYN_SMEM(smem_raw);                                              \
@@ -288,7 +288,8 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
 // Row teams: TEAMS rows per block, N/E threads per row.  Fields are paired
 // into complex transforms z = a + i b.  PROD::apply(double in[FI], double out[FO]).
 // A team that fits in one wavefront synchronises with wavefront fences only;
-// its LDS row is padded (p -> p + p/8) so radix-8 writes are bank-conflict free.
+// its LDS row is XOR-swizzled (lpos, row_swizzle) so that the strided radix-E
+// writes and the contiguous reads are bank-conflict free.
 // FRONT: every input of the row is requested before the first transform
 // (latency hiding for the 6-field products); without it each pair is loaded
 // just before its transform, which keeps the 9- and 12-field products of the
@@ -337,7 +338,7 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   const bool valid = row < nrows;
   const long m = valid ? row / nrow_m : 0, rr = valid ? row % nrow_m : 0;
   cpx<P> *tl = lds + team * row_len<N>();
-  constexpr int SW = row_swizzle<P, E>();
+  constexpr int SW = row_swizzle<E>();
   const int ix = g.iktx;
   auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + rr * g.xp; };
   cpx<P> vin[NV][E];

@@ -75,8 +75,8 @@ template <typename P, int N, int E, int SW, bool INV> static void one() {
 template <typename P, int N, int E> static void sizes() {
   one<P, N, E, 0, false>();
   one<P, N, E, 0, true>();
-  one<P, N, E, row_swizzle<P, E>(), false>();
-  one<P, N, E, row_swizzle<P, E>(), true>();
+  one<P, N, E, row_swizzle<E>(), false>();
+  one<P, N, E, row_swizzle<E>(), true>();
 }
 template <typename P, int E> static void lengths() {
   if constexpr (E <= 16) sizes<P, 16, E>();
