@@ -75,9 +75,9 @@ k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__r
     }
     fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
     if (valid) {
-      cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz + (long)iky * g.xp + ikx;
+      cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz;
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.ikty * g.xp] = v[mm];
+      for (int mm = 0; mm < EPT; mm++) o[zoff(g, ikx, iky, j + TP * mm)] = v[mm];
     }
   }
 }

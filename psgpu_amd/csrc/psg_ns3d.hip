@@ -241,15 +241,15 @@ k_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, int nf, long nr
   const long row = (long)blockIdx.x * TE + team;
   const bool valid = row < nrow;
   const int f = blockIdx.y;
-  const cpx<P> *ra = in + (long)f * g.ysz + (valid ? row : 0) * g.xp;
+  const cpx<P> *ra = in + (long)f * g.ysz + yrow(g, valid ? row : 0);
   cpx<P> v[EPT];
 #pragma unroll
   for (int mm = 0; mm < EPT; mm++) {
     const int k = j + TPX * mm;
     cpx<P> a{0, 0};
     if (valid) {
-      if (k < g.iktx) { a = ra[k]; if (k == 0) a.y = 0; }
-      else if (k > N - g.iktx) { a = ra[N - k]; a.y = -a.y; }
+      if (k < g.iktx) { a = ra[ycol(g, k)]; if (k == 0) a.y = 0; }
+      else if (k > N - g.iktx) { a = ra[ycol(g, N - k)]; a.y = -a.y; }
     }
     v[mm] = a;
   }

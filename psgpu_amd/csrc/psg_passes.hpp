@@ -85,6 +85,29 @@ __device__ __forceinline__ int wrapk(int p, int n, int kt) {
 }
 #define PSG_NOK (-0x40000000)
 
+// Pass-buffer layouts, per field.  kx columns are grouped in tiles of LW = 8
+// (xp/LW tiles), and each buffer is ordered so that the workgroups of the pass
+// that WRITES it fill whole tiles contiguously: scattered 128-byte row pieces
+// write at well under the HBM rate that contiguous tiles reach, while the
+// reading passes gather 128-byte pieces at close to full rate.
+//   Z  (z-inverse -> y-inverse):  [tile][iky][z][LW]
+//   Y  (y-inverse -> x-pass):     [z][tile][y][LW]
+//   Z' (y-forward -> z-forward):  [tile][z][iky][LW]
+//   C  (x-pass -> y-forward):     row-major [z][y][xp] (the x-pass writes rows)
+constexpr int LW = 8;
+__device__ __forceinline__ long zoff(const Grid &g, int ikx, int iky, int z) {
+  return ((long)(ikx >> 3) * g.ikty + iky) * ((long)g.nz * LW) + z * LW + (ikx & (LW - 1));
+}
+__device__ __forceinline__ long yoff(const Grid &g, int ikx, int y, int z) {
+  return ((long)z * (g.xp / LW) + (ikx >> 3)) * ((long)g.ny * LW) + y * LW + (ikx & (LW - 1));
+}
+__device__ __forceinline__ long zfoff(const Grid &g, int ikx, int iky, int z) {
+  return ((long)(ikx >> 3) * g.nz + z) * ((long)g.ikty * LW) + iky * LW + (ikx & (LW - 1));
+}
+// x row rr = z*ny + y of a Y buffer: element k sits at yrow(rr)[ycol(k)]
+__device__ __forceinline__ long yrow(const Grid &g, long rr) { return yoff(g, 0, (int)(rr % g.ny), (int)(rr / g.ny)); }
+__device__ __forceinline__ long ycol(const Grid &g, int k) { return (long)(k >> 3) * g.ny * LW + (k & (LW - 1)); }
+
 // Biot-Savart u = i k x w / max(|k|^2,1e-3): 3D_NavierStokes/src/kernels.h:187-201
 __device__ __forceinline__ void biot_savart(double kx, double ky, double kz, cpx<double> xi, cpx<double> et,
                                             cpx<double> zt, double wki, cpx<double> &u, cpx<double> &v,
@@ -170,9 +193,9 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
     }
     fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
     if (valid) {
-      cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz + (long)iky * g.xp + ikx;
+      cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz;
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.ikty * g.xp] = v[mm];
+      for (int mm = 0; mm < EPT; mm++) o[zoff(g, ikx, iky, j + TP * mm)] = v[mm];
     }
   }
 }
@@ -187,19 +210,19 @@ k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
   const TileId tid = xcd_tile();
   const int ikx = tid.x * T + col, z = tid.y, mf = tid.z;
   const bool valid = ikx < g.iktx;
-  const cpx<P> *src = in + (long)mf * g.zsz + (long)z * g.ikty * g.xp + ikx;
+  const cpx<P> *src = in + (long)mf * g.zsz;
   cpx<P> v[EPT];
 #pragma unroll
   for (int mm = 0; mm < EPT; mm++) {
     const int y = j + TP * mm;
     const int ky = wrapk(y, g.ny, g.kty);
-    v[mm] = (valid && ky != PSG_NOK) ? src[(long)(ky + g.kty) * g.xp] : cpx<P>{0, 0};
+    v[mm] = (valid && ky != PSG_NOK) ? src[zoff(g, ikx, ky + g.kty, z)] : cpx<P>{0, 0};
   }
   fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
   if (valid) {
-    cpx<P> *o = out + (long)mf * g.ysz + (long)z * g.ny * g.xp + ikx;
+    cpx<P> *o = out + (long)mf * g.ysz;
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.xp] = v[mm];
+    for (int mm = 0; mm < EPT; mm++) o[yoff(g, ikx, j + TP * mm, z)] = v[mm];
   }
 }
 
@@ -219,11 +242,11 @@ k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
   for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[(long)(j + TP * mm) * g.xp] : cpx<P>{0, 0};
   fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
   if (valid) {
-    cpx<P> *o = out + (long)mf * g.zsz + (long)z * g.ikty * g.xp + ikx;
+    cpx<P> *o = out + (long)mf * g.zsz;
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
       const int ky = wrapk(j + TP * mm, g.ny, g.kty);
-      if (ky != PSG_NOK) o[(long)(ky + g.kty) * g.xp] = v[mm];
+      if (ky != PSG_NOK) o[zfoff(g, ikx, ky + g.kty, z)] = v[mm];
     }
   }
 }
@@ -246,9 +269,9 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
   cpx<double> res[FO][EPT];
   cpx<P> vin[NV][EPT];
   auto load = [&](int f, cpx<P>(&v)[EPT]) {
-    const cpx<P> *src = in + ((long)m * FO + f) * g.zsz + (long)iky * g.xp + ikx;
+    const cpx<P> *src = in + ((long)m * FO + f) * g.zsz;
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[(long)(j + TP * mm) * g.ikty * g.xp] : cpx<P>{0, 0};
+    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[zfoff(g, ikx, iky, j + TP * mm)] : cpx<P>{0, 0};
   };
   if constexpr (FRONT) {
 #pragma unroll
@@ -295,10 +318,12 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
 // just before its transform, which keeps the 9- and 12-field products of the
 // tangent-linear and adjoint passes within the register file.
 template <typename P, int N, int E> struct XRow {
-  // pair p of a row: z = a + i b from the half spectra of fields fa, fb
+  // pair p of a row: z = a + i b from the half spectra of fields fa, fb, whose
+  // row starts ra, rb hold element k at ycol(g, k)
   __device__ __forceinline__ static void load(cpx<P> (&v)[E], const cpx<P> *ra, const cpx<P> *rb, bool hasb,
-                                              bool valid, int j, int ix) {
+                                              bool valid, int j, const Grid &g) {
     constexpr int TPX = N / E;
+    const int ix = g.iktx;
 #pragma unroll
     for (int mm = 0; mm < E; mm++) {
       const int k = j + TPX * mm;
@@ -306,13 +331,13 @@ template <typename P, int N, int E> struct XRow {
       cpx<P> z{0, 0};
       if (valid) {
         if (k < ix) {
-          a = ra[k];
-          if (hasb) b = rb[k];
+          a = ra[ycol(g, k)];
+          if (hasb) b = rb[ycol(g, k)];
           if (k == 0) { a.y = 0; b.y = 0; }
           z = cpx<P>{a.x - b.y, a.y + b.x};
         } else if (k > N - ix) {
-          a = ra[N - k];
-          if (hasb) b = rb[N - k];
+          a = ra[ycol(g, N - k)];
+          if (hasb) b = rb[ycol(g, N - k)];
           z = cpx<P>{a.x + b.y, b.x - a.y};
         }
       }
@@ -340,13 +365,13 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   cpx<P> *tl = lds + team * row_len<N>();
   constexpr int SW = row_swizzle<E>();
   const int ix = g.iktx;
-  auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + rr * g.xp; };
+  auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + yrow(g, rr); };
   cpx<P> vin[NV][E];
   if constexpr (FRONT) {
 #pragma unroll
     for (int p = 0; p < PI; p++)
       XRow<P, N, E>::load(vin[p], rowp(2 * p), rowp(2 * p + 1 < FI ? 2 * p + 1 : 2 * p), 2 * p + 1 < FI, valid, j,
-                          ix);
+                          g);
   }
   P phys[2 * PI][E];
 #pragma unroll
@@ -357,7 +382,7 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
 #pragma unroll
       for (int mm = 0; mm < E; mm++) v[mm] = vin[p][mm];
     } else {
-      XRow<P, N, E>::load(v, rowp(fa), rowp(fb < FI ? fb : fa), fb < FI, valid, j, ix);
+      XRow<P, N, E>::load(v, rowp(fa), rowp(fb < FI ? fb : fa), fb < FI, valid, j, g);
     }
     fft_pencil<P, N, E, true, WAVE, SW>(v, tl, j, 1, 0, tw);
 #pragma unroll
@@ -504,9 +529,9 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
       }
       fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
       if (valid) {
-        cpx<P> *o = out + (long)m * out_ms + (long)(3 * src + f) * g.ysz + ikx;
+        cpx<P> *o = out + (long)m * out_ms + (long)(3 * src + f) * g.ysz;
 #pragma unroll
-        for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.xp] = v[mm];
+        for (int mm = 0; mm < EPT; mm++) o[yoff(g, ikx, j + TP * mm, 0)] = v[mm];
       }
     }
   }

@@ -99,9 +99,9 @@ k_zinv_curl(const cpx<double> *__restrict__ st, long fs, cpx<P> *__restrict__ ou
     for (int mm = 0; mm < EPT; mm++) v[mm] = to<P>(w[f][mm]);
     fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
     if (valid) {
-      cpx<P> *o = out + (long)f * g.zsz + (long)iky * g.xp + ikx;
+      cpx<P> *o = out + (long)f * g.zsz;
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) o[(long)(j + TP * mm) * g.ikty * g.xp] = v[mm];
+      for (int mm = 0; mm < EPT; mm++) o[zoff(g, ikx, iky, j + TP * mm)] = v[mm];
     }
   }
 }
