@@ -113,7 +113,7 @@ int psg_device_count(void);
 /* ---------- kernel shapes and autotuning ----------
  * Shape variants of a pass (launch geometry, points per thread, register
  * staging); results are identical up to FP rounding of the transform order.
- * x-pass of the 3D Navier-Stokes and Euler right-hand side, variants 0..9
+ * x-pass of the 3D Navier-Stokes and Euler right-hand side, variants 0..11
  * (rows per block: as many as fit in 256 threads unless stated):
  *   0  8 points/thread, inputs front-loaded, product kept for all outputs
  *   1  8 points/thread, inputs front-loaded, product re-evaluated per output pair
@@ -126,6 +126,8 @@ int psg_device_count(void);
  *   8  16 points/thread (radix-16), inputs per pair, product per pair (FP32-bulk;
  *      in the FP64 form 8 and 9 run variant 1)
  *   9  as 8 with 4 rows per block
+ *  10  as 1 with registers capped for 4 waves/SIMD (FP64) or 5 (FP32-bulk)
+ *  11  as 1 with registers capped for 5 waves/SIMD (FP64) or 6 (FP32-bulk)
  * psg_autotune times `steps` steps of every variant on the context's own state
  * (restored afterwards), keeps the fastest and writes a text report. */
 #define PSG_PASS_XPASS 0

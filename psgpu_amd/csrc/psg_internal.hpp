@@ -25,7 +25,7 @@ struct Grid {
 };
 
 enum { PREC64 = 0, PREC32 = 1 };
-constexpr int PSG_XPASS_VARIANTS = 10;
+constexpr int PSG_XPASS_VARIANTS = 12;
 
 struct ProfEntry {
   std::string name;

@@ -140,6 +140,13 @@ template <typename P> static int xpass_ns(Ctx *c, const cpx<P> *B, cpx<P> *Cb, i
         return launch_xpass<P, 6, 3, ProdNS<P>, 16, false, true, 4>(c, B, Cb, nr, nm, ProdNS<P>{});
       }
       return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true>(c, B, Cb, nr, nm, ProdNS<P>{});
+    // register caps for occupancy: FP64 4 or 5 waves per SIMD, FP32 5 or 6
+    case 10:
+      return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 0, sizeof(P) == 8 ? 4 : 5>(c, B, Cb, nr, nm,
+                                                                                       ProdNS<P>{});
+    case 11:
+      return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 0, sizeof(P) == 8 ? 5 : 6>(c, B, Cb, nr, nm,
+                                                                                       ProdNS<P>{});
     default: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, false>(c, B, Cb, nr, nm, ProdNS<P>{});
   }
 }

@@ -67,7 +67,7 @@ static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, l
 
 // TEAMS_MAX caps the rows per block (0: as many as fit in 256 threads)
 template <typename P, int FI, int FO, class PROD, int E = EPT, bool FRONT = true, bool LAZY = false,
-          int TEAMS_MAX = 0>
+          int TEAMS_MAX = 0, int WPE = 0>
 static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nx, {
@@ -77,7 +77,7 @@ static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int 
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
     size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
-    auto k = k_xpass<P, N, TE, FI, FO, PROD, E, FRONT, LAZY>;
+    auto k = k_xpass<P, N, TE, FI, FO, PROD, E, FRONT, LAZY, WPE>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(TE * TPX), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0), prod);
   });

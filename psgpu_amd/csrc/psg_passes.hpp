@@ -67,6 +67,10 @@ __device__ __forceinline__ TileId xcd_tile() {
 }
 
 // dynamic LDS of a kernel (a CPU build for bounds checking supplies its own)
+// occupancy request of a kernel (a CPU build for bounds checking drops it)
+#ifndef PSG_WAVES_PER_EU
+#define PSG_WAVES_PER_EU(w) __attribute__((amdgpu_waves_per_eu(w)))
+#endif
 #ifndef PSG_DYN_SMEM
 #define PSG_DYN_SMEM(name) extern __shared__ __attribute__((aligned(16))) char name[]
 #endif
@@ -86,23 +90,20 @@ __device__ __forceinline__ int wrapk(int p, int n, int kt) {
 #define PSG_NOK (-0x40000000)
 
 // Pass-buffer layouts, per field.  kx columns are grouped in tiles of LW = 8
-// (xp/LW tiles), and each buffer is ordered so that the workgroups of the pass
-// that WRITES it fill whole tiles contiguously: scattered 128-byte row pieces
-// write at well under the HBM rate that contiguous tiles reach, while the
-// reading passes gather 128-byte pieces at close to full rate.
-//   Z  (z-inverse -> y-inverse):  [tile][iky][z][LW]
-//   Y  (y-inverse -> x-pass):     [z][tile][y][LW]
-//   Z' (y-forward -> z-forward):  [tile][z][iky][LW]
-//   C  (x-pass -> y-forward):     row-major [z][y][xp] (the x-pass writes rows)
+// (xp/LW tiles).  A column pass's workgroup owns one tile of one plane, so
+// the y passes read and write whole tiles contiguously (21.9 KB of ky and
+// 32 KB of y at 256^3 FP64) instead of 128-byte pieces of rows; the z passes
+// see 128-byte pieces either way, ordered so that the tiles of neighbouring
+// workgroups (consecutive kx tiles of a plane) are adjacent.
+//   Z, Z' (z-inverse -> y-inverse, y-forward -> z-forward):  [z][tile][iky][LW]
+//   Y     (y-inverse -> x-pass):                            [z][tile][y][LW]
+//   C     (x-pass -> y-forward): row-major [z][y][xp] (the x-pass writes rows)
 constexpr int LW = 8;
 __device__ __forceinline__ long zoff(const Grid &g, int ikx, int iky, int z) {
-  return ((long)(ikx >> 3) * g.ikty + iky) * ((long)g.nz * LW) + z * LW + (ikx & (LW - 1));
+  return ((long)z * (g.xp / LW) + (ikx >> 3)) * ((long)g.ikty * LW) + iky * LW + (ikx & (LW - 1));
 }
 __device__ __forceinline__ long yoff(const Grid &g, int ikx, int y, int z) {
   return ((long)z * (g.xp / LW) + (ikx >> 3)) * ((long)g.ny * LW) + y * LW + (ikx & (LW - 1));
-}
-__device__ __forceinline__ long zfoff(const Grid &g, int ikx, int iky, int z) {
-  return ((long)(ikx >> 3) * g.nz + z) * ((long)g.ikty * LW) + iky * LW + (ikx & (LW - 1));
 }
 // x row rr = z*ny + y of a Y buffer: element k sits at yrow(rr)[ycol(k)]
 __device__ __forceinline__ long yrow(const Grid &g, long rr) { return yoff(g, 0, (int)(rr % g.ny), (int)(rr / g.ny)); }
@@ -246,7 +247,7 @@ k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
       const int ky = wrapk(j + TP * mm, g.ny, g.kty);
-      if (ky != PSG_NOK) o[zfoff(g, ikx, ky + g.kty, z)] = v[mm];
+      if (ky != PSG_NOK) o[zoff(g, ikx, ky + g.kty, z)] = v[mm];
     }
   }
 }
@@ -271,7 +272,7 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
   auto load = [&](int f, cpx<P>(&v)[EPT]) {
     const cpx<P> *src = in + ((long)m * FO + f) * g.zsz;
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[zfoff(g, ikx, iky, j + TP * mm)] : cpx<P>{0, 0};
+    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[zoff(g, ikx, iky, j + TP * mm)] : cpx<P>{0, 0};
   };
   if constexpr (FRONT) {
 #pragma unroll
@@ -348,9 +349,11 @@ template <typename P, int N, int E> struct XRow {
 
 // LAZY: the product is re-evaluated per output pair instead of being kept
 // for all outputs, which frees 2*FO*E registers for occupancy.
+// WPE > 0 asks the compiler for at least WPE waves per SIMD (a register cap
+// of 512/WPE VGPRs), trading spills or rematerialisation for occupancy.
 template <typename P, int N, int TEAMS, int FI, int FO, class PROD, int E = EPT, bool FRONT = true,
-          bool LAZY = false>
-__global__ void __launch_bounds__(TEAMS *(N / E))
+          bool LAZY = false, int WPE = 0>
+__global__ void __launch_bounds__(TEAMS *(N / E)) PSG_WAVES_PER_EU(WPE > 0 ? WPE : 1)
 k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
         const cpx<P> *__restrict__ tw, PROD prod) {
   PSG_LDS_TWIDDLES(P, N, E);

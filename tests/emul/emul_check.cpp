@@ -200,7 +200,7 @@ int main(int argc, char **argv) {
   omp_set_max_active_levels(1);
   const bool quick = argc > 1 && strcmp(argv[1], "quick") == 0;
   // 3D Navier-Stokes: forcings, precisions, every x-pass variant, graph path, aspect ratio
-  for (int v = 0; v < 10; v++) {
+  for (int v = 0; v < 12; v++) {
     char nm[96];
     snprintf(nm, sizeof nm, "ns3d 16^3 fp64 xpass variant %d", v);
     ns3d_case(nm, PSG_NS3D, 16, 16, 16, 1.0, PSG_FP64, v, 0, 2, PSG_FORCE_KOLMO, 1e-11);

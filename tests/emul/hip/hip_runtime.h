@@ -29,6 +29,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define PSG_WAVES_PER_EU(...)
 #define __shared__ static thread_local
 #define PSG_DYN_SMEM(name) char *name = ::emu::dyn_smem
 

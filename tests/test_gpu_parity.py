@@ -210,7 +210,7 @@ def test_bous_aspect_batch(gpu):
             assert rel(s.get_state(m), orc.run(sts[m], 10)) < 1e-11
 
 
-@pytest.mark.parametrize("variant", range(10))
+@pytest.mark.parametrize("variant", range(12))
 def test_ns3d_xpass_variants_match_oracle(gpu, variant):
     n = 32
     st = O.ic_solenoidal(O.grid3(n, n, n), 24)
@@ -230,7 +230,7 @@ def test_autotune_keeps_state_and_picks_a_variant(gpu):
         s.set_state(st)
         rep = s.autotune(steps=2)
         assert "chosen xpass variant" in rep
-        assert 0 <= s.get_variant() < 10
+        assert 0 <= s.get_variant() < 12
         assert np.array_equal(s.get_state(), st)
         s.run(10)
         got = s.get_state()
