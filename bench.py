@@ -133,14 +133,15 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-6):
     return out
 
 
-def newton_case(P, n=512, t_spin=100.0, t_window=200.0, dt_sample=0.25, nguess=8, max_newton=60,
-                tmin=5.0, tmax=20.0):
+def newton_case(P, n=512, t_spin=100.0, t_window=300.0, dt_sample=0.25, nguess=4, max_newton=20,
+                tmin=5.0, tmax=30.0):
     """2D Kolmogorov flow n^2, Re=40, sin(4y) forcing, dt=0.005: spin up a seeded
     random-phase state, take the closest shift-minimised near-recurrences with
     T in [tmin, tmax] as guesses, and run Newton-GMRES-hookstep (Krylov 40,
-    residual 1e-10) on all guesses with finite-difference Jv, then on the same
-    guesses with tangent-linear Jv, one guess after the other on one context;
-    s/orbit = total time / converged."""
+    residual 1e-10, at most max_newton iterations) on all guesses with
+    finite-difference Jv, then on the same guesses with tangent-linear Jv, one
+    guess after the other on one context; s/orbit = total time / converged,
+    and "paired" lists the guesses both forms converged with their times."""
     from psgpu_amd import recurrence as R
     g = P.hgrid("ns2d", n, n)
     L = P.tables(g)[3]
@@ -178,6 +179,11 @@ def newton_case(P, n=512, t_spin=100.0, t_window=200.0, dt_sample=0.25, nguess=8
                          "s_per_orbit": total / len(conv) if conv else None,
                          "newton_iters": [r["newton_iters"] for r in rs], "status": [r["status_name"] for r in rs]}
         out["runs"] = runs
+        fd = {r["guess"]: r for r in runs if r["jv"] == "finite-difference" and r["status_name"] == "CONVERGED"}
+        tl = {r["guess"]: r for r in runs if r["jv"] == "tangent-linear" and r["status_name"] == "CONVERGED"}
+        out["paired"] = [{"guess": k, "T": fd[k]["T"], "fd_seconds": fd[k]["seconds"], "tl_seconds": tl[k]["seconds"],
+                          "fd_iters": fd[k]["newton_iters"], "tl_iters": tl[k]["newton_iters"]}
+                         for k in sorted(set(fd) & set(tl))]
     return out
 
 
