@@ -46,6 +46,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# rows of the traffic summary for the FP64 pass names of the step profile
+TRAFFIC_ROW = {"zinv": "k_zinv<double, 256", "yinv": "k_yinv<double, 256", "xpass": "k_xpass<double, 256",
+               "yfwd": "k_yfwd<double, 256"}
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of a kernel as measured by the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/ns3d_256_hbm_traffic.txt (calibrated
+    MB read + written); None when the summary or the kernel's row is absent."""
+    path = os.path.join(ROOT, "profiles", "ns3d_256_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(kernel_prefix):
+            vals = line[60:].split()
+            return (float(vals[0]) + float(vals[2])) * 1e6
+    return None
+
+
 def model_bytes_step(n, b):
     """5-pass pruned-pencil bytes per Heun step (2 RHS) for an n^3 grid."""
     kt = n // 3
@@ -273,7 +292,10 @@ def main():
     if dom:
         ach = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom["kernel"], "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": ach / PEAK_HBM_GBS, "traffic": None, "bytes_per_launch": dom["bytes"],
+                "frac": ach / PEAK_HBM_GBS,
+                "traffic": measured_traffic(TRAFFIC_ROW.get(dom["kernel"], "-")) if n == 256 else None,
+                "traffic_source": "profiles/ns3d_256_hbm_traffic.txt (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE)",
+                "bytes_per_launch": dom["bytes"],
                 "avg_launch_ms": dom["avg_ms"]}
     mb = model_bytes_step(n, 16)
     model = {"gb_per_step": mb / 1e9, "achieved_gbs": mb * sps64 / 1e9, "frac": mb * sps64 / 1e9 / PEAK_HBM_GBS}
