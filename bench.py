@@ -72,10 +72,11 @@ def model_bytes_step(n, b):
     nkt = iktx * ikty * ikty
     Z, Y = iktx * ikty * n, iktx * n * n
     S, Fi, Fo = 3, 6, 3
-    rhs = b * (S * nkt + 2 * Fi * Z + 2 * Fi * Y + 2 * Fo * Y + 2 * Fo * Z)
-    # Heun state/RH traffic (FP64): predictor reads w^n, writes w* and RH; corrector reads w*, RH,
-    # writes w^{n+1}
-    stage = 6 * S * nkt * 16
+    # pass 1 reads the FP64 state; the pass buffers are in the pass precision b
+    rhs = 16 * S * nkt + b * (2 * Fi * Z + 2 * Fi * Y + 2 * Fo * Y + 2 * Fo * Z)
+    # Heun state/RH traffic: predictor reads w^n, writes w* and RH; corrector reads w*, RH, writes
+    # w^{n+1}; the state is FP64, RH is kept in the pass precision
+    stage = S * nkt * (4 * 16 + 2 * b)
     return 2 * rhs + stage
 
 
@@ -101,7 +102,7 @@ def time_form(P, st0, n, precision, steps, warmup, tune=True, prof_steps=20):
         s.sync()
         prof = s.profile_report()
         s.profile(False)
-    variant = s.get_variant()
+    variant = {"xpass": s.get_variant(0), "zfwd": s.get_variant(1)}
     s.close()
     return dt, prof, e, variant
 
@@ -283,7 +284,7 @@ def main():
         t32, prof32, e32, var32 = time_form(P, st0, n, "fp32", a.steps, a.warmup, not a.no_tune)
         sps32 = a.steps / t32
         fp32 = {"steps_per_s": sps32, "ms_per_step": 1e3 * t32 / a.steps, "speedup_vs_fp64": sps32 / sps64,
-                "energy_rel_err_vs_fp64_gpu": abs(e32 - e64) / e64, "steps": a.steps, "xpass_variant": var32,
+                "energy_rel_err_vs_fp64_gpu": abs(e32 - e64) / e64, "steps": a.steps, "pass_variants": var32,
                 "kernels": prof32}
         log("bench: fp32-bulk %.2f steps/s (x%.2f)" % (sps32, sps32 / sps64))
     # dominant kernel of the FP64 step
@@ -326,7 +327,7 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "ns3d_heun_cn_%d3_kolmogorov" % n, "grid": [n, n, n], "nu": 0.01, "dt": 0.002,
                       "forcing": "sin(4y) x", "ic": "solenoidal E(k)~k^4exp(-2(k/4)^2), E=0.5, seed 20240601",
-                      "members": 1, "xpass_variant": var64},
+                      "members": 1, "pass_variants": var64},
            "roofline": roof, "model": model, "fp32_bulk": fp32, "kernels_fp64": prof64,
            "adjoint_128": adj, "bous_ensemble_512": bous, "newton_2d": newton, "cpu_baseline": cpu}
     print(json.dumps(out))

@@ -128,9 +128,15 @@ int psg_device_count(void);
  *   9  as 8 with 4 rows per block
  *  10  as 1 with registers capped for 4 waves/SIMD (FP64) or 5 (FP32-bulk)
  *  11  as 1 with registers capped for 5 waves/SIMD (FP64) or 6 (FP32-bulk)
- * psg_autotune times `steps` steps of every variant on the context's own state
- * (restored afterwards), keeps the fastest and writes a text report. */
+ * z-forward + update pass of the 3D Navier-Stokes and Euler step (PSG_PASS_ZFWD),
+ * variants 0..1:
+ *   0  the three fields' inputs front-loaded
+ *   1  each field loaded just before its transform (fewer registers)
+ * psg_autotune times `steps` steps of every variant of each pass in turn on the
+ * context's own state (restored afterwards), keeps the fastest of each and
+ * writes a text report. */
 #define PSG_PASS_XPASS 0
+#define PSG_PASS_ZFWD 1
 /* Step graphs: psg_run launches its first step directly, then replays a
  * captured HIP graph of `steps` steps (default 16) as often as fits, then
  * launches the remainder directly; results are bitwise those of direct

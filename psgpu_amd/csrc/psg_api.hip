@@ -589,11 +589,12 @@ void *psg_stream(psg_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 int psg_set_variant(psg_ctx *c, int pass, int variant) {
   if (!c) return PSG_EINVAL;
-  if (pass != PSG_PASS_XPASS || variant < 0 || variant >= PSG_XPASS_VARIANTS) {
+  const int nv = pass == PSG_PASS_XPASS ? PSG_XPASS_VARIANTS : (pass == PSG_PASS_ZFWD ? PSG_ZFWD_VARIANTS : 0);
+  if (variant < 0 || variant >= nv) {
     set_error("psg_set_variant: unknown pass %d or variant %d", pass, variant);
     return PSG_EINVAL;
   }
-  c->xvar = variant;
+  (pass == PSG_PASS_XPASS ? c->xvar : c->zvar) = variant;
   c->graph_gen++;
   return 0;
 }
@@ -607,8 +608,8 @@ int psg_set_graph(psg_ctx *c, int steps) {
 }
 
 int psg_get_variant(psg_ctx *c, int pass) {
-  if (!c || pass != PSG_PASS_XPASS) return PSG_EINVAL;
-  return c->xvar;
+  if (!c || (pass != PSG_PASS_XPASS && pass != PSG_PASS_ZFWD)) return PSG_EINVAL;
+  return pass == PSG_PASS_XPASS ? c->xvar : c->zvar;
 }
 
 int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
@@ -628,24 +629,34 @@ int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
     hipEvent_t e0, e1;
     PSG_HIP(hipEventCreate(&e0));
     PSG_HIP(hipEventCreate(&e1));
-    double best = 1e300;
-    int bestv = c->xvar, rc = 0;
-    for (int v = 0; v < PSG_XPASS_VARIANTS && rc == 0; v++) {
-      c->xvar = v;
-      c->graph_gen++;
-      if ((rc = run_steps(c, 0, 1))) break;
-      hipEventRecord(e0, c->stream);
-      if ((rc = run_steps(c, 0, steps))) break;
-      hipEventRecord(e1, c->stream);
-      hipEventSynchronize(e1);
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, e0, e1);
-      const double per = ms / steps;
-      char line[96];
-      snprintf(line, sizeof line, "xpass variant %d: %.4f ms/step\n", v, per);
+    int rc = 0;
+    // passes are tuned one after the other, each with the others' choices fixed
+    struct { const char *name; int *var; int n; } passes[] = {{"xpass", &c->xvar, PSG_XPASS_VARIANTS},
+                                                                {"zfwd", &c->zvar, PSG_ZFWD_VARIANTS}};
+    for (auto &ps : passes) {
+      double best = 1e300;
+      int bestv = *ps.var;
+      for (int v = 0; v < ps.n && rc == 0; v++) {
+        *ps.var = v;
+        c->graph_gen++;
+        if ((rc = run_steps(c, 0, 1))) break;
+        hipEventRecord(e0, c->stream);
+        if ((rc = run_steps(c, 0, steps))) break;
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        const double per = ms / steps;
+        char line[96];
+        snprintf(line, sizeof line, "%s variant %d: %.4f ms/step\n", ps.name, v, per);
+        rep += line;
+        if (per < best) { best = per; bestv = v; }
+        hipMemcpyAsync(c->state, save, sb, hipMemcpyDeviceToDevice, c->stream);
+      }
+      *ps.var = bestv;
+      char line[64];
+      snprintf(line, sizeof line, "chosen %s variant %d\n", ps.name, bestv);
       rep += line;
-      if (per < best) { best = per; bestv = v; }
-      hipMemcpyAsync(c->state, save, sb, hipMemcpyDeviceToDevice, c->stream);
     }
     hipStreamSynchronize(c->stream);
     hipEventDestroy(e0);
@@ -653,12 +664,8 @@ int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
     hipFree(save);
     c->time = t0;
     c->prof = prof;
-    c->xvar = bestv;
     c->graph_gen++;
     if (rc) return rc;
-    char line[64];
-    snprintf(line, sizeof line, "chosen xpass variant %d\n", bestv);
-    rep += line;
   }
   if (report && report_len > 0) {
     strncpy(report, rep.c_str(), (size_t)report_len - 1);

@@ -231,6 +231,10 @@ template <typename P, int FI, int FO, class PROD>
 static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nx, {
+    if (g.iktx != xband<N>()) {
+      set_error("x-pass: IKTX %d is not NX/3+1", g.iktx);
+      return PSG_EINVAL;
+    }
     constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
     dim3 grid((unsigned)((nrows + TE - 1) / TE));

@@ -26,6 +26,7 @@ struct Grid {
 
 enum { PREC64 = 0, PREC32 = 1 };
 constexpr int PSG_XPASS_VARIANTS = 12;
+constexpr int PSG_ZFWD_VARIANTS = 2;
 
 struct ProfEntry {
   std::string name;
@@ -70,6 +71,7 @@ struct Ctx {
   bool graph_failed = false;       // a capture failed once: eager from then on
   // kernel shape variants (psg_autotune / psg_set_variant)
   int xvar = 0;    // Navier-Stokes x-pass: 0..PSG_XPASS_VARIANTS-1
+  int zvar = 0;    // Navier-Stokes z-forward/update: 0..PSG_ZFWD_VARIANTS-1
   // diagnostics scratch
   double *dred = nullptr;
   size_t dred_n = 0;

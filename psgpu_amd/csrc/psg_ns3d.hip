@@ -15,9 +15,12 @@
 
 namespace psg {
 
-template <int PHASE, bool EULER> struct UpdHeun {
+// R: storage precision of the predictor's right-hand side RH, which the
+// corrector subtracts (the pass precision: in FP32-bulk it only carries
+// FP32-accurate products).
+template <int PHASE, bool EULER, typename R = double> struct UpdHeun {
   cpx<double> *st;
-  cpx<double> *rh;
+  cpx<R> *rh;
   long ms, fs;  // member stride, field stride (same for st and rh)
   const double *nuzn, *nu1;
   const int8_t *L;
@@ -27,13 +30,13 @@ template <int PHASE, bool EULER> struct UpdHeun {
                                              cpx<double> (&N)[3]) const {
     PSG_CURL
     cpx<double> *s = st + (long)m * ms + ik;
-    cpx<double> *r = rh + (long)m * ms + ik;
+    cpx<R> *r = rh + (long)m * ms + ik;
     const double ll = (double)L[ik];
     if constexpr (PHASE == 0) {
       if constexpr (EULER) {
-        r[0] = cpx<double>{ll * xiX, ll * xiY};
-        r[fs] = cpx<double>{ll * etX, ll * etY};
-        r[2 * fs] = cpx<double>{ll * ztX, ll * ztY};
+        r[0] = to<R>(cpx<double>{ll * xiX, ll * xiY});
+        r[fs] = to<R>(cpx<double>{ll * etX, ll * etY});
+        r[2 * fs] = to<R>(cpx<double>{ll * ztX, ll * ztY});
         cpx<double> a = s[0], b = s[fs], c = s[2 * fs];
         s[0] = cpx<double>{a.x + dt * xiX, a.y + dt * xiY};
         s[fs] = cpx<double>{b.x + dt * etX, b.y + dt * etY};
@@ -42,9 +45,9 @@ template <int PHASE, bool EULER> struct UpdHeun {
         const double NUZN = nuzn[ik];
         const double NU1 = ll * nu1[ik];
         add_forcing(F, kx, ky, kz, xiX, etY, ztX, ztY);
-        r[0] = cpx<double>{xiX, xiY};
-        r[fs] = cpx<double>{etX, etY};
-        r[2 * fs] = cpx<double>{ztX, ztY};
+        r[0] = to<R>(cpx<double>{xiX, xiY});
+        r[fs] = to<R>(cpx<double>{etX, etY});
+        r[2 * fs] = to<R>(cpx<double>{ztX, ztY});
         cpx<double> a = s[0], b = s[fs], c = s[2 * fs];
         s[0] = cpx<double>{NU1 * (NUZN * a.x + xiX), NU1 * (NUZN * a.y + xiY)};
         s[fs] = cpx<double>{NU1 * (NUZN * b.x + etX), NU1 * (NUZN * b.y + etY)};
@@ -58,7 +61,7 @@ template <int PHASE, bool EULER> struct UpdHeun {
         add_forcing(F, kx, ky, kz, xiX, etY, ztX, ztY);
       }
       cpx<double> a = s[0], b = s[fs], c = s[2 * fs];
-      cpx<double> ra = r[0], rb = r[fs], rc = r[2 * fs];
+      const cpx<double> ra = up(r[0]), rb = up(r[fs]), rc = up(r[2 * fs]);
       s[0] = cpx<double>{ll * (a.x + NU2 * (xiX - ra.x)), ll * (a.y + NU2 * (xiY - ra.y))};
       s[fs] = cpx<double>{ll * (b.x + NU2 * (etX - rb.x)), ll * (b.y + NU2 * (etY - rb.y))};
       s[2 * fs] = cpx<double>{ll * (c.x + NU2 * (ztX - rc.x)), ll * (c.y + NU2 * (ztY - rc.y))};
@@ -171,7 +174,8 @@ static int rhs3(Ctx *c, const cpx<double> *st, long st_ms, int umask, UPD upd, l
   PSG_TRY(launch_yfwd_g<P>(c, Cb, A, 3 * nm));
   prof_end(c);
   prof_begin(c, "zfwd_update", (pb[4] + state_bytes) * nm);
-  PSG_TRY((launch_zfwd<P, 3>(c, A, upd, nm)));
+  if (c->zvar == 1) PSG_TRY((launch_zfwd<P, 3, UPD, false>(c, A, upd, nm)));
+  else PSG_TRY((launch_zfwd<P, 3, UPD, true>(c, A, upd, nm)));
   prof_end(c);
   return 0;
 }
@@ -181,19 +185,21 @@ template <typename P> static int run_t(Ctx *c, int nsteps, bool first) {
   const bool euler = c->cfg.flavour == PSG_EULER3D;
   Force F{c->cfg.force, c->cfg.kf, c->cfg.ampfor};
   const long ms = 3 * g.nkt, fs = g.nkt;
-  const long sb_pre = 9 * g.nkt * 16, sb_cor = 9 * g.nkt * 16;  // state + RH traffic
+  // state (FP64) + RH (pass precision) traffic per mode of each half-step
+  const long sb = (6 * 16 + 3 * (long)sizeof(cpx<P>)) * g.nkt;
+  cpx<P> *rh = (cpx<P> *)c->rh;
   for (int nt = 0; nt < nsteps; nt++) {
     const int umask = (first && nt == 0) ? 1 : 0;
     if (euler) {
-      UpdHeun<0, true> up{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb_pre, c->batch));
-      UpdHeun<1, true> uc{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb_cor, c->batch));
+      UpdHeun<0, true, P> up{c->state, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb, c->batch));
+      UpdHeun<1, true, P> uc{c->state, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb, c->batch));
     } else {
-      UpdHeun<0, false> up{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb_pre, c->batch));
-      UpdHeun<1, false> uc{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb_cor, c->batch));
+      UpdHeun<0, false, P> up{c->state, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb, c->batch));
+      UpdHeun<1, false, P> uc{c->state, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb, c->batch));
     }
   }
   return 0;
@@ -212,11 +218,12 @@ template <typename P> static int step_rec_t(Ctx *c, int umask, cpx<double> *wn, 
   const long ms = 3 * g.nkt, fs = g.nkt;
   const size_t bytes = sizeof(cpx<double>) * 3 * g.nkt;
   if (wn) PSG_HIP(hipMemcpyAsync(wn, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
-  UpdHeun<0, false> up{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-  PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, 9 * g.nkt * 16, 1));
+  const long sb = (6 * 16 + 3 * (long)sizeof(cpx<P>)) * g.nkt;
+  UpdHeun<0, false, P> up{c->state, (cpx<P> *)c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+  PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb, 1));
   if (wstar) PSG_HIP(hipMemcpyAsync(wstar, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
-  UpdHeun<1, false> uc{c->state, c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-  PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, 9 * g.nkt * 16, 1));
+  UpdHeun<1, false, P> uc{c->state, (cpx<P> *)c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+  PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb, 1));
   return 0;
 }
 

@@ -71,6 +71,10 @@ template <typename P, int FI, int FO, class PROD, int E = EPT, bool FRONT = true
 static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nx, {
+    if (g.iktx != xband<N>()) {
+      set_error("x-pass: IKTX %d is not NX/3+1", g.iktx);
+      return PSG_EINVAL;
+    }
     constexpr int TPX = N / E;
     constexpr int TE0 = (256 / TPX) < 1 ? 1 : 256 / TPX;
     constexpr int TE = (TEAMS_MAX > 0 && TEAMS_MAX < TE0) ? TEAMS_MAX : TE0;
@@ -85,13 +89,14 @@ static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int 
   return 0;
 }
 
-template <typename P, int FO, class UPD> static int launch_zfwd(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
+template <typename P, int FO, class UPD, bool FRONT = (FO <= 4)>
+static int launch_zfwd(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
-    auto k = k_zfwd<P, N, T, FO, UPD>;
+    auto k = k_zfwd<P, N, T, FO, UPD, FRONT>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);
   });

@@ -162,17 +162,19 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
   const int ky = iky - g.kty;
   const double kx = (double)ikx * g.alpha, kyd = (double)ky;
   const cpx<double> *s0 = st + (long)m * st_ms;
-  // every state value of the pencil is requested before the first transform
-  cpx<double> w[3][EPT];
-  double lm[EPT];
+  // every state value of the pencil is requested before the first transform;
+  // it is kept in the pass precision (in FP32-bulk the Biot-Savart product then
+  // starts from the rounded vorticity, and the registers halve)
+  cpx<P> w[3][EPT];
+  P lm[EPT];
 #pragma unroll
   for (int mm = 0; mm < EPT; mm++) {
     const int kz = valid ? wrapk(j + TP * mm, g.nz, g.ktz) : PSG_NOK;
     const FillIdx q = fill_index(g, ikx, iky, ky, kz);
-    w[0][mm] = ld_fill(s0, q);
-    w[1][mm] = ld_fill(s0 + st_fs, q);
-    w[2][mm] = ld_fill(s0 + 2 * st_fs, q);
-    lm[mm] = (umask && q.idx >= 0) ? (double)mask[q.idx] : 1.0;
+    w[0][mm] = to<P>(ld_fill(s0, q));
+    w[1][mm] = to<P>(ld_fill(s0 + st_fs, q));
+    w[2][mm] = to<P>(ld_fill(s0 + 2 * st_fs, q));
+    lm[mm] = (umask && q.idx >= 0) ? (P)mask[q.idx] : (P)1;
   }
   constexpr int NFO = BS ? 6 : 3;
 #pragma unroll
@@ -181,14 +183,14 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
       if (f < 3) {
-        v[mm] = to<P>(w[f][mm]);
+        v[mm] = w[f][mm];
       } else {
         const int kz = wrapk(j + TP * mm, g.nz, g.ktz);
         const double kzd = kz == PSG_NOK ? 0.0 : (double)kz;
         const double wk = kx * kx + kyd * kyd + kzd * kzd;
-        const double wki = lm[mm] / (wk > 0.001 ? wk : 0.001);
+        const double wki = (double)lm[mm] / (wk > 0.001 ? wk : 0.001);
         cpx<double> u, vv, ww;
-        biot_savart(kx, kyd, kzd, w[0][mm], w[1][mm], w[2][mm], wki, u, vv, ww);
+        biot_savart(kx, kyd, kzd, up(w[0][mm]), up(w[1][mm]), up(w[2][mm]), wki, u, vv, ww);
         v[mm] = to<P>(f == 3 ? u : (f == 4 ? vv : ww));
       }
     }
@@ -254,14 +256,15 @@ k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
 
 // ---------------------------------------------------------------- pass 5 (3D)
 // UPD::operator()(m, ik, ikx, kx, ky, kz, N[FO]) with N already scaled by 1/NR.
-template <typename P, int N, int T, int FO, class UPD>
+// FRONT: every input is requested before the first transform (the default up
+// to 4 fields); otherwise each field is loaded just before its transform,
+// which frees 4*(FO-1)*EPT registers (the 6-field tangent-linear/adjoint
+// updates always do this).
+template <typename P, int N, int T, int FO, class UPD, bool FRONT = (FO <= 4)>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
   PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
-  // up to 4 fields every input is requested before the first transform; the
-  // 6-field tangent-linear/adjoint updates load each field before its transform
-  constexpr bool FRONT = FO <= 4;
   constexpr int NV = FRONT ? FO : 1;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
@@ -318,13 +321,17 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
 // (latency hiding for the 6-field products); without it each pair is loaded
 // just before its transform, which keeps the 9- and 12-field products of the
 // tangent-linear and adjoint passes within the register file.
+// kept kx modes of an N-point x transform under the 2/3 rule: 0..N/3
+// (IKTX = NX/3 + 1 for every grid the context builds)
+template <int N> constexpr int xband() { return N / 3 + 1; }
+
 template <typename P, int N, int E> struct XRow {
   // pair p of a row: z = a + i b from the half spectra of fields fa, fb, whose
   // row starts ra, rb hold element k at ycol(g, k)
   __device__ __forceinline__ static void load(cpx<P> (&v)[E], const cpx<P> *ra, const cpx<P> *rb, bool hasb,
                                               bool valid, int j, const Grid &g) {
     constexpr int TPX = N / E;
-    const int ix = g.iktx;
+    constexpr int ix = xband<N>();
 #pragma unroll
     for (int mm = 0; mm < E; mm++) {
       const int k = j + TPX * mm;
@@ -367,7 +374,7 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   const long m = valid ? row / nrow_m : 0, rr = valid ? row % nrow_m : 0;
   cpx<P> *tl = lds + team * row_len<N>();
   constexpr int SW = row_swizzle<E>();
-  const int ix = g.iktx;
+  constexpr int ix = xband<N>();
   auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + yrow(g, rr); };
   cpx<P> vin[NV][E];
   if constexpr (FRONT) {
@@ -427,7 +434,10 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
     fft_pencil<P, N, E, false, WAVE, SW>(v, tl, j, 1, 0, tw);
     team_sync<WAVE>();
 #pragma unroll
-    for (int mm = 0; mm < E; mm++) tl[lpos<SW>(j + TPX * mm)] = v[mm];
+    for (int mm = 0; mm < E; mm++) {
+      const int k = j + TPX * mm;   // only the kept modes and their mirrors are read back
+      if (k < ix || k > N - ix) tl[lpos<SW>(k)] = v[mm];
+    }
     team_sync<WAVE>();
     if (valid) {
       cpx<P> *oa = out + ((long)m * FO + 2 * q) * g.ysz + rr * g.xp;

@@ -48,11 +48,12 @@ static V ic3(int nx, int ny, int nz, double alpha, uint64_t seed, int nf) {
 }
 
 static void ns3d_case(const char *name, int flav, int nx, int ny, int nz, double alpha, int prec, int variant,
-                      int graph, int steps, int force, double tol) {
+                      int graph, int steps, int force, double tol, int zvariant = 0) {
   psg_config c = cfg3(flav, nx, ny, nz, alpha, prec, 1, flav == PSG_EULER3D ? 0.0 : 0.02, 0.004, force);
   psg_ctx *ctx;
   if (psg_create(&c, &ctx)) { check(false, name, 1); return; }
   psg_set_variant(ctx, PSG_PASS_XPASS, variant);
+  psg_set_variant(ctx, PSG_PASS_ZFWD, zvariant);
   psg_set_graph(ctx, graph);
   V st = ic3(nx, ny, nz, alpha, 11, 3), got(st.size());
   psg_set_state(ctx, 0, st.data());
@@ -199,6 +200,21 @@ int main(int argc, char **argv) {
   omp_set_dynamic(0);
   omp_set_max_active_levels(1);
   const bool quick = argc > 1 && strcmp(argv[1], "quick") == 0;
+  if (argc > 1 && strcmp(argv[1], "fp32") == 0) {
+    // FP32-bulk forms only, at 16^3
+    const int vs[] = {0, 1, 3, 8, 10};
+    for (int v : vs) {
+      char nm[96];
+      snprintf(nm, sizeof nm, "ns3d 16^3 fp32 xpass variant %d", v);
+      ns3d_case(nm, PSG_NS3D, 16, 16, 16, 1.0, PSG_FP32BULK, v, 0, 2, PSG_FORCE_KOLMO, 1e-4);
+    }
+    ns3d_case("ns3d 16^3 fp32 zfwd variant 1 graph", PSG_NS3D, 16, 16, 16, 1.0, PSG_FP32BULK, 0, 2, 5,
+              PSG_FORCE_KOLMO, 1e-4, 1);
+    ns3d_case("euler 16^3 fp32", PSG_EULER3D, 16, 16, 16, 1.0, PSG_FP32BULK, 0, 0, 3, PSG_FORCE_NONE, 1e-4);
+    tladj_case(16, PSG_FP32BULK);
+    printf("%d passed, %d failed\n", g_pass, g_fail);
+    return g_fail ? 1 : 0;
+  }
   // 3D Navier-Stokes: forcings, precisions, every x-pass variant, graph path, aspect ratio
   for (int v = 0; v < 12; v++) {
     char nm[96];
@@ -208,6 +224,7 @@ int main(int argc, char **argv) {
     snprintf(nm, sizeof nm, "ns3d 32^3 fp32 xpass variant %d", v);
     ns3d_case(nm, PSG_NS3D, 32, 32, 32, 1.0, PSG_FP32BULK, v, 0, 2, PSG_FORCE_KOLMO, 1e-4);
   }
+  ns3d_case("ns3d 16^3 fp64 zfwd variant 1", PSG_NS3D, 16, 16, 16, 1.0, PSG_FP64, 0, 0, 2, PSG_FORCE_KOLMO, 1e-11, 1);
   ns3d_case("ns3d 32^3 fp64 graph 2 (dns forcing)", PSG_NS3D, 32, 32, 32, 1.0, PSG_FP64, 1, 2, 6, PSG_FORCE_DNS, 1e-11);
   ns3d_case("ns3d 32x16x16 a=0.5 fp64 graph (gmres forcing)", PSG_NS3D, 32, 16, 16, 0.5, PSG_FP64, 0, 2, 5,
             PSG_FORCE_GMRES, 1e-11);

@@ -223,6 +223,19 @@ def test_ns3d_xpass_variants_match_oracle(gpu, variant):
         assert rel(s.get_state(), want) < 1e-11
 
 
+@pytest.mark.parametrize("variant", range(2))
+def test_ns3d_zfwd_variants_match_oracle(gpu, variant):
+    n = 32
+    st = O.ic_solenoidal(O.grid3(n, n, n), 27)
+    want = O.NS3D(n, v2=0.01, dt=0.002, force=3).run(st, 20)
+    with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, precision="fp64") as s:
+        s.set_variant(variant, 1)
+        assert s.get_variant(1) == variant
+        s.set_state(st)
+        s.run(20)
+        assert rel(s.get_state(), want) < 1e-11
+
+
 def test_autotune_keeps_state_and_picks_a_variant(gpu):
     n = 64
     st = O.ic_solenoidal(O.grid3(n, n, n), 25)
