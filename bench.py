@@ -9,10 +9,12 @@ E = 0.5 (synthetic data).  The state is resident in HBM before timing starts.
 `value` is the plain FP64 form's steps/s (captured step graphs, kernel shapes
 autotuned first).  The FP32-bulk form (FFTs and products in FP32,
 state/update/diagnostics FP64) is timed on the same input and reported under
-"fp32_bulk" with its speed-up and its energy error against the FP64 GPU run.
+"fp32_bulk" with its speed-up and its energy error against the FP64 GPU run,
+and, under "energy_1000", the same error every 100 steps of a 1000-step run.
 "roofline" is the dominant kernel's achieved HBM bandwidth: algorithmic bytes
 per launch / mean launch time from HIP events on the context stream over a
-separate profiled run of 20 steps.  "model" is the whole-step fraction of the
+separate profiled run of 20 steps; its "traffic" is the kernel's measured HBM
+bytes from the committed rocprofv3 summary.  "model" is the whole-step fraction of the
 8 TB/s roofline under the 5-pass pruned-pencil byte model.  "cpu_baseline" is
 the FP64 C oracle (test infrastructure) timed on this host's cores on a
 bounded sample of the same workload.
@@ -105,6 +107,23 @@ def time_form(P, st0, n, precision, steps, warmup, tune=True, prof_steps=20):
     variant = {"xpass": s.get_variant(0), "zfwd": s.get_variant(1)}
     s.close()
     return dt, prof, e, variant
+
+
+def energy_drift(P, st0, n, nsteps=1000, every=100):
+    """FP32-bulk against plain FP64 over nsteps Heun steps from the same state:
+    relative energy difference every `every` steps (the target is <= 1e-6 at 1000)."""
+    series = {}
+    for precision in ("fp64", "fp32"):
+        with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov", precision=precision) as s:
+            s.set_state(st0)
+            es = []
+            for _ in range(nsteps // every):
+                s.run(every)
+                es.append(s.diag()[0])
+            series[precision] = es
+    rel = [abs(a - b) / abs(b) for a, b in zip(series["fp32"], series["fp64"])]
+    return {"steps": nsteps, "every": every, "energy_fp64": series["fp64"], "rel_err": rel, "rel_err_final": rel[-1],
+            "rel_err_max": max(rel)}
 
 
 def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-6):
@@ -287,6 +306,9 @@ def main():
                 "energy_rel_err_vs_fp64_gpu": abs(e32 - e64) / e64, "steps": a.steps, "pass_variants": var32,
                 "kernels": prof32}
         log("bench: fp32-bulk %.2f steps/s (x%.2f)" % (sps32, sps32 / sps64))
+        fp32["energy_1000"] = energy_drift(P, st0, n)
+        log("bench: fp32-bulk energy vs fp64 after 1000 steps: %.2e (max over the run %.2e)" %
+            (fp32["energy_1000"]["rel_err_final"], fp32["energy_1000"]["rel_err_max"]))
     # dominant kernel of the FP64 step
     dom = max(prof64, key=lambda r: r["avg_ms"] * r["count"]) if prof64 else None
     roof = None
