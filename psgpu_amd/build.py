@@ -6,6 +6,8 @@ Outputs (git-ignored, shipped with the tree):
     psgpu_amd/_lib/libpsg.so           namespaced C-ABI (include/psg.h, psg_host.h)
     psgpu_amd/_lib/libpsg_legacy_*.so  one per reference flavour, exporting the
                                        reference symbols (include/psg_legacy.h)
+    psgpu_amd/_lib/psg_gmres           the Newton-GMRES driver (C restatement of the
+                                       reference's src_gmres/main.F90 callers)
 HIP sources are compiled for gfx950 only; the C host layer with gcc.
 """
 import argparse
@@ -35,6 +37,10 @@ LEGACY = [("ns3d_dns", "legacy/legacy_ns3d_dns.c"), ("ns3d_gmres", "legacy/legac
           ("euler", "legacy/legacy_euler.c"), ("ns2d_gmres", "legacy/legacy_ns2d_gmres.c"),
           ("ns2d_dns", "legacy/legacy_ns2d_dns.c"), ("bous_dns", "legacy/legacy_bous_dns.c"),
           ("bous_gmres", "legacy/legacy_bous_gmres.c")]
+
+
+# C drivers restating the reference's Fortran callers: (executable, source)
+DRIVERS = [("psg_gmres", "drivers/psg_gmres.c")]
 
 
 def _run(cmd, verbose):
@@ -74,6 +80,11 @@ def build(verbose=False, jobs=None):
     tmp = lib + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + core + ["-lm"], verbose)
     os.replace(tmp, lib)
+    for name, src in DRIVERS:
+        out = os.path.join(OUT, name)
+        _run(["gcc"] + C_FLAGS + [os.path.join(CSRC, src), "-o", out + ".tmp", "-L" + OUT, "-lpsg",
+              "-Wl,-rpath,$ORIGIN", "-lm"], verbose)
+        os.replace(out + ".tmp", out)
     for suffix, src in LEGACY:
         out = os.path.join(OUT, "libpsg_legacy_%s.so" % suffix)
         _run(["gcc", "-shared", "-fPIC", "-o", out + ".tmp", objs[src], "-L" + OUT, "-lpsg",

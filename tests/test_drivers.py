@@ -1,0 +1,120 @@
+"""The C Newton-GMRES driver (psgpu_amd/_lib/psg_gmres), a restatement of the
+reference's src_gmres/main.F90 callers: parameter files, guess records in the
+Zk.out layout, and the UPOinfo.dat / UPOs.out / Zk.out outputs
+(2D_NavierStokes/src_RK4/src_gmres/main.F90:146-153, 224-225, 645-646,
+1744-1835).  Argument and parameter errors are reported before any GPU work;
+the GPU test converges the laminar Kolmogorov state from a perturbed guess."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import psgpu_amd as P
+
+EXE = os.path.join(os.path.dirname(P.__file__), "_lib", "psg_gmres")
+
+# order-fixed labels of 2D_NavierStokes/src_RK4/src_gmres/parameters.F90:146-203
+MAIN_2D = ["ISEED", "IREST", "infile", "infile type", "Re", "TSTEP", "outfreq1", "outfreq2", "DELT", "AMPFOR",
+           "ResidualThreshold", "STFLAG", "RCFLAG", "ICFLAG", "WNFLAG", "initial x shift", "initial y shift",
+           "lamFLAG", "UPOflag"]
+EXTRA_2D = ["IGUESSEND", "maxPERIOD", "minPERIOD", "gmres ResidualThreshold", "arclcFLAG", "1stOrder",
+            "2ndOrder PredOnly", "Full 2ndOrder", "drN_reset", "Fixed Period", "Fixed Period Size", "delta_v2",
+            "ArnoldiFLAG"]
+
+
+def write_params(path, labels, values):
+    with open(path, "w") as f:
+        for lab in labels:
+            f.write("%s = %s\n" % (lab, values[lab]))
+
+
+def main_2d(**kw):
+    v = {"ISEED": 1, "IREST": 1, "infile": "guess.out", "infile type": "D", "Re": "5.0", "TSTEP": "1.0",
+         "outfreq1": "1.0", "outfreq2": "1.0", "DELT": "0.01", "AMPFOR": "-2.0", "ResidualThreshold": "1.0",
+         "STFLAG": 0, "RCFLAG": 0, "ICFLAG": 0, "WNFLAG": 0, "initial x shift": "0.0", "initial y shift": "0.0",
+         "lamFLAG": 0, "UPOflag": 1}
+    v.update(kw)
+    return v
+
+
+def extra_2d(**kw):
+    v = {"IGUESSEND": 1, "maxPERIOD": "100.0", "minPERIOD": "0.1", "gmres ResidualThreshold": "1.0",
+         "arclcFLAG": "F", "1stOrder": "F", "2ndOrder PredOnly": "F", "Full 2ndOrder": "F", "drN_reset": "F",
+         "Fixed Period": "T", "Fixed Period Size": "0.2", "delta_v2": "0.0", "ArnoldiFLAG": "F"}
+    v.update(kw)
+    return v
+
+
+def rec(payload):
+    return struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload))
+
+
+def read_recs(path):
+    out, b = [], open(path, "rb").read()
+    i = 0
+    while i < len(b):
+        n = struct.unpack("<i", b[i:i + 4])[0]
+        out.append(b[i + 4:i + 4 + n])
+        i += n + 8
+    return out
+
+
+def setup_case(d, z, T=0.2, main=None, extra=None):
+    g = P.hgrid("ns2d", 32, 32)
+    write_params(os.path.join(d, "params_gmres.txt"), MAIN_2D, main or main_2d())
+    write_params(os.path.join(d, "paramsEXTRA.txt"), EXTRA_2D, extra or extra_2d())
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<3i", 1, g.iktx, g.ikty)))
+        f.write(rec(struct.pack("<6d", 0.0, 0.2, T, 0.0, 0.0, 0.3) + np.ascontiguousarray(z, np.complex128).tobytes()))
+    return g
+
+
+def run(args, d):
+    return subprocess.run([EXE] + args + ["--dir", d], capture_output=True, text=True, timeout=300)
+
+
+def test_driver_built_and_usage():
+    assert os.path.exists(EXE)
+    r = subprocess.run([EXE], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([EXE, "bous", "32"], capture_output=True, text=True)
+    assert r.returncode == 1 and "flavour" in r.stderr
+
+
+def test_driver_rejects_unsupported_inputs(tmp_path):
+    d = str(tmp_path)
+    g = P.hgrid("ns2d", 32, 32)
+    setup_case(d, np.zeros(g.nkt, complex), main=main_2d(**{"infile type": "A"}))
+    r = run(["ns2d", "32"], d)
+    assert r.returncode == 1 and "infile type D" in r.stderr
+    setup_case(d, np.zeros(g.nkt, complex), extra=extra_2d(arclcFLAG="T"))
+    r = run(["ns2d", "32"], d)
+    assert r.returncode == 1 and "arc-length" in r.stderr
+    os.remove(os.path.join(d, "paramsEXTRA.txt"))
+    r = run(["ns2d", "32"], d)
+    assert r.returncode == 1
+
+
+@pytest.mark.gpu
+def test_driver_converges_laminar_state_and_writes_outputs(gpu, tmp_path):
+    d = str(tmp_path)
+    g = P.hgrid("ns2d", 32, 32)
+    kx, ky, _, L = P.tables(g)
+    lam = np.zeros(g.nkt, complex)
+    lam[(kx == 0) & (ky == 4)] = -5.0 / 8          # laminar Kolmogorov vorticity at Re = 5
+    z0 = lam + 1e-3 * P.ic_ns2d_cfg(g, 9, 1.0)[0]
+    setup_case(d, z0)
+    r = run(["ns2d", "32", "--noshift", "--maxnewton", "8"], d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    info = open(os.path.join(d, "UPOinfo.dat")).read().splitlines()
+    assert len(info) == 2 and '"CONVERGED"' in info[1]
+    upos = read_recs(os.path.join(d, "UPOs.out"))
+    assert struct.unpack("<3i", upos[0]) == (1, g.iktx, g.ikty) and len(upos) == 2
+    sc = struct.unpack("<6d", upos[1][:48])
+    assert sc[2] == 0.2 and sc[5] <= 1e-12
+    z = np.frombuffer(upos[1][48:], np.complex128)
+    assert np.abs((z - lam)[L == 1]).max() < 1e-9
+    zk = read_recs(os.path.join(d, "Zk.out"))
+    assert len(zk) == 3 and np.array_equal(np.frombuffer(zk[1][48:], np.complex128), z0)
