@@ -8,6 +8,7 @@ Outputs (git-ignored, shipped with the tree):
                                        reference symbols (include/psg_legacy.h)
     psgpu_amd/_lib/psg_gmres           the Newton-GMRES driver (C restatement of the
                                        reference's src_gmres/main.F90 callers)
+    psgpu_amd/_lib/psg_dns             the DNS driver (3D NS, Euler, Boussinesq main.F90)
 HIP sources are compiled for gfx950 only; the C host layer with gcc.
 """
 import argparse
@@ -39,8 +40,10 @@ LEGACY = [("ns3d_dns", "legacy/legacy_ns3d_dns.c"), ("ns3d_gmres", "legacy/legac
           ("bous_gmres", "legacy/legacy_bous_gmres.c")]
 
 
-# C drivers restating the reference's Fortran callers: (executable, source)
-DRIVERS = [("psg_gmres", "drivers/psg_gmres.c")]
+# C drivers restating the reference's Fortran callers: (executable, source, libraries)
+DRIVERS = [("psg_gmres", "drivers/psg_gmres.c", ["psg"]),
+           ("psg_dns", "drivers/psg_dns.c", ["psg", "psg_legacy_ns3d_dns", "psg_legacy_euler",
+                                             "psg_legacy_bous_dns"])]
 
 
 def _run(cmd, verbose):
@@ -80,15 +83,15 @@ def build(verbose=False, jobs=None):
     tmp = lib + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + core + ["-lm"], verbose)
     os.replace(tmp, lib)
-    for name, src in DRIVERS:
-        out = os.path.join(OUT, name)
-        _run(["gcc"] + C_FLAGS + [os.path.join(CSRC, src), "-o", out + ".tmp", "-L" + OUT, "-lpsg",
-              "-Wl,-rpath,$ORIGIN", "-lm"], verbose)
-        os.replace(out + ".tmp", out)
     for suffix, src in LEGACY:
         out = os.path.join(OUT, "libpsg_legacy_%s.so" % suffix)
         _run(["gcc", "-shared", "-fPIC", "-o", out + ".tmp", objs[src], "-L" + OUT, "-lpsg",
               "-Wl,-rpath,$ORIGIN", "-lm"], verbose)
+        os.replace(out + ".tmp", out)
+    for name, src, libs in DRIVERS:
+        out = os.path.join(OUT, name)
+        _run(["gcc"] + C_FLAGS + [os.path.join(CSRC, src), "-o", out + ".tmp", "-L" + OUT]
+             + ["-l" + l for l in libs] + ["-Wl,-rpath,$ORIGIN", "-lm"], verbose)
         os.replace(out + ".tmp", out)
     return lib
 

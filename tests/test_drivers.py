@@ -118,3 +118,72 @@ def test_driver_converges_laminar_state_and_writes_outputs(gpu, tmp_path):
     assert np.abs((z - lam)[L == 1]).max() < 1e-9
     zk = read_recs(os.path.join(d, "Zk.out"))
     assert len(zk) == 3 and np.array_equal(np.frombuffer(zk[1][48:], np.complex128), z0)
+
+
+# ---------------------------------------------------------------- DNS driver
+DNS_EXE = os.path.join(os.path.dirname(P.__file__), "_lib", "psg_dns")
+# order-fixed labels of 3D_NavierStokes/src/parameters.F90:134-149 and 3D_Euler/src/parameters.F90:127-136
+MAIN_NS3D = ["ISEED", "IREST", "infile", "infile type", "KF", "Re", "TSTEP", "ResidualThreshold", "outfreq1",
+             "outfreq2", "DELT", "AMPV", "STATSFLAG", "RCFLAG", "UPOflag", "ISTOP"]
+MAIN_EULER = ["ISEED", "IREST", "infile", "infile type", "TSTEP", "outfreq1", "outfreq2", "DELT", "AMPV",
+              "STATSFLAG"]
+
+
+def ns3d_params(**kw):
+    v = {"ISEED": 4242, "IREST": 0, "infile": "none", "infile type": "D", "KF": 4, "Re": "100.0", "TSTEP": "0.1",
+         "ResidualThreshold": "1.0", "outfreq1": "0.025", "outfreq2": "0.05", "DELT": "0.005", "AMPV": "1.0",
+         "STATSFLAG": 1, "RCFLAG": 0, "UPOflag": 0, "ISTOP": 0}
+    v.update(kw)
+    return v
+
+
+def test_dns_driver_usage_and_parameter_errors(tmp_path):
+    assert os.path.exists(DNS_EXE)
+    r = subprocess.run([DNS_EXE], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([DNS_EXE, "ns2d", "32"], capture_output=True, text=True)
+    assert r.returncode == 1 and "flavour" in r.stderr
+    r = subprocess.run([DNS_EXE, "ns3d", "32", "--dir", str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode == 1   # no JOB0/params.txt
+    os.makedirs(tmp_path / "JOB0")
+    write_params(str(tmp_path / "JOB0" / "params.txt"), MAIN_NS3D[:5], ns3d_params())
+    r = subprocess.run([DNS_EXE, "ns3d", "32", "--dir", str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode == 1 and "label" in r.stderr
+
+
+@pytest.mark.gpu
+def test_dns_driver_ns3d_reference_run_matches_oracle(gpu, tmp_path):
+    import oracle_lib as O
+    n = 32
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "JOB0"))
+    write_params(os.path.join(d, "JOB0", "params.txt"), MAIN_NS3D, ns3d_params())
+    r = subprocess.run([DNS_EXE, "ns3d", str(n), "--dir", d], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = O.grid3(n, n, n)
+    st = O.ic_ns3d_ref(g, 4242, 1.0)
+    want = O.NS3D(n, v2=0.01, dt=0.005, force=1, kf=4.0).run(st, 20)   # NSTOP = INT(0.1/0.005)
+    recs = read_recs(os.path.join(d, "JOB0", "final_000.out"))
+    assert struct.unpack("<4i", recs[0]) == (1, g.iktx, g.ikty, g.iktz)
+    got = np.frombuffer(recs[1][56:], np.complex128).reshape(3, g.nkt)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
+    assert len(open(os.path.join(d, "JOB0", "stats.dat")).read().strip().splitlines()) == 3
+    assert struct.unpack("<4i", read_recs(os.path.join(d, "JOB0", "Zk.out"))[0]) == (1, g.iktx, g.ikty, g.iktz)
+
+
+@pytest.mark.gpu
+def test_dns_driver_euler_reference_run_matches_oracle(gpu, tmp_path):
+    import oracle_lib as O
+    n = 32
+    d = str(tmp_path)
+    v = {"ISEED": 837920, "IREST": 0, "infile": "none", "infile type": "D", "TSTEP": "0.05", "outfreq1": "0.01",
+         "outfreq2": "0.05", "DELT": "0.01", "AMPV": "0.001", "STATSFLAG": 0}
+    write_params(os.path.join(d, "params.txt"), MAIN_EULER, v)
+    r = subprocess.run([DNS_EXE, "euler", str(n), "--dir", d], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = O.grid3(n, n, n)
+    st = O.ic_euler_ref(g, 837920, 0.001)
+    want = O.NS3D(n, dt=0.01, flavour=1, force=0).run(st, 5)   # NSTOP = INT(0.05/0.01)
+    recs = read_recs(os.path.join(d, "final_000.out"))
+    got = np.frombuffer(recs[1][56:], np.complex128).reshape(3, g.nkt)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
