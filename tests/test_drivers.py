@@ -187,3 +187,38 @@ def test_dns_driver_euler_reference_run_matches_oracle(gpu, tmp_path):
     recs = read_recs(os.path.join(d, "final_000.out"))
     got = np.frombuffer(recs[1][56:], np.complex128).reshape(3, g.nkt)
     assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
+
+
+MAIN_BOUS = ["ISEED", "IREST", "infile", "infile type", "KF", "Re", "Ri", "Theta", "Sc", "TSTEP",
+             "ResidualThreshold", "outfreq1", "outfreq2", "DELT", "AMPV", "STATSFLAG", "RCFLAG", "UPOflag",
+             "ADAPTflag", "ISTOP", "Dtarget"]
+
+
+@pytest.mark.gpu
+def test_dns_driver_bous_reference_run_matches_oracle(gpu, tmp_path):
+    import oracle_lib as O
+    n = 32
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "JOB0"))
+    v = {"ISEED": 5151, "IREST": 0, "infile": "none", "infile type": "D", "KF": 4, "Re": "100.0", "Ri": "1.0",
+         "Theta": "90.0", "Sc": "1.0", "TSTEP": "0.06", "ResidualThreshold": "1.0", "outfreq1": "0.02",
+         "outfreq2": "1.0", "DELT": "0.004", "AMPV": "1.0", "STATSFLAG": 0, "RCFLAG": 0, "UPOflag": 0,
+         "ADAPTflag": 0, "ISTOP": 0, "Dtarget": "0.0"}
+    write_params(os.path.join(d, "JOB0", "params.txt"), MAIN_BOUS, v)
+    r = subprocess.run([DNS_EXE, "bous", str(n), "--dir", d], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = O.grid3(n, n, n)
+    st = O.ic_bous_ref(g, 5151, 1.0)
+    # stepping while t <= TSTEP with dt = TSTEP / INT(TSTEP/DELT) (3D_Boussinesq/src/cuda.cu:68-99)
+    dt, tstep = 0.004, 0.06
+    nstop = int(tstep / dt)
+    dd = tstep / nstop if nstop * dt != tstep else dt
+    tme, nst = 0.0, 0
+    while tme <= tstep:
+        tme += dd
+        nst += 1
+    want = O.BOUS(n, v2=0.01, sc=1.0, ri=1.0, theta=np.pi / 2, dt=dd).run(st, nst)
+    recs = read_recs(os.path.join(d, "JOB0", "final_000.out"))
+    assert struct.unpack("<d", recs[1][:8])[0] == pytest.approx(tme, abs=1e-14)
+    got = np.frombuffer(recs[1][56:], np.complex128).reshape(4, g.nkt)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
