@@ -8,7 +8,7 @@ Outputs (git-ignored, shipped with the tree):
                                        reference symbols (include/psg_legacy.h)
     psgpu_amd/_lib/psg_gmres           the Newton-GMRES driver (C restatement of the
                                        reference's src_gmres/main.F90 callers)
-    psgpu_amd/_lib/psg_dns             the DNS driver (3D NS, Euler, Boussinesq main.F90)
+    psgpu_amd/_lib/psg_dns             the DNS driver (3D NS, Euler, Boussinesq, 2D NS main.F90)
 HIP sources are compiled for gfx950 only; the C host layer with gcc.
 """
 import argparse
@@ -42,7 +42,7 @@ LEGACY = [("ns3d_dns", "legacy/legacy_ns3d_dns.c"), ("ns3d_gmres", "legacy/legac
 
 # C drivers restating the reference's Fortran callers: (executable, source, libraries)
 DRIVERS = [("psg_gmres", "drivers/psg_gmres.c", ["psg"]),
-           ("psg_dns", "drivers/psg_dns.c", ["psg", "psg_legacy_ns3d_dns", "psg_legacy_euler",
+           ("psg_dns", "drivers/psg_dns.c", ["psg", "psg_legacy_ns3d_dns", "psg_legacy_euler", "psg_legacy_ns2d_dns",
                                              "psg_legacy_bous_dns"])]
 
 

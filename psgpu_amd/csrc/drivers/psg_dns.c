@@ -9,6 +9,11 @@
  *   bous   3D_Boussinesq/src/main.F90:43-230: JOB<rank>/params.txt
  *          (parameters.F90:118-140, params_update: AMPFOR = -KF/2, Theta in
  *          degrees converted to radians), stepping while t <= TSTEP.
+ *   ns2d   2D_NavierStokes/src_RK4/main.F90:1-305: params.txt
+ *          (parameters.F90:75-101), one run of INT(TSTEP/DELT) RK4 steps, or
+ *          of one period PERIODin of a restart record when UPOflag = 1; the
+ *          final state, and with UPOflag = 1 its shifted copy and the
+ *          residual ||F||/||Z||, appended to Zk.out.
  *
  * INIT (3D_NavierStokes/src/init.F90:25-230 and the flavours' counterparts):
  * wavenumbers and the mask from psgh_tables, the CN tables from psgh_nutab,
@@ -19,6 +24,7 @@
  * (global_params.F90), is given on the command line; rank 0.
  *
  * usage: psg_dns ns3d|euler|bous NX [NY NZ ALPHA] [--dir D]
+ *        psg_dns ns2d NX [NY ALPHA] [--dir D]
  */
 #include <math.h>
 #include <stdio.h>
@@ -55,14 +61,130 @@ static int load_record(const char *path, long irest, const psgh_grid *g, int nf,
   return rc;
 }
 
+/* scalars ahead of the field in a 2D restart record, by infile type, and
+ * where PERIODin, SHIFTXin, SHIFTYin, NERin and v2 sit among them (-1: not
+ * present); 2D_NavierStokes/src_RK4/init.F90:66-106.  Type B's first record
+ * holds the time only. */
+typedef struct { char type; int ns, period, shiftx, shifty, ner, v2; } rec2d_t;
+static const rec2d_t REC2D[] = {
+  {'A', 1, -1, -1, -1, -1, -1}, {'B', 5, 1, 2, -1, 3, -1}, {'C', 5, 1, 2, -1, 3, -1},
+  {'D', 6, 2, 3, -1, 4, 1},     {'E', 6, 2, 3, -1, 4, -1}, {'G', 5, 1, 2, 3, 4, -1},
+  {'H', 6, 2, 3, 4, 5, 1},      {'I', 6, 2, 3, 4, 5, -1},  {'J', 7, 2, 3, 4, 5, -1}};
+
+static int run_ns2d(int nx, int ny, double alpha) {
+  char msg[256];
+  static psgh_kv kv;
+  if (psgh_params_read("params.txt", PSGH_PARAMS_NS2D_DNS, &kv, msg, sizeof msg)) return die(msg);
+  const int ISEED = (int)psgh_kv_int(&kv, "ISEED", 1), IREST = (int)psgh_kv_int(&kv, "IREST", 0);
+  const int UPOflag = (int)psgh_kv_int(&kv, "UPOflag", 0);
+  int STFLAG = (int)psgh_kv_int(&kv, "STFLAG", 0), RCFLAG = 0;
+  const char *infile = psgh_kv_str(&kv, "infile"), *itype = psgh_kv_str(&kv, "infile type");
+  double v2 = 1.0 / psgh_kv_float(&kv, "Re", 1.0), DELT = psgh_kv_float(&kv, "DELT", 0.01);
+  double TSTEP = psgh_kv_float(&kv, "TSTEP", 1.0), ResThresh = psgh_kv_float(&kv, "ResidualThreshold", 1.0);
+  const double AMPV = 1.0;   /* params_update */
+  int NOUT = (int)(psgh_kv_float(&kv, "outfreq1", DELT) / DELT);
+  if (NOUT == 0) NOUT = 1;
+
+  psgh_grid g;
+  if (psgh_grid2(&g, nx, ny, alpha)) return die("bad grid");
+  const long nkt = g.nkt;
+  double *kx = malloc(sizeof(double) * nkt), *ky = malloc(sizeof(double) * nkt);
+  double *z = calloc(2 * nkt, sizeof(double)), *z0 = calloc(2 * nkt, sizeof(double));
+  double *zr = malloc(sizeof(double) * (long)nx * ny), *rec = NULL;
+  int *L = malloc(sizeof(int) * nkt), *ikf = malloc(sizeof(int) * (long)g.nx2 * ny), *ikn = malloc(sizeof(int) * nkt);
+  if (!kx || !ky || !z || !z0 || !zr || !L || !ikf || !ikn) return die("out of memory");
+  int IKTX = g.iktx, IKTY = g.ikty, KTY = g.kty, NX = nx, NY = ny;
+
+  psgh_rng_seed(ISEED);   /* TIME = RANNO(ISEED) */
+  FILE *fzk = fopen("Zk.out", "wb");
+  if (!fzk) return die("cannot create Zk.out");
+  psgh_zk_header(fzk, 1, IKTX, IKTY, 0);
+  double TSTART = TSTEP, PERIODin = 0, SHIFTXin = 0, SHIFTYin = 0, NERin = 0;
+  psgh_tables(&g, kx, ky, NULL, L);
+  if (IREST == 0) {
+    psgh_ic_ns2d_ref(&g, ISEED, AMPV, z);
+  } else {
+    const rec2d_t *rt = NULL;
+    for (unsigned i = 0; i < sizeof REC2D / sizeof REC2D[0]; i++)
+      if (itype && itype[0] == REC2D[i].type && itype[1] == 0) rt = &REC2D[i];
+    if (!rt) return die("restart: infile type unknown");
+    FILE *f = infile ? fopen(infile, "rb") : NULL;
+    if (!f) return die("restart: cannot open infile");
+    int hdr[3];
+    if (psgh_rec_read(f, hdr, sizeof hdr) < 12) return die("restart: infile has no header record");
+    rec = malloc(sizeof(double) * (8 + 2 * nkt));
+    for (int ntime = 1; ntime <= IREST; ntime++) {
+      const int ns = (rt->type == 'B' && ntime == 1) ? 1 : rt->ns;
+      const long want = 8L * ns + 16L * nkt;
+      if (psgh_rec_read(f, rec, want) != want) return die("restart: infile record does not match the grid or has fewer records than IREST");
+      if (ns > 1) {
+        PERIODin = rt->period >= 0 ? rec[rt->period] : PERIODin;
+        SHIFTXin = rt->shiftx >= 0 ? rec[rt->shiftx] : SHIFTXin;
+        SHIFTYin = rt->shifty >= 0 ? rec[rt->shifty] : SHIFTYin;
+        NERin = rt->ner >= 0 ? rec[rt->ner] : NERin;
+        if (rt->v2 >= 0) v2 = rec[rt->v2];
+      }
+      memcpy(z, rec + ns, sizeof(double) * 2 * nkt);
+      printf(" Just read time= %.15g\n", rec[0]);
+    }
+    fclose(f);
+    for (long i = 0; i < nkt; i++)
+      if (L[i] != 1) z[2 * i] = z[2 * i + 1] = 0.0;
+  }
+  if (UPOflag == 1) TSTEP = PERIODin;
+  const double TIME = TSTART;
+  for (long i = 0; i < nkt; i++)
+    if (L[i]) { z0[2 * i] = z[2 * i]; z0[2 * i + 1] = z[2 * i + 1]; }
+  int NIT = (int)(TSTEP / DELT);
+  psgh_ikf_ikn(&g, ikf, ikn);
+  if (UPOflag == 1) {
+    if (PERIODin <= 0.0) return die("UPOflag = 1 needs a restart record with a period");
+    NIT = (int)(PERIODin / DELT);
+    DELT = PERIODin / (double)NIT;
+    TSTEP = PERIODin;
+  }
+  const clock_t c0 = clock();
+  timestep_cuda_ns2d_dns((psg_double2 *)z, zr, kx, ky, &TSTEP, &TSTART, &DELT, &alpha, &v2, &SHIFTXin, &ResThresh,
+                         ikf, ikn, &IKTX, &IKTY, &KTY, &NX, &NY, &NIT, &NOUT, L, &RCFLAG, &STFLAG);
+  printf(" CPU time required for main loop = %7.3f s\n", (double)(clock() - c0) / CLOCKS_PER_SEC);
+  const double sc[6] = {TIME, v2, TSTEP, SHIFTXin, 0.0, 0.0};
+  psgh_zk_record(fzk, sc, 6, z, nkt);
+  if (UPOflag == 1) {
+    /* shift the final state back and compare with the initial one (main.F90:243-290) */
+    double *zs = calloc(2 * nkt, sizeof(double)), nz2 = 0, nf2 = 0;
+    for (long i = 0; i < nkt; i++) {
+      if (!L[i]) continue;
+      const double ph = -(kx[i] * SHIFTXin + ky[i] * SHIFTYin), c = cos(ph), s = sin(ph);
+      const double a = z[2 * i], b = z[2 * i + 1];
+      zs[2 * i] = c * a - s * b;
+      zs[2 * i + 1] = s * a + c * b;
+      nz2 += a * a + b * b;
+      const double fr = zs[2 * i] - z0[2 * i], fi = zs[2 * i + 1] - z0[2 * i + 1];
+      nf2 += fr * fr + fi * fi;
+    }
+    const double NER = sqrt(nf2) / sqrt(nz2);
+    printf(" Newton convergence check: ||F|| / ||Z|| = %.15g\n", NER);
+    const double sc2[6] = {TIME, v2, 0.0, 0.0, 0.0, 0.0};
+    psgh_zk_record(fzk, sc2, 6, zs, nkt);
+    if (NER != NERin) printf(" WARNING: calculated ||F||/||Z|| does not match GMRES\n");
+    free(zs);
+  }
+  fclose(fzk);
+  printf(" Finished\n");
+  free(kx); free(ky); free(z); free(z0); free(zr); free(rec); free(L); free(ikf); free(ikn);
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc < 3) return die("usage: psg_dns ns3d|euler|bous NX [NY NZ ALPHA] [--dir D]");
+  if (argc < 3) return die("usage: psg_dns ns3d|euler|bous NX [NY NZ ALPHA] [--dir D] | ns2d NX [NY ALPHA] [--dir D]");
   const char *fl = argv[1];
   const int ns3d = !strcmp(fl, "ns3d"), euler = !strcmp(fl, "euler"), bous = !strcmp(fl, "bous");
-  if (!ns3d && !euler && !bous) return die("flavour must be ns3d, euler or bous");
+  const int ns2d = !strcmp(fl, "ns2d");
+  if (!ns3d && !euler && !bous && !ns2d) return die("flavour must be ns3d, euler, bous or ns2d");
   int nx = atoi(argv[2]), ny = nx, nz = nx, ai = 3;
   double alpha = 1.0;
-  if (argc > 5 && argv[3][0] != '-') { ny = atoi(argv[3]); nz = atoi(argv[4]); alpha = atof(argv[5]); ai = 6; }
+  if (ns2d && argc > 4 && argv[3][0] != '-') { ny = atoi(argv[3]); alpha = atof(argv[4]); ai = 5; }
+  if (!ns2d && argc > 5 && argv[3][0] != '-') { ny = atoi(argv[3]); nz = atoi(argv[4]); alpha = atof(argv[5]); ai = 6; }
   for (int i = ai; i < argc; i++) {
     if (!strcmp(argv[i], "--dir") && i + 1 < argc) {
       if (chdir(argv[++i])) return die("cannot enter the run directory");
@@ -70,6 +192,7 @@ int main(int argc, char **argv) {
       return die("unknown option");
     }
   }
+  if (ns2d) return run_ns2d(nx, ny, alpha);
   int RANK = 0;
   const char *simdir = euler ? "." : "JOB0";
   char path[1024], msg[256];

@@ -141,7 +141,7 @@ def test_dns_driver_usage_and_parameter_errors(tmp_path):
     assert os.path.exists(DNS_EXE)
     r = subprocess.run([DNS_EXE], capture_output=True, text=True)
     assert r.returncode == 1 and "usage" in r.stderr
-    r = subprocess.run([DNS_EXE, "ns2d", "32"], capture_output=True, text=True)
+    r = subprocess.run([DNS_EXE, "ns1d", "32"], capture_output=True, text=True)
     assert r.returncode == 1 and "flavour" in r.stderr
     r = subprocess.run([DNS_EXE, "ns3d", "32", "--dir", str(tmp_path)], capture_output=True, text=True)
     assert r.returncode == 1   # no JOB0/params.txt
@@ -222,3 +222,76 @@ def test_dns_driver_bous_reference_run_matches_oracle(gpu, tmp_path):
     assert struct.unpack("<d", recs[1][:8])[0] == pytest.approx(tme, abs=1e-14)
     got = np.frombuffer(recs[1][56:], np.complex128).reshape(4, g.nkt)
     assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
+
+
+MAIN_NS2D_DNS = ["ISEED", "IREST", "infile", "infile type", "Re", "TSTEP", "outfreq1", "outfreq2", "DELT", "AMPFOR",
+                 "SCALE", "ResidualThreshold", "STFLAG", "TDFCFLAG", "ICFLAG", "WNFLAG", "initial x shift",
+                 "initial y shift", "lamFLAG", "UPOflag"]
+
+
+def ns2d_dns_params(**kw):
+    v = {"ISEED": 837920, "IREST": 0, "infile": "guess.out", "infile type": "D", "Re": "40.0", "TSTEP": "0.2",
+         "outfreq1": "0.05", "outfreq2": "1.0", "DELT": "0.005", "AMPFOR": "-2.0", "SCALE": "0.0",
+         "ResidualThreshold": "1.0", "STFLAG": 1, "TDFCFLAG": 0, "ICFLAG": 0, "WNFLAG": 0, "initial x shift": "0.0",
+         "initial y shift": "0.0", "lamFLAG": 0, "UPOflag": 0}
+    v.update(kw)
+    return v
+
+
+def test_dns_driver_ns2d_rejects_unknown_infile_type(tmp_path):
+    d = str(tmp_path)
+    write_params(os.path.join(d, "params.txt"), MAIN_NS2D_DNS, ns2d_dns_params(IREST=1, **{"infile type": "Q"}))
+    r = subprocess.run([DNS_EXE, "ns2d", "32", "--dir", d], capture_output=True, text=True)
+    assert r.returncode == 1 and "infile type" in r.stderr
+
+
+@pytest.mark.gpu
+def test_dns_driver_ns2d_reference_run_matches_oracle(gpu, tmp_path):
+    import oracle_lib as O
+    n = 64
+    d = str(tmp_path)
+    write_params(os.path.join(d, "params.txt"), MAIN_NS2D_DNS, ns2d_dns_params())
+    r = subprocess.run([DNS_EXE, "ns2d", str(n), "--dir", d], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = O.grid2(n, n)
+    want = O.NS2D(n, v2=1 / 40.0, dt=0.005).run(O.ic_ns2d_ref(g, 837920, 1.0), 40)   # INT(0.2/0.005)
+    recs = read_recs(os.path.join(d, "Zk.out"))
+    assert struct.unpack("<3i", recs[0]) == (1, g.iktx, g.ikty) and len(recs) == 2
+    sc = struct.unpack("<6d", recs[1][:48])
+    assert sc[:3] == (0.2, 1 / 40.0, 0.2)          # TIME = TSTART = TSTEP, v2, TSTEP
+    got = np.frombuffer(recs[1][48:], np.complex128)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
+    assert len(open(os.path.join(d, "stats.dat")).read().strip().splitlines()) == 4   # every 10 of 40 steps
+
+
+@pytest.mark.gpu
+def test_dns_driver_ns2d_upo_restart_shift_and_residual(gpu, tmp_path):
+    import oracle_lib as O
+    n = 32
+    d = str(tmp_path)
+    g = O.grid2(n, n)
+    kx, ky, _, L = O.tables(g)
+    z0 = O.ic_ns2d_ref(g, 4321, 1.0)
+    period, sx, v2 = 0.33, 0.7, 1 / 30.0
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<3i", 1, g.iktx, g.ikty)))
+        f.write(rec(struct.pack("<6d", 5.0, v2, period, sx, 0.25, 0.0) + z0.astype(np.complex128).tobytes()))
+    # Re in params.txt is overridden by the v2 of the type-D record (init.F90:82-84)
+    write_params(os.path.join(d, "params.txt"), MAIN_NS2D_DNS,
+                 ns2d_dns_params(IREST=1, UPOflag=1, STFLAG=0, Re="100.0", DELT="0.01"))
+    r = subprocess.run([DNS_EXE, "ns2d", str(n), "--dir", d], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    nit = int(period / 0.01)
+    zT = O.NS2D(n, v2=v2, dt=period / nit).run(z0, nit)
+    recs = read_recs(os.path.join(d, "Zk.out"))
+    assert len(recs) == 3
+    assert struct.unpack("<4d", recs[1][:32]) == (0.2, v2, period, sx)
+    got = np.frombuffer(recs[1][48:], np.complex128)
+    assert np.abs(got - zT).max() / np.abs(zT).max() < 1e-11
+    shifted = np.where(L == 1, np.exp(-1j * kx * sx) * zT, 0)
+    got_s = np.frombuffer(recs[2][48:], np.complex128)
+    assert np.abs(got_s - shifted).max() / np.abs(shifted).max() < 1e-11
+    ner = np.linalg.norm((shifted - z0)[L == 1]) / np.linalg.norm(zT[L == 1])
+    line = [l for l in r.stdout.splitlines() if "||F|| / ||Z||" in l][0]
+    assert float(line.split("=")[-1]) == pytest.approx(ner, rel=1e-9)
+    assert "WARNING" in r.stdout   # NER differs from the record's 0.25
