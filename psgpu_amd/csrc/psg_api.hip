@@ -218,7 +218,7 @@ static void free_ctx(Ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->graph_exec)
     if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
-  void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
+  void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->st32, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
                   c->tl_rh, c->tl_bufA, c->tl_bufB, c->tl_bufC, c->bufA, c->bufB, c->bufC,
                   c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred};
   for (void *p : ptrs)

@@ -47,6 +47,7 @@ struct Ctx {
   cpx<double> *rh = nullptr;        // Heun: RH store; RK4: stage arrays
   cpx<double> *stage = nullptr;     // RK4: Z0 stage input / generic scratch
   cpx<double> *tmp = nullptr;       // scratch state-sized buffer
+  cpx<float> *st32 = nullptr;       // FP32-bulk 3D Navier-Stokes/Euler: working state of a run
   double *nuzn = nullptr, *nu1 = nullptr;  // CN tables (host-built, reference formula)
   double *nu = nullptr;                    // NU = 10^(log10 v2 + 2 log10|k|), for dz/dt
   int8_t *mask = nullptr;           // active mask L

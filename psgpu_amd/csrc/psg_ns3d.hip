@@ -17,9 +17,10 @@ namespace psg {
 
 // R: storage precision of the predictor's right-hand side RH, which the
 // corrector subtracts (the pass precision: in FP32-bulk it only carries
-// FP32-accurate products).
-template <int PHASE, bool EULER, typename R = double> struct UpdHeun {
-  cpx<double> *st;
+// FP32-accurate products).  S: storage type of the state (double; float for
+// the working state of an FP32-bulk run).  The update is evaluated in FP64.
+template <int PHASE, bool EULER, typename R = double, typename S = double> struct UpdHeun {
+  cpx<S> *st;
   cpx<R> *rh;
   long ms, fs;  // member stride, field stride (same for st and rh)
   const double *nuzn, *nu1;
@@ -29,7 +30,7 @@ template <int PHASE, bool EULER, typename R = double> struct UpdHeun {
   __device__ __forceinline__ void operator()(int m, long ik, int, double kx, double ky, double kz,
                                              cpx<double> (&N)[3]) const {
     PSG_CURL
-    cpx<double> *s = st + (long)m * ms + ik;
+    cpx<S> *s = st + (long)m * ms + ik;
     cpx<R> *r = rh + (long)m * ms + ik;
     const double ll = (double)L[ik];
     if constexpr (PHASE == 0) {
@@ -37,10 +38,10 @@ template <int PHASE, bool EULER, typename R = double> struct UpdHeun {
         r[0] = to<R>(cpx<double>{ll * xiX, ll * xiY});
         r[fs] = to<R>(cpx<double>{ll * etX, ll * etY});
         r[2 * fs] = to<R>(cpx<double>{ll * ztX, ll * ztY});
-        cpx<double> a = s[0], b = s[fs], c = s[2 * fs];
-        s[0] = cpx<double>{a.x + dt * xiX, a.y + dt * xiY};
-        s[fs] = cpx<double>{b.x + dt * etX, b.y + dt * etY};
-        s[2 * fs] = cpx<double>{c.x + dt * ztX, c.y + dt * ztY};
+        const cpx<double> a = up(s[0]), b = up(s[fs]), c = up(s[2 * fs]);
+        s[0] = to<S>(cpx<double>{a.x + dt * xiX, a.y + dt * xiY});
+        s[fs] = to<S>(cpx<double>{b.x + dt * etX, b.y + dt * etY});
+        s[2 * fs] = to<S>(cpx<double>{c.x + dt * ztX, c.y + dt * ztY});
       } else {
         const double NUZN = nuzn[ik];
         const double NU1 = ll * nu1[ik];
@@ -48,10 +49,10 @@ template <int PHASE, bool EULER, typename R = double> struct UpdHeun {
         r[0] = to<R>(cpx<double>{xiX, xiY});
         r[fs] = to<R>(cpx<double>{etX, etY});
         r[2 * fs] = to<R>(cpx<double>{ztX, ztY});
-        cpx<double> a = s[0], b = s[fs], c = s[2 * fs];
-        s[0] = cpx<double>{NU1 * (NUZN * a.x + xiX), NU1 * (NUZN * a.y + xiY)};
-        s[fs] = cpx<double>{NU1 * (NUZN * b.x + etX), NU1 * (NUZN * b.y + etY)};
-        s[2 * fs] = cpx<double>{NU1 * (NUZN * c.x + ztX), NU1 * (NUZN * c.y + ztY)};
+        const cpx<double> a = up(s[0]), b = up(s[fs]), c = up(s[2 * fs]);
+        s[0] = to<S>(cpx<double>{NU1 * (NUZN * a.x + xiX), NU1 * (NUZN * a.y + xiY)});
+        s[fs] = to<S>(cpx<double>{NU1 * (NUZN * b.x + etX), NU1 * (NUZN * b.y + etY)});
+        s[2 * fs] = to<S>(cpx<double>{NU1 * (NUZN * c.x + ztX), NU1 * (NUZN * c.y + ztY)});
       }
     } else {
       double NU2;
@@ -60,11 +61,11 @@ template <int PHASE, bool EULER, typename R = double> struct UpdHeun {
         NU2 = 0.5 * nu1[ik];
         add_forcing(F, kx, ky, kz, xiX, etY, ztX, ztY);
       }
-      cpx<double> a = s[0], b = s[fs], c = s[2 * fs];
+      const cpx<double> a = up(s[0]), b = up(s[fs]), c = up(s[2 * fs]);
       const cpx<double> ra = up(r[0]), rb = up(r[fs]), rc = up(r[2 * fs]);
-      s[0] = cpx<double>{ll * (a.x + NU2 * (xiX - ra.x)), ll * (a.y + NU2 * (xiY - ra.y))};
-      s[fs] = cpx<double>{ll * (b.x + NU2 * (etX - rb.x)), ll * (b.y + NU2 * (etY - rb.y))};
-      s[2 * fs] = cpx<double>{ll * (c.x + NU2 * (ztX - rc.x)), ll * (c.y + NU2 * (ztY - rc.y))};
+      s[0] = to<S>(cpx<double>{ll * (a.x + NU2 * (xiX - ra.x)), ll * (a.y + NU2 * (xiY - ra.y))});
+      s[fs] = to<S>(cpx<double>{ll * (b.x + NU2 * (etX - rb.x)), ll * (b.y + NU2 * (etY - rb.y))});
+      s[2 * fs] = to<S>(cpx<double>{ll * (c.x + NU2 * (ztX - rc.x)), ll * (c.y + NU2 * (ztY - rc.y))});
     }
   }
 };
@@ -155,12 +156,12 @@ template <typename P> static int xpass_ns(Ctx *c, const cpx<P> *B, cpx<P> *Cb, i
 }
 
 // One right-hand side evaluation for all members, ending in functor upd.
-template <typename P, class UPD>
-static int rhs3(Ctx *c, const cpx<double> *st, long st_ms, int umask, UPD upd, long state_bytes, int nm) {
+template <typename P, class UPD, typename S = double>
+static int rhs3(Ctx *c, const cpx<S> *st, long st_ms, int umask, UPD upd, long state_bytes, int nm) {
   const Grid &g = c->g;
   cpx<P> *A = (cpx<P> *)c->bufA, *B = (cpx<P> *)c->bufB, *Cb = (cpx<P> *)c->bufC;
   long pb[5];
-  pass_bytes(c, 6, 3, sizeof(cpx<P>), pb);
+  pass_bytes(c, 6, 3, sizeof(cpx<P>), pb, sizeof(cpx<S>));
   prof_begin(c, "zinv", pb[0] * nm);
   PSG_TRY(launch_zinv<P>(c, st, st_ms, A, 6 * g.zsz, true, umask, nm));
   prof_end(c);
@@ -180,33 +181,54 @@ static int rhs3(Ctx *c, const cpx<double> *st, long st_ms, int umask, UPD upd, l
   return 0;
 }
 
-template <typename P> static int run_t(Ctx *c, int nsteps, bool first) {
+// Heun steps of the state st (storage type S) with the passes in precision P
+template <typename P, typename S> static int run_t(Ctx *c, cpx<S> *st, int nsteps, bool first) {
   const Grid &g = c->g;
   const bool euler = c->cfg.flavour == PSG_EULER3D;
   Force F{c->cfg.force, c->cfg.kf, c->cfg.ampfor};
   const long ms = 3 * g.nkt, fs = g.nkt;
-  // state (FP64) + RH (pass precision) traffic per mode of each half-step
-  const long sb = (6 * 16 + 3 * (long)sizeof(cpx<P>)) * g.nkt;
+  // state + RH (pass precision) traffic per mode of each half-step
+  const long sb = (6 * (long)sizeof(cpx<S>) + 3 * (long)sizeof(cpx<P>)) * g.nkt;
   cpx<P> *rh = (cpx<P> *)c->rh;
   for (int nt = 0; nt < nsteps; nt++) {
     const int umask = (first && nt == 0) ? 1 : 0;
     if (euler) {
-      UpdHeun<0, true, P> up{c->state, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb, c->batch));
-      UpdHeun<1, true, P> uc{c->state, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb, c->batch));
+      UpdHeun<0, true, P, S> up{st, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      PSG_TRY(rhs3<P>(c, st, ms, umask, up, sb, c->batch));
+      UpdHeun<1, true, P, S> uc{st, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      PSG_TRY(rhs3<P>(c, st, ms, 0, uc, sb, c->batch));
     } else {
-      UpdHeun<0, false, P> up{c->state, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb, c->batch));
-      UpdHeun<1, false, P> uc{c->state, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb, c->batch));
+      UpdHeun<0, false, P, S> up{st, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      PSG_TRY(rhs3<P>(c, st, ms, umask, up, sb, c->batch));
+      UpdHeun<1, false, P, S> uc{st, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      PSG_TRY(rhs3<P>(c, st, ms, 0, uc, sb, c->batch));
     }
   }
   return 0;
 }
 
+// element-wise precision change of n complex values
+template <typename A, typename B> __global__ void k_convert(const cpx<A> *__restrict__ a, cpx<B> *__restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = cpx<B>{(B)a[i].x, (B)a[i].y};
+}
+
+template <typename A, typename B> static int convert(Ctx *c, const cpx<A> *a, cpx<B> *b, long n) {
+  hipLaunchKernelGGL((k_convert<A, B>), dim3((unsigned)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0,
+                     c->stream, a, b, n);
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+// FP32-bulk steps a float copy of the state (st32): FP64 is kept for the
+// state between runs, where Newton's residual, the diagnostics and the API
+// read it; the run starts from the state rounded to FP32.
 int ns3d_run(Ctx *c, int nsteps, bool first) {
-  return c->cfg.precision == PSG_FP32BULK ? run_t<float>(c, nsteps, first) : run_t<double>(c, nsteps, first);
+  if (c->cfg.precision != PSG_FP32BULK) return run_t<double, double>(c, c->state, nsteps, first);
+  const long n = 3 * c->g.nkt * c->batch;
+  PSG_TRY((convert<double, float>(c, c->state, c->st32, n)));
+  PSG_TRY((run_t<float, float>(c, c->st32, nsteps, first)));
+  return convert<float, double>(c, c->st32, c->state, n);
 }
 
 // One Navier-Stokes step of member 0 that records w^n (before the step) and
@@ -316,6 +338,7 @@ int ns3d_init(Ctx *c) {
   PSG_HIP(hipMalloc(&c->bufC, c->szC));
   PSG_HIP(hipMalloc(&c->rh, sizeof(cpx<double>) * 3 * g.nkt * c->batch));
   PSG_HIP(hipMemsetAsync(c->rh, 0, sizeof(cpx<double>) * 3 * g.nkt * c->batch, c->stream));
+  if (c->cfg.precision == PSG_FP32BULK) PSG_HIP(hipMalloc(&c->st32, sizeof(cpx<float>) * 3 * g.nkt * c->batch));
   return 0;
 }
 

@@ -40,8 +40,8 @@ __device__ __forceinline__ void add_forcing(const Force &F, double kx, double ky
 template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
 template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
 
-template <typename P>
-static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, long out_ms, bool bs, int umask,
+template <typename P, typename S = double>
+static int launch_zinv(Ctx *c, const cpx<S> *st, long st_ms, cpx<P> *out, long out_ms, bool bs, int umask,
                        int nm) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nz, {
@@ -49,12 +49,12 @@ static int launch_zinv(Ctx *c, const cpx<double> *st, long st_ms, cpx<P> *out, l
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     if (bs) {
-      auto k = k_zinv<P, N, T, true>;
+      auto k = k_zinv<P, N, T, true, S>;
       PSG_HIP(lds_attr(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
                          twp<P>(c, 2), c->mask, umask);
     } else {
-      auto k = k_zinv<P, N, T, false>;
+      auto k = k_zinv<P, N, T, false, S>;
       PSG_HIP(lds_attr(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
                          twp<P>(c, 2), c->mask, umask);
@@ -112,10 +112,10 @@ static int launch_zfwd(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
 
 // bytes of the five passes, per member, for the profiler (algorithmic:
 // every buffer element read or written once)
-inline void pass_bytes(const Ctx *c, int fi, int fo, size_t b, long out[5]) {
+inline void pass_bytes(const Ctx *c, int fi, int fo, size_t b, long out[5], size_t sb = 16) {
   const Grid &g = c->g;
   long Z = (long)g.iktx * g.ikty * g.nz, Y = (long)g.iktx * g.ny * g.nz;
-  out[0] = 3 * g.nkt * 16 + fi * Z * (long)b;
+  out[0] = 3 * g.nkt * (long)sb + fi * Z * (long)b;
   out[1] = fi * (Z + Y) * (long)b;
   out[2] = (fi + fo) * Y * (long)b;
   out[3] = fo * (Y + Z) * (long)b;

@@ -80,6 +80,7 @@ __device__ __forceinline__ TileId xcd_tile() {
   tw = stage_twiddles<P, N, E>(tw, reinterpret_cast<cpx<P> *>(smem_raw))
 
 template <typename P> __device__ __forceinline__ cpx<P> to(cpx<double> a) { return {(P)a.x, (P)a.y}; }
+template <typename P> __device__ __forceinline__ cpx<P> to(cpx<float> a) { return {(P)a.x, (P)a.y}; }
 __device__ __forceinline__ cpx<double> up(cpx<double> a) { return a; }
 __device__ __forceinline__ cpx<double> up(cpx<float> a) { return {(double)a.x, (double)a.y}; }
 
@@ -142,16 +143,19 @@ __device__ __forceinline__ FillIdx fill_index(const Grid &g, int ikx, int iky, i
   r.conj = mirror;
   return r;
 }
-__device__ __forceinline__ cpx<double> ld_fill(const cpx<double> *__restrict__ f, const FillIdx &q) {
-  if (q.idx < 0) return cpx<double>{0.0, 0.0};
-  cpx<double> a = f[q.idx];
+template <typename S>
+__device__ __forceinline__ cpx<S> ld_fill(const cpx<S> *__restrict__ f, const FillIdx &q) {
+  if (q.idx < 0) return cpx<S>{0, 0};
+  cpx<S> a = f[q.idx];
   if (q.conj) a.y = -a.y;
   return a;
 }
 
-template <typename P, int N, int T, bool BS>
+// S: the state's storage type (double; float for the FP32 working state of
+// an FP32-bulk run)
+template <typename P, int N, int T, bool BS, typename S = double>
 __global__ void __launch_bounds__(T *(N / EPT))
-k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
+k_zinv(const cpx<S> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
        const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask) {
   PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
@@ -161,7 +165,7 @@ k_zinv(const cpx<double> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__res
   const bool valid = ikx < g.iktx;
   const int ky = iky - g.kty;
   const double kx = (double)ikx * g.alpha, kyd = (double)ky;
-  const cpx<double> *s0 = st + (long)m * st_ms;
+  const cpx<S> *s0 = st + (long)m * st_ms;
   // every state value of the pencil is requested before the first transform;
   // it is kept in the pass precision (in FP32-bulk the Biot-Savart product then
   // starts from the rounded vorticity, and the registers halve)

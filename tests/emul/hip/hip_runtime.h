@@ -186,7 +186,15 @@ template <typename F> struct Entry {
     setcontext(&g->sched);
   }
 };
-inline thread_local std::vector<char *> stack_pool;
+// fiber stacks of this host thread, reused by its workgroups and freed when
+// the thread ends (the OpenMP runtime may retire workers before exit)
+struct StackPool {
+  std::vector<char *> v;
+  ~StackPool() {
+    for (char *p : v) free(p);
+  }
+};
+inline thread_local StackPool stack_pool;
 // Run one workgroup: in each round every unfinished fiber runs to its next
 // barrier or to its end.  A round in which some fibers stop at a barrier and
 // others end is a barrier that not every thread reaches (a hang on the device):
@@ -198,10 +206,10 @@ template <typename F> void run_group(int nt, F &&body) {
   grp = &g;
   using FD = std::decay_t<F>;
   Entry<FD>::fn = &body;
-  while ((int)stack_pool.size() < nt) stack_pool.push_back((char *)malloc(kStack));
+  while ((int)stack_pool.v.size() < nt) stack_pool.v.push_back((char *)malloc(kStack));
   for (int i = 0; i < nt; i++) {
     Fiber &f = g.f[i];
-    f.stack = stack_pool[i];
+    f.stack = stack_pool.v[i];
     getcontext(&f.ctx);
     f.ctx.uc_stack.ss_sp = f.stack;
     f.ctx.uc_stack.ss_size = kStack;
