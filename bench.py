@@ -67,18 +67,19 @@ def measured_traffic(kernel_prefix):
     return None
 
 
-def model_bytes_step(n, b):
-    """5-pass pruned-pencil bytes per Heun step (2 RHS) for an n^3 grid."""
+def model_bytes_step(n, b, s=16):
+    """5-pass pruned-pencil bytes per Heun step (2 RHS) for an n^3 grid: pass buffers of
+    b bytes per complex value, state of s bytes (FP64 16; the FP32-bulk working state 8)."""
     kt = n // 3
     iktx, ikty = kt + 1, 2 * kt + 1
     nkt = iktx * ikty * ikty
     Z, Y = iktx * ikty * n, iktx * n * n
     S, Fi, Fo = 3, 6, 3
-    # pass 1 reads the FP64 state; the pass buffers are in the pass precision b
-    rhs = 16 * S * nkt + b * (2 * Fi * Z + 2 * Fi * Y + 2 * Fo * Y + 2 * Fo * Z)
+    # pass 1 reads the state; the pass buffers are in the pass precision b
+    rhs = s * S * nkt + b * (2 * Fi * Z + 2 * Fi * Y + 2 * Fo * Y + 2 * Fo * Z)
     # Heun state/RH traffic: predictor reads w^n, writes w* and RH; corrector reads w*, RH, writes
-    # w^{n+1}; the state is FP64, RH is kept in the pass precision
-    stage = S * nkt * (4 * 16 + 2 * b)
+    # w^{n+1}; RH is kept in the pass precision
+    stage = S * nkt * (4 * s + 2 * b)
     return 2 * rhs + stage
 
 
@@ -323,7 +324,7 @@ def main():
     mb = model_bytes_step(n, 16)
     model = {"gb_per_step": mb / 1e9, "achieved_gbs": mb * sps64 / 1e9, "frac": mb * sps64 / 1e9 / PEAK_HBM_GBS}
     if fp32:
-        mb32 = model_bytes_step(n, 8)
+        mb32 = model_bytes_step(n, 8, 8)
         fp32["model"] = {"gb_per_step": mb32 / 1e9, "achieved_gbs": mb32 * fp32["steps_per_s"] / 1e9,
                          "frac": mb32 * fp32["steps_per_s"] / 1e9 / PEAK_HBM_GBS}
     adj = None

@@ -284,8 +284,22 @@ static int build_graph(Ctx *c, int kind) {
   return 0;
 }
 
+static int run_steps_in(Ctx *c, int kind, int nsteps);
+
+// the FP32 working state (3D Navier-Stokes/Euler FP32-bulk) is converted once
+// around all of a call's steps, outside the captured graphs
 int run_steps(Ctx *c, int kind, int nsteps) {
   if (nsteps <= 0) return 0;
+  const bool work = kind == 0 && c->st32;
+  if (work)
+    if (int r0 = ns3d_work_begin(c)) return r0;
+  const int r = run_steps_in(c, kind, nsteps);
+  if (!work) return r;
+  const int r2 = ns3d_work_end(c);
+  return r ? r : r2;
+}
+
+static int run_steps_in(Ctx *c, int kind, int nsteps) {
   const int G = c->graph_steps;
   if (G <= 0 || c->prof || c->graph_failed || nsteps < G + 1) return run_eager(c, kind, nsteps, true);
   // the first step of a call is eager: it may mask the velocity / copy the stage input

@@ -237,6 +237,10 @@ static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, in
     }
     constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
+    if (nrows >= (1L << 31)) {
+      set_error("x-pass: %ld rows exceed the 32-bit row index", nrows);
+      return PSG_EINVAL;
+    }
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
     size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD>;

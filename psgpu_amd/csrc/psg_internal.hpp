@@ -48,6 +48,7 @@ struct Ctx {
   cpx<double> *stage = nullptr;     // RK4: Z0 stage input / generic scratch
   cpx<double> *tmp = nullptr;       // scratch state-sized buffer
   cpx<float> *st32 = nullptr;       // FP32-bulk 3D Navier-Stokes/Euler: working state of a run
+  bool st32_live = false;           // st32 holds the current state (inside run_steps)
   double *nuzn = nullptr, *nu1 = nullptr;  // CN tables (host-built, reference formula)
   double *nu = nullptr;                    // NU = 10^(log10 v2 + 2 log10|k|), for dz/dt
   int8_t *mask = nullptr;           // active mask L
@@ -102,6 +103,8 @@ void prof_end(Ctx *c);
 // per-flavour drivers
 int ns3d_init(Ctx *c);
 int ns3d_run(Ctx *c, int nsteps, bool first);
+int ns3d_work_begin(Ctx *c);
+int ns3d_work_end(Ctx *c);
 int ns3d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
 int ns3d_physical(Ctx *c, int member, double *out_dev);
 int ns2d_init(Ctx *c);

@@ -23,7 +23,7 @@ template <int PHASE, bool EULER, typename R = double, typename S = double> struc
   cpx<S> *st;
   cpx<R> *rh;
   long ms, fs;  // member stride, field stride (same for st and rh)
-  const double *nuzn, *nu1;
+  const double *nuzn, *nu1;   // FP64 always: rounded tables would bias the damping every step
   const int8_t *L;
   Force F;
   double dt;
@@ -190,30 +190,33 @@ template <typename P, typename S> static int run_t(Ctx *c, cpx<S> *st, int nstep
   // state + RH (pass precision) traffic per mode of each half-step
   const long sb = (6 * (long)sizeof(cpx<S>) + 3 * (long)sizeof(cpx<P>)) * g.nkt;
   cpx<P> *rh = (cpx<P> *)c->rh;
+  const double *nuzn = c->nuzn, *nu1 = c->nu1;
   for (int nt = 0; nt < nsteps; nt++) {
     const int umask = (first && nt == 0) ? 1 : 0;
     if (euler) {
-      UpdHeun<0, true, P, S> up{st, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      UpdHeun<0, true, P, S> up{st, rh, ms, fs, nuzn, nu1, c->mask, F, c->cfg.dt};
       PSG_TRY(rhs3<P>(c, st, ms, umask, up, sb, c->batch));
-      UpdHeun<1, true, P, S> uc{st, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      UpdHeun<1, true, P, S> uc{st, rh, ms, fs, nuzn, nu1, c->mask, F, c->cfg.dt};
       PSG_TRY(rhs3<P>(c, st, ms, 0, uc, sb, c->batch));
     } else {
-      UpdHeun<0, false, P, S> up{st, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      UpdHeun<0, false, P, S> up{st, rh, ms, fs, nuzn, nu1, c->mask, F, c->cfg.dt};
       PSG_TRY(rhs3<P>(c, st, ms, umask, up, sb, c->batch));
-      UpdHeun<1, false, P, S> uc{st, rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+      UpdHeun<1, false, P, S> uc{st, rh, ms, fs, nuzn, nu1, c->mask, F, c->cfg.dt};
       PSG_TRY(rhs3<P>(c, st, ms, 0, uc, sb, c->batch));
     }
   }
   return 0;
 }
 
-// element-wise precision change of n complex values
-template <typename A, typename B> __global__ void k_convert(const cpx<A> *__restrict__ a, cpx<B> *__restrict__ b, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    b[i] = cpx<B>{(B)a[i].x, (B)a[i].y};
+// element-wise precision change of n values (complex or real)
+template <typename A, typename B> __global__ void k_convert(const A *__restrict__ a, B *__restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    if constexpr (std::is_arithmetic<A>::value) b[i] = (B)a[i];
+    else b[i] = B{(decltype(b[i].x))a[i].x, (decltype(b[i].x))a[i].y};
+  }
 }
 
-template <typename A, typename B> static int convert(Ctx *c, const cpx<A> *a, cpx<B> *b, long n) {
+template <typename A, typename B> static int convert(Ctx *c, const A *a, B *b, long n) {
   hipLaunchKernelGGL((k_convert<A, B>), dim3((unsigned)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0,
                      c->stream, a, b, n);
   PSG_HIP(hipGetLastError());
@@ -222,13 +225,31 @@ template <typename A, typename B> static int convert(Ctx *c, const cpx<A> *a, cp
 
 // FP32-bulk steps a float copy of the state (st32): FP64 is kept for the
 // state between runs, where Newton's residual, the diagnostics and the API
-// read it; the run starts from the state rounded to FP32.
+// read it; a run starts from the state rounded to FP32.  Between
+// ns3d_work_begin and ns3d_work_end the copy is live and ns3d_run steps it
+// without converting (run_steps brackets a call's eager steps and graph
+// replays this way).
+int ns3d_work_begin(Ctx *c) {
+  if (!c->st32 || c->st32_live) return 0;
+  PSG_TRY((convert<cpx<double>, cpx<float>>(c, c->state, c->st32, 3 * c->g.nkt * c->batch)));
+  c->st32_live = true;
+  return 0;
+}
+
+int ns3d_work_end(Ctx *c) {
+  if (!c->st32_live) return 0;
+  c->st32_live = false;
+  return convert<cpx<float>, cpx<double>>(c, c->st32, c->state, 3 * c->g.nkt * c->batch);
+}
+
 int ns3d_run(Ctx *c, int nsteps, bool first) {
   if (c->cfg.precision != PSG_FP32BULK) return run_t<double, double>(c, c->state, nsteps, first);
-  const long n = 3 * c->g.nkt * c->batch;
-  PSG_TRY((convert<double, float>(c, c->state, c->st32, n)));
-  PSG_TRY((run_t<float, float>(c, c->st32, nsteps, first)));
-  return convert<float, double>(c, c->st32, c->state, n);
+  const bool own = !c->st32_live;
+  if (own) PSG_TRY(ns3d_work_begin(c));
+  const int r = run_t<float, float>(c, c->st32, nsteps, first);
+  if (!own) return r;
+  const int r2 = ns3d_work_end(c);
+  return r ? r : r2;
 }
 
 // One Navier-Stokes step of member 0 that records w^n (before the step) and

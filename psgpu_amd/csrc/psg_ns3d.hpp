@@ -79,6 +79,10 @@ static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int 
     constexpr int TE0 = (256 / TPX) < 1 ? 1 : 256 / TPX;
     constexpr int TE = (TEAMS_MAX > 0 && TEAMS_MAX < TE0) ? TEAMS_MAX : TE0;
     long nrows = nrow_m * nm;
+    if (nrows >= (1L << 31)) {
+      set_error("x-pass: %ld rows exceed the 32-bit row index", nrows);
+      return PSG_EINVAL;
+    }
     dim3 grid((unsigned)((nrows + TE - 1) / TE));
     size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD, E, FRONT, LAZY, WPE>;

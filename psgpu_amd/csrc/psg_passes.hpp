@@ -32,19 +32,25 @@ constexpr int EPT = 8;  // points per thread in every pencil transform
 // they are read at LDS latency and without bank conflicts.  The packed table
 // has fewer than N entries; the kernel's data region starts N entries in
 // (launches add N*sizeof(cpx<P>) to the LDS size).
+// packed entry i of the stages from NS on, whose entries start at BASE:
+// stage NS (radix R) holds w^(r k N/(NS R)) at BASE + k (R-1) + r-1
+template <typename P, int N, int E, int NS, int BASE>
+__device__ __forceinline__ void stage_twiddle(const cpx<P> *__restrict__ tw, cpx<P> *lds_tw, int i) {
+  if constexpr (NS < N) {
+    constexpr int R = radix_at(N, E, NS), CNT = NS * (R - 1);
+    if (i < BASE + CNT) {
+      const int q = i - BASE, k = q / (R - 1), r = q % (R - 1) + 1;
+      lds_tw[i] = tw[(r * k) * (N / (NS * R))];
+    } else {
+      stage_twiddle<P, N, E, NS * R, BASE + CNT>(tw, lds_tw, i);
+    }
+  }
+}
 template <typename P, int N, int E>
 __device__ __forceinline__ const cpx<P> *stage_twiddles(const cpx<P> *__restrict__ tw, cpx<P> *lds_tw) {
   constexpr int M = tw_count(N, E);
   static_assert(M < N, "packed twiddles fit the reserved N entries");
-  for (int i = threadIdx.x; i < M; i += blockDim.x) {
-    int ns = radix_at(N, E, 1), base = 0;
-    while (i >= base + ns * (radix_at(N, E, ns) - 1)) {
-      base += ns * (radix_at(N, E, ns) - 1);
-      ns *= radix_at(N, E, ns);
-    }
-    const int R = radix_at(N, E, ns), q = i - base, k = q / (R - 1), r = q % (R - 1) + 1;
-    lds_tw[i] = tw[(r * k) * (N / (ns * R))];
-  }
+  for (int i = threadIdx.x; i < M; i += blockDim.x) stage_twiddle<P, N, E, radix_at(N, E, 1), 0>(tw, lds_tw, i);
   __syncthreads();
   return lds_tw;
 }
@@ -108,7 +114,7 @@ __device__ __forceinline__ long yoff(const Grid &g, int ikx, int y, int z) {
 }
 // x row rr = z*ny + y of a Y buffer: element k sits at yrow(rr)[ycol(k)]
 __device__ __forceinline__ long yrow(const Grid &g, long rr) { return yoff(g, 0, (int)(rr % g.ny), (int)(rr / g.ny)); }
-__device__ __forceinline__ long ycol(const Grid &g, int k) { return (long)(k >> 3) * g.ny * LW + (k & (LW - 1)); }
+__device__ __forceinline__ int ycol(const Grid &g, int k) { return (k >> 3) * (g.ny * LW) + (k & (LW - 1)); }
 
 // Biot-Savart u = i k x w / max(|k|^2,1e-3): 3D_NavierStokes/src/kernels.h:187-201
 __device__ __forceinline__ void biot_savart(double kx, double ky, double kz, cpx<double> xi, cpx<double> et,
@@ -373,13 +379,21 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2;
   constexpr int NV = FRONT ? PI : 1;
   const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
-  const long row = (long)blockIdx.x * TEAMS + team;
-  const bool valid = row < nrows;
-  const long m = valid ? row / nrow_m : 0, rr = valid ? row % nrow_m : 0;
+  // row = (m, z, y) with y fastest: the block's first row is split once per
+  // block (uniform), each team steps y from it (launches keep nrows < 2^31)
+  const unsigned row0 = blockIdx.x * (unsigned)TEAMS, nrm = (unsigned)nrow_m, ny = (unsigned)g.ny;
+  const unsigned m0 = row0 / nrm, rr0 = row0 - m0 * nrm, z0 = rr0 / ny, nzr = nrm / ny;
+  unsigned y = rr0 - z0 * ny + team, z = z0, m = m0;
+  while (y >= ny) { y -= ny; z++; }
+  while (z >= nzr) { z -= nzr; m++; }
+  const bool valid = row0 + team < (unsigned long)nrows;
+  if (!valid) m = 0, z = 0, y = 0;
+  const long rr = (long)z * ny + y;
   cpx<P> *tl = lds + team * row_len<N>();
   constexpr int SW = row_swizzle<E>();
   constexpr int ix = xband<N>();
-  auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + yrow(g, rr); };
+  const long ro = yoff(g, 0, (int)y, (int)z);
+  auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + ro; };
   cpx<P> vin[NV][E];
   if constexpr (FRONT) {
 #pragma unroll
