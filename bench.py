@@ -127,14 +127,18 @@ def energy_drift(P, st0, n, nsteps=1000, every=100):
             "rel_err_max": max(rel)}
 
 
-def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-6):
+def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-4):
     """Gradient of J = ||Phi_T(u0) - u0||^2 at 128^3, 200 steps, checkpoints every 20,
-    against central differences along seeded solenoidal directions (FD with the
-    FP64 stepper); FP64 and FP32-bulk adjoints."""
+    against fourth-order central differences along seeded solenoidal directions (FD
+    with the FP64 stepper); FP64 and FP32-bulk adjoints.  With J ~ 10 and h = 1e-4 the
+    FD's round-off is ~1e-10 and its truncation error O(h^4): a second-order
+    difference at h = 1e-6 would carry ~5e-9 of round-off, 5e-6 relative on a
+    direction whose derivative is 1e-3."""
     g = P.hgrid("ns3d", n, n, n)
     L = P.tables(g)[3].astype(float)
     z0 = P.ic_solenoidal(g, 20240603, 0.5)
-    out = {"grid": [n, n, n], "steps": nsteps, "dt": 0.002, "ckpt": ckpt, "directions": ndir, "eps": eps}
+    out = {"grid": [n, n, n], "steps": nsteps, "dt": 0.002, "ckpt": ckpt, "directions": ndir, "eps": eps,
+           "fd_order": 4}
     with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov") as s:
         def J(x):
             s.set_state(x)
@@ -159,7 +163,7 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-6):
         for k in range(ndir):
             v = P.ic_solenoidal(g, 777000 + k, 0.5)
             v /= np.sqrt(np.sum(np.abs(v) ** 2))
-            fd = (J(z0 + eps * v) - J(z0 - eps * v)) / (2 * eps)
+            fd = (8 * (J(z0 + eps * v) - J(z0 - eps * v)) - (J(z0 + 2 * eps * v) - J(z0 - 2 * eps * v))) / (12 * eps)
             ip = lambda a: float(np.sum((np.conj(a) * v).real * L[None, :]))
             fds.append(fd)
             errs64.append(abs(ip(g64) - fd) / abs(fd))
@@ -179,9 +183,10 @@ def newton_case(P, n=512, t_spin=100.0, t_window=300.0, dt_sample=0.25, nguess=4
     random-phase state, take the closest shift-minimised near-recurrences with
     T in [tmin, tmax] as guesses, and run Newton-GMRES-hookstep (Krylov 40,
     residual 1e-10, at most max_newton iterations) on all guesses with
-    finite-difference Jv, then on the same guesses with tangent-linear Jv, one
-    guess after the other on one context; s/orbit = total time / converged,
-    and "paired" lists the guesses both forms converged with their times."""
+    finite-difference Jv, then on the same guesses with tangent-linear Jv in FP64
+    and in FP32-bulk passes (jv 0, 1, 2), one guess after the other on one
+    context; s/orbit = total time / converged, and "paired" lists the guesses
+    the finite difference and a tangent-linear form both converged, with times."""
     from psgpu_amd import recurrence as R
     g = P.hgrid("ns2d", n, n)
     L = P.tables(g)[3]
@@ -201,8 +206,9 @@ def newton_case(P, n=512, t_spin=100.0, t_window=300.0, dt_sample=0.25, nguess=4
         log("bench: newton guesses " + ", ".join("t=%.2f T=%.3f s=%.3f d=%.3f" % (h["t"], h["T"], h["shift"],
                                                                                   h["dist"]) for h in hits))
         runs = []
-        for jv in (0, 1):
-            name = "tangent-linear" if jv else "finite-difference"
+        names = {0: "finite-difference", 1: "tangent-linear", 2: "tangent-linear-fp32"}
+        for jv in (0, 1, 2):
+            name = names[jv]
             rs = []
             for k, h in enumerate(hits):
                 z, res = s.newton(snaps[h["i"]][None], T=h["T"], sx=h["shift"], jv=jv, netol=1e-10, max_gmres=40,
@@ -219,11 +225,15 @@ def newton_case(P, n=512, t_spin=100.0, t_window=300.0, dt_sample=0.25, nguess=4
                          "s_per_orbit": total / len(conv) if conv else None,
                          "newton_iters": [r["newton_iters"] for r in rs], "status": [r["status_name"] for r in rs]}
         out["runs"] = runs
-        fd = {r["guess"]: r for r in runs if r["jv"] == "finite-difference" and r["status_name"] == "CONVERGED"}
-        tl = {r["guess"]: r for r in runs if r["jv"] == "tangent-linear" and r["status_name"] == "CONVERGED"}
-        out["paired"] = [{"guess": k, "T": fd[k]["T"], "fd_seconds": fd[k]["seconds"], "tl_seconds": tl[k]["seconds"],
-                          "fd_iters": fd[k]["newton_iters"], "tl_iters": tl[k]["newton_iters"]}
-                         for k in sorted(set(fd) & set(tl))]
+        conv = lambda nm: {r["guess"]: r for r in runs if r["jv"] == nm and r["status_name"] == "CONVERGED"}
+        fd = conv("finite-difference")
+        out["paired"] = []
+        for nm in ("tangent-linear", "tangent-linear-fp32"):
+            tl = conv(nm)
+            out["paired"] += [{"guess": k, "T": fd[k]["T"], "jv": nm, "fd_seconds": fd[k]["seconds"],
+                               "tl_seconds": tl[k]["seconds"], "fd_iters": fd[k]["newton_iters"],
+                               "tl_iters": tl[k]["newton_iters"], "fd_phi_evals": fd[k]["phi_evals"],
+                               "tl_phi_evals": tl[k]["phi_evals"]} for k in sorted(set(fd) & set(tl))]
     return out
 
 

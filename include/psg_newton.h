@@ -14,7 +14,9 @@
  * Documented deviations: the Krylov dimension is capped (max_gmres) and at
  * the cap the hookstep is taken in the space built so far instead of
  * failing; Jv may come from the tangent-linear stepper (jv = 1) instead of
- * the finite difference (jv = 0, the reference's epsilon = 1e-7).
+ * the finite difference (jv = 0, the reference's epsilon = 1e-7), in FP64
+ * or FP32-bulk passes (jv = 2: the FP64 residual then acts as iterative
+ * refinement of the FP32 Jacobian).
  */
 #ifndef PSG_NEWTON_H
 #define PSG_NEWTON_H
@@ -46,7 +48,8 @@ typedef struct {
   double delta0;      /* initial trust region GMRESdelta 0.1 */
   int nshift;         /* 1: sx unknown; 2: sx and sz unknown (Boussinesq) */
   int fixed_period;   /* FixedP: T not an unknown */
-  int jv;             /* 0 finite difference, 1 tangent linear */
+  int jv;             /* 0 finite difference, 1 tangent linear, 2 tangent linear with FP32-bulk
+                         passes (the residual flow keeps the context's precision) */
   int time_policy;    /* PSG_TIME_* */
   double dt0;         /* TRUEDELT */
   double sy;          /* fixed y shift SNEWTY */

@@ -63,6 +63,7 @@ struct Ctx {
   // its stage input and stage increments, and pass buffers sized for the
   // combined base + perturbation transforms
   cpx<double> *tl_state = nullptr, *tl_stage = nullptr, *tl_rh = nullptr;
+  bool tl_fp32 = false;             // tangent-linear passes in FP32-bulk in an FP64 context
   void *tl_bufA = nullptr, *tl_bufB = nullptr, *tl_bufC = nullptr;
   // captured step graphs: [0] plain stepper, [1] tangent linear; rebuilt when
   // graph_gen changes (dt, variants, graph length)

@@ -296,10 +296,30 @@ int psg_tangent(psg_ctx *c, const double *z, const double *dz0, double T, double
   return r;
 }
 
+// jv = 2: the tangent-linear flows of the solve run their passes in FP32-bulk
+// (the residual flow stays in the context's precision); captured step graphs
+// are rebuilt on each switch
+struct TlPrecision {
+  Ctx *c;
+  bool old;
+  TlPrecision(Ctx *ctx, bool fp32) : c(ctx), old(ctx->tl_fp32) {
+    if (fp32 != old) { c->tl_fp32 = fp32; c->graph_gen++; }
+  }
+  ~TlPrecision() {
+    if (c->tl_fp32 != old) { c->tl_fp32 = old; c->graph_gen++; }
+  }
+};
+
 int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szio, const psg_newton_opts *o,
                      psg_newton_result *res) {
   if (!c || !zh || !Tio || !o || !res) { set_error("null argument"); return PSG_EINVAL; }
   if (c->batch != 1) { set_error("psg_newton_solve needs a batch-1 context"); return PSG_EINVAL; }
+  if (o->jv < 0 || o->jv > 2) { set_error("newton: jv %d is not 0, 1 or 2", o->jv); return PSG_EINVAL; }
+  if (o->jv == 2 && c->cfg.flavour != PSG_NS2D && c->cfg.flavour != PSG_NS3D) {
+    set_error("newton: tangent-linear Jv is 2D/3D Navier-Stokes only");
+    return PSG_EINVAL;
+  }
+  TlPrecision tlp(c, o->jv == 2);
   const auto t_start = std::chrono::steady_clock::now();
   memset(res, 0, sizeof(*res));
   VecOps V(c);
@@ -392,7 +412,7 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
     // J applied to (dz, dT, ds): rows 1..N into out, scalar rows into vn
     auto apply_J = [&](const cpx<double> *dz, const double *dsc, cpx<double> *out, double *vn) -> int {
       int r;
-      if (o->jv == 1) {
+      if (o->jv >= 1) {
         if ((r = tl_flow_map(c, ZNEWT, dz, work, out, T, dt0, o->time_policy, true))) return r;
         shift_of(out, out, sh[0], sh[1]);   // S dPhi dz
       } else {

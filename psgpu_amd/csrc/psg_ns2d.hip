@@ -81,9 +81,9 @@ template <typename P> static int launch_yinv2(Ctx *c, const cpx<double> *st, cpx
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
-    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    dim3 grid((g.iktx + T - 1) / T, 3, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
-    auto k = k2_yinv<P, N, T>;
+    auto k = k2_yinv<P, N, T, 1, true>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, g.nkt, out, 3 * g.ysz, g,
                        twp<P>(c, 1), c->mask, (const cpx<double> *)nullptr);
@@ -114,18 +114,25 @@ static int launch_xpass2(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int
       set_error("x-pass: IKTX %d is not NX/3+1", g.iktx);
       return PSG_EINVAL;
     }
-    constexpr int TE = Row<N>::TEAMS;
     long nrows = nrow_m * nm;
     if (nrows >= (1L << 31)) {
       set_error("x-pass: %ld rows exceed the 32-bit row index", nrows);
       return PSG_EINVAL;
     }
-    dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
-    auto k = k_xpass<P, N, TE, FI, FO, PROD>;
-    PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
-                       prod);
+    // fewer rows than about 8 per CU: one row per block, so that the rows
+    // spread over the CUs instead of filling a few blocks
+    auto go = [&](auto te) {
+      constexpr int TE = decltype(te)::value;
+      dim3 grid((unsigned)((nrows + TE - 1) / TE));
+      size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
+      auto k = k_xpass<P, N, TE, FI, FO, PROD>;
+      PSG_HIP(lds_attr(k, lds));
+      hipLaunchKernelGGL(k, grid, dim3(TE * Row<N>::TPX), lds, c->stream, in, out, nrow_m, nrows, g,
+                         twp<P>(c, 0), prod);
+      return 0;
+    };
+    int rl = nrows < 2048 ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, Row<N>::TEAMS>{});
+    if (rl) return rl;
   });
   PSG_HIP(hipGetLastError());
   return 0;
@@ -256,13 +263,23 @@ template <int STAGE> struct Upd2DTL {
   cpx<double> *dz, *dz0, *dzs;
   __device__ __forceinline__ void operator()(int m, long ik, int i0, double kx, double ky,
                                              cpx<double> (&N)[4]) const {
-    cpx<double> nb[2] = {N[0], N[1]};
+    cpx<double> nb[2] = {N[0], N[1]}, np[2] = {N[2], N[3]};
     base(m, ik, i0, kx, ky, nb);
+    pert(m, ik, kx, ky, np);
+  }
+  // group 0: the base update from N[0..1]; group 1: the perturbation's from N[2..3]
+  __device__ __forceinline__ void part(int grp, int m, long ik, int i0, double kx, double ky,
+                                       cpx<double> (&N)[2]) const {
+    if (grp == 0) base(m, ik, i0, kx, ky, N);
+    else pert(m, ik, kx, ky, N);
+  }
+  // N = {FFT(dv w + v dw), FFT(du w + u dw)} / NR
+  __device__ __forceinline__ void pert(int m, long ik, double kx, double ky, cpx<double> (&N)[2]) const {
     const long nkt = base.nkt;
     cpx<double> *Z = dz + (long)m * nkt + ik;
     if (base.L[ik] == 1) {
-      const double NX = kx * N[3].y + ky * N[2].y;
-      const double NY = -kx * N[3].x - ky * N[2].x;
+      const double NX = kx * N[1].y + ky * N[0].y;
+      const double NY = -kx * N[1].x - ky * N[0].x;
       const double dt = base.dt;
       const double ZX = dt * NX, ZY = dt * NY;
       cpx<double> *Z0 = dz0 + (long)m * nkt + ik;
@@ -296,9 +313,9 @@ template <typename P, class UPD> static int rhs2_tl(Ctx *c, UPD upd, int nm) {
   cpx<P> *B = (cpx<P> *)c->tl_bufB, *Cb = (cpx<P> *)c->tl_bufC;
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
-    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    dim3 grid((g.iktx + T - 1) / T, 6, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
-    auto k = k2_yinv<P, N, T, 2>;
+    auto k = k2_yinv<P, N, T, 2, true>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, c->stage, g.nkt, B, 6 * g.ysz, g,
                        twp<P>(c, 1), c->mask, c->tl_stage);
@@ -307,9 +324,9 @@ template <typename P, class UPD> static int rhs2_tl(Ctx *c, UPD upd, int nm) {
   PSG_TRY2((launch_xpass2<P, 6, 4>(c, B, Cb, g.ny, nm, Prod2DTL<P>{})));
   PSG_DISPATCH_N(g.ny, {
     constexpr int T = Col<N>::T;
-    dim3 grid((g.iktx + T - 1) / T, 1, nm);
+    dim3 grid((g.iktx + T - 1) / T, 2, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
-    auto k = k2_yfwd<P, N, T, UPD, 4>;
+    auto k = k2_yfwd<P, N, T, UPD, 4, 2>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, Cb, 4 * g.ysz, g, twp<P>(c, 1), upd);
   });
@@ -342,7 +359,8 @@ template <typename P> static int run2tl_t(Ctx *c, int nsteps, bool first) {
 }
 
 int ns2d_tl_run(Ctx *c, int nsteps, bool first) {
-  return c->cfg.precision == PSG_FP32BULK ? run2tl_t<float>(c, nsteps, first) : run2tl_t<double>(c, nsteps, first);
+  return (c->cfg.precision == PSG_FP32BULK || c->tl_fp32) ? run2tl_t<float>(c, nsteps, first)
+                                                          : run2tl_t<double>(c, nsteps, first);
 }
 
 int ns2d_init(Ctx *c) {

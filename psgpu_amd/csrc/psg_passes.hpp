@@ -508,7 +508,9 @@ template <typename P> struct Prod2D {
 // y-inverse straight from the state: Hermitian fill of kx=0 (in place in the
 // reference, 2D_NavierStokes/src_RK4/kernels.h:82-96), mask, Biot-Savart
 // (kernels.h:112-147), zero-pad, inverse y-FFT.  out fields w,u,v.
-template <typename P, int N, int T, int NSRC = 1>
+// SPLIT: blockIdx.y picks one of the 3*NSRC output fields (3*NSRC times the
+// blocks, for grids too small to fill the GPU with column tiles alone)
+template <typename P, int N, int T, int NSRC = 1, bool SPLIT = false>
 __global__ void __launch_bounds__(T *(N / EPT))
 k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out, long out_ms, Grid g,
         const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask,
@@ -517,11 +519,12 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
-  const int ikx = tid.x * T + col, m = tid.z;
+  const int ikx = tid.x * T + col, m = tid.z, sel = tid.y;
   const bool valid = ikx < g.iktx;
   const double kx = (double)ikx * g.alpha;
 #pragma unroll 1
   for (int src = 0; src < NSRC; src++) {
+    if (SPLIT && src != sel / 3) continue;
     const cpx<double> *s0 = (src == 0 ? st : st2) + (long)m * st_ms;
     cpx<double> w[EPT];
     double kyv[EPT];
@@ -544,6 +547,7 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) {
+      if (SPLIT && f != sel % 3) continue;
       cpx<P> v[EPT];
 #pragma unroll
       for (int mm = 0; mm < EPT; mm++) {
@@ -570,19 +574,21 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
 
 // y-forward + update (2D).  UPD::operator()(m, ik, ikx, kx, ky, N[2]) with
 // N[0] = FFT(v w)/NR, N[1] = FFT(u w)/NR  (the reference's NZK halves).
-template <typename P, int N, int T, class UPD, int NF = 2>
+// NG > 1: the NF fields form NG independent groups, blockIdx.y picks one and
+// its update is UPD::part(group, m, ik, ikx, kx, ky, N[NF/NG]).
+template <typename P, int N, int T, class UPD, int NF = 2, int NG = 1>
 __global__ void __launch_bounds__(T *(N / EPT))
 k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
   PSG_LDS_TWIDDLES(P, N, EPT);
-  constexpr int TP = N / EPT;
+  constexpr int TP = N / EPT, NFG = NF / NG;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
-  const int ikx = tid.x * T + col, m = tid.z;
+  const int ikx = tid.x * T + col, m = tid.z, grp = NG > 1 ? tid.y : 0;
   const bool valid = ikx < g.iktx;
-  cpx<double> res[NF][EPT];
+  cpx<double> res[NFG][EPT];
 #pragma unroll
-  for (int f = 0; f < NF; f++) {
-    const cpx<P> *src = in + (long)m * in_ms + (long)f * g.ysz + ikx;
+  for (int f = 0; f < NFG; f++) {
+    const cpx<P> *src = in + (long)m * in_ms + (long)(grp * NFG + f) * g.ysz + ikx;
     cpx<P> v[EPT];
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[(long)(j + TP * mm) * g.xp] : cpx<P>{0, 0};
@@ -600,10 +606,11 @@ k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restr
     const int ky = wrapk(j + TP * mm, g.ny, g.kty);
     if (ky == PSG_NOK) continue;
     const long ik = (long)(ky + g.kty) * g.iktx + ikx;
-    cpx<double> nn[NF];
+    cpx<double> nn[NFG];
 #pragma unroll
-    for (int f = 0; f < NF; f++) nn[f] = res[f][mm];
-    upd(m, ik, ikx, kx, (double)ky, nn);
+    for (int f = 0; f < NFG; f++) nn[f] = res[f][mm];
+    if constexpr (NG > 1) upd.part(grp, m, ik, ikx, kx, (double)ky, nn);
+    else upd(m, ik, ikx, kx, (double)ky, nn);
   }
 }
 

@@ -218,7 +218,8 @@ int ns3d_tl_run(Ctx *c, int nsteps, bool first) {
     return PSG_EINVAL;
   }
   if (int r = tl_alloc(c)) return r;
-  return c->cfg.precision == PSG_FP32BULK ? tl_run_t<float>(c, nsteps, first) : tl_run_t<double>(c, nsteps, first);
+  return (c->cfg.precision == PSG_FP32BULK || c->tl_fp32) ? tl_run_t<float>(c, nsteps, first)
+                                                          : tl_run_t<double>(c, nsteps, first);
 }
 
 // one application of C'(w)^T to the zinv input v (scaled by fac*tab), ending in upd

@@ -38,7 +38,7 @@ def test_tl_2d_matches_oracle_central_fd(gpu):
     assert np.abs(dz2 - 3.0 * dz1).max() < 1e-12 * np.abs(dz2).max()
 
 
-@pytest.mark.parametrize("jv", [0, 1])
+@pytest.mark.parametrize("jv", [0, 1, 2])
 def test_newton_converges_to_laminar_equilibrium(gpu, jv):
     # below the first instability the Kolmogorov laminar state is the unique
     # nearby fixed point of Phi_T: omega(0,4) = -Re/8 (2D forcing -2 at (0,4))
