@@ -27,7 +27,7 @@ FP32-bulk) and, with --newton K, "newton_2d" (2D Kolmogorov 512^2 Newton-GMRES-
 hookstep from K recurrence guesses, finite-difference vs tangent-linear Jv).
 
     python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-fp32] [--no-tune]
-                    [--no-adjoint] [--no-bous] [--newton K]
+                    [--no-adjoint] [--no-bous] [--no-energy] [--newton K]
 """
 import argparse
 import json
@@ -293,6 +293,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel shapes (no autotune)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^3 adjoint-gradient case")
     ap.add_argument("--no-bous", action="store_true", help="skip the 512^3 Boussinesq ensemble case")
+    ap.add_argument("--no-energy", action="store_true",
+                    help="skip the 1000-step FP32-bulk energy series (profiled runs)")
     ap.add_argument("--newton", type=int, default=0, metavar="NGUESS",
                     help="run the 2D 512^2 Newton case (TL vs FD Jv) on NGUESS recurrence guesses")
     ap.add_argument("--newton-n", type=int, default=512)
@@ -317,9 +319,10 @@ def main():
                 "energy_rel_err_vs_fp64_gpu": abs(e32 - e64) / e64, "steps": a.steps, "pass_variants": var32,
                 "kernels": prof32}
         log("bench: fp32-bulk %.2f steps/s (x%.2f)" % (sps32, sps32 / sps64))
-        fp32["energy_1000"] = energy_drift(P, st0, n)
-        log("bench: fp32-bulk energy vs fp64 after 1000 steps: %.2e (max over the run %.2e)" %
-            (fp32["energy_1000"]["rel_err_final"], fp32["energy_1000"]["rel_err_max"]))
+        if not a.no_energy:
+            fp32["energy_1000"] = energy_drift(P, st0, n)
+            log("bench: fp32-bulk energy vs fp64 after 1000 steps: %.2e (max over the run %.2e)" %
+                (fp32["energy_1000"]["rel_err_final"], fp32["energy_1000"]["rel_err_max"]))
     # dominant kernel of the FP64 step
     dom = max(prof64, key=lambda r: r["avg_ms"] * r["count"]) if prof64 else None
     roof = None
