@@ -22,9 +22,10 @@
  * on the command line.  Arc-length continuation, Arnoldi/Floquet analysis and
  * the other infile types are not part of this path and are rejected.
  *
- * usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl] [--maxnewton K] [--noshift]
- * (--jv tl: tangent-linear Jv; --noshift: the x shift is not an unknown, for
- * equilibria and other shift-free solutions)
+ * usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift]
+ * (--jv tl: tangent-linear Jv; tl32: the same in FP32-bulk passes under the
+ * FP64 residual; --noshift: the x shift is not an unknown, for equilibria and
+ * other shift-free solutions)
  */
 #include <math.h>
 #include <stdio.h>
@@ -60,7 +61,7 @@ static int die(const char *what) {
 
 int main(int argc, char **argv) {
   if (argc < 3)
-    return die("usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl] [--maxnewton K] [--noshift]");
+    return die("usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift]");
   const int is2d = strcmp(argv[1], "ns2d") == 0;
   if (!is2d && strcmp(argv[1], "ns3d") != 0) return die("flavour must be ns2d or ns3d");
   int nx = atoi(argv[2]), ny = nx, nz = nx, ai = 3, jv = 0, maxnewton = -1, noshift = 0;
@@ -69,7 +70,11 @@ int main(int argc, char **argv) {
   if (argc > 5 && argv[3][0] != '-') { ny = atoi(argv[3]); nz = atoi(argv[4]); alpha = atof(argv[5]); ai = 6; }
   for (int i = ai; i < argc; i++) {
     if (!strcmp(argv[i], "--dir") && i + 1 < argc) dir = argv[++i];
-    else if (!strcmp(argv[i], "--jv") && i + 1 < argc) jv = !strcmp(argv[++i], "tl");
+    else if (!strcmp(argv[i], "--jv") && i + 1 < argc) {
+      const char *f = argv[++i];
+      jv = !strcmp(f, "tl") ? 1 : (!strcmp(f, "tl32") ? 2 : 0);
+      if (jv == 0 && strcmp(f, "fd")) return die("--jv must be fd, tl or tl32");
+    }
     else if (!strcmp(argv[i], "--maxnewton") && i + 1 < argc) maxnewton = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--noshift")) noshift = 1;
     else return die("unknown option");

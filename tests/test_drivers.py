@@ -86,6 +86,8 @@ def test_driver_built_and_usage():
 def test_driver_rejects_unsupported_inputs(tmp_path):
     d = str(tmp_path)
     g = P.hgrid("ns2d", 32, 32)
+    r = run(["ns2d", "32", "--jv", "exact"], d)
+    assert r.returncode == 1 and "--jv" in r.stderr
     setup_case(d, np.zeros(g.nkt, complex), main=main_2d(**{"infile type": "A"}))
     r = run(["ns2d", "32"], d)
     assert r.returncode == 1 and "infile type D" in r.stderr
@@ -98,7 +100,8 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
 
 
 @pytest.mark.gpu
-def test_driver_converges_laminar_state_and_writes_outputs(gpu, tmp_path):
+@pytest.mark.parametrize("jv", ["fd", "tl32"])
+def test_driver_converges_laminar_state_and_writes_outputs(gpu, tmp_path, jv):
     d = str(tmp_path)
     g = P.hgrid("ns2d", 32, 32)
     kx, ky, _, L = P.tables(g)
@@ -106,7 +109,7 @@ def test_driver_converges_laminar_state_and_writes_outputs(gpu, tmp_path):
     lam[(kx == 0) & (ky == 4)] = -5.0 / 8          # laminar Kolmogorov vorticity at Re = 5
     z0 = lam + 1e-3 * P.ic_ns2d_cfg(g, 9, 1.0)[0]
     setup_case(d, z0)
-    r = run(["ns2d", "32", "--noshift", "--maxnewton", "8"], d)
+    r = run(["ns2d", "32", "--noshift", "--maxnewton", "8", "--jv", jv], d)
     assert r.returncode == 0, r.stdout + r.stderr
     info = open(os.path.join(d, "UPOinfo.dat")).read().splitlines()
     assert len(info) == 2 and '"CONVERGED"' in info[1]
