@@ -20,11 +20,12 @@ the FP64 C oracle (test infrastructure) timed on this host's cores on a
 bounded sample of the same workload.
 
 Further cases on the same line: "adjoint_128" (gradient of ||Phi_T(u0)-u0||^2
-at 128^3 over 200 steps with checkpoints every 20, error against central
-differences along 16 seeded solenoidal directions, FP64 and FP32-bulk),
+at 128^3 over 200 steps with checkpoints every 20, error against fourth-order
+central differences along 16 seeded solenoidal directions, FP64 and FP32-bulk),
 "bous_ensemble_512" (3D Boussinesq 512^3, 4 members stepped together, FP64 and
-FP32-bulk) and, with --newton K, "newton_2d" (2D Kolmogorov 512^2 Newton-GMRES-
-hookstep from K recurrence guesses, finite-difference vs tangent-linear Jv).
+FP32-bulk) and "newton_2d" (2D Kolmogorov 512^2 Newton-GMRES-hookstep from the
+closest K recurrence guesses, K = 1 by default and 0 to skip: seconds per
+converged orbit with finite-difference, tangent-linear and FP32 tangent-linear Jv).
 
     python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-fp32] [--no-tune]
                     [--no-adjoint] [--no-bous] [--no-energy] [--newton K]
@@ -295,8 +296,8 @@ def main():
     ap.add_argument("--no-bous", action="store_true", help="skip the 512^3 Boussinesq ensemble case")
     ap.add_argument("--no-energy", action="store_true",
                     help="skip the 1000-step FP32-bulk energy series (profiled runs)")
-    ap.add_argument("--newton", type=int, default=0, metavar="NGUESS",
-                    help="run the 2D 512^2 Newton case (TL vs FD Jv) on NGUESS recurrence guesses")
+    ap.add_argument("--newton", type=int, default=1, metavar="NGUESS",
+                    help="2D 512^2 Newton case (FD vs TL Jv) on NGUESS recurrence guesses (0: skip)")
     ap.add_argument("--newton-n", type=int, default=512)
     a = ap.parse_args()
     if a.gpus != 1:
