@@ -81,3 +81,36 @@ def best_recurrences(times, snaps, iktx, L, tmin, tmax, top=8, M=512, separation
         if len(out) == top:
             break
     return out
+
+
+# UPO_ZK.out: the reference's guess file, a C stream of records, each 7 doubles
+# followed by the state (2D_NavierStokes/src_RK4/functions.h:294-301:
+# time, v2, period, shift x, shift y, residual, time under threshold, Z(nkt);
+# 3D_NavierStokes/src/functions.h:368-377: time, v2, period, shift x, y, z,
+# residual, XII, ETA, ZET(nkt each)).  Complex values are (re, im) doubles.
+UPOZK_SCALARS = 7
+
+
+def write_upozk(path, hits, snaps, v2, append=False):
+    """Write 2D recurrence hits (best_recurrences) as UPO_ZK.out records: the
+    state at the start of the near-return, its time, the period and x shift,
+    y shift 0, the relative distance as the residual and 0 for the time under
+    threshold (the search has no threshold)."""
+    with open(path, "ab" if append else "wb") as f:
+        for h in hits:
+            sc = np.array([h["t"], v2, h["T"], h["shift"], 0.0, h["dist"], 0.0], np.float64)
+            f.write(sc.tobytes())
+            f.write(np.ascontiguousarray(snaps[h["i"]], np.complex128).tobytes())
+
+
+def read_upozk(path, nkt, nfields=1):
+    """Records of a UPO_ZK.out file with nfields states of nkt modes each (1 in 2D,
+    3 in 3D): a list of (scalars[7], state[nfields, nkt] complex)."""
+    rec = UPOZK_SCALARS + 2 * nfields * nkt
+    raw = np.fromfile(path, np.float64)
+    if raw.size % rec:
+        raise ValueError("%s: %d doubles is not a whole number of %d-double records" % (path, raw.size, rec))
+    out = []
+    for r in raw.reshape(-1, rec):
+        out.append((r[:UPOZK_SCALARS].copy(), r[UPOZK_SCALARS:].view(np.complex128).reshape(nfields, nkt)))
+    return out
