@@ -119,20 +119,24 @@ static int launch_xpass2(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int
       set_error("x-pass: %ld rows exceed the 32-bit row index", nrows);
       return PSG_EINVAL;
     }
-    // fewer rows than about 8 per CU: one row per block, so that the rows
-    // spread over the CUs instead of filling a few blocks
-    auto go = [&](auto te) {
-      constexpr int TE = decltype(te)::value;
+    if (nrows < 2048) {
+      // few rows: one row per block, its field pairs transformed side by side
+      constexpr int NT = xsplit_teams<FI, FO>() * Row<N>::TPX;
+      static_assert(NT <= 1024, "split-row x-pass block size");
+      const size_t lds = xsplit_lds<P, N, FI, FO>();
+      auto k = k_xpass_split<P, N, FI, FO, PROD>;
+      PSG_HIP(lds_attr(k, lds));
+      hipLaunchKernelGGL(k, dim3((unsigned)nrows), dim3(NT), lds, c->stream, in, out, nrow_m, nrows, g,
+                         twp<P>(c, 0), prod);
+    } else {
+      constexpr int TE = Row<N>::TEAMS;
       dim3 grid((unsigned)((nrows + TE - 1) / TE));
       size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
       auto k = k_xpass<P, N, TE, FI, FO, PROD>;
       PSG_HIP(lds_attr(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(TE * Row<N>::TPX), lds, c->stream, in, out, nrow_m, nrows, g,
+      hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g,
                          twp<P>(c, 0), prod);
-      return 0;
-    };
-    int rl = nrows < 2048 ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, Row<N>::TEAMS>{});
-    if (rl) return rl;
+    }
   });
   PSG_HIP(hipGetLastError());
   return 0;

@@ -472,6 +472,89 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   }
 }
 
+// Split-row x-pass for grids with few rows (2D): one row per block and one
+// team per field pair, so that a row's transforms run side by side instead of
+// one after another in a single team; the kernel time is then about one
+// inverse and one forward transform.  Phase 1: team p inverse-transforms
+// input pair p and leaves the physical values in LDS (phys[f][n]).  Phase 2:
+// team q forms the product at its points and forward-transforms output pair
+// q.  Every team runs both phases (on zeros where it has no pair) so that the
+// synchronisation is the same in every thread.
+template <int FI, int FO> constexpr int xsplit_teams() { return (FI + 1) / 2 > (FO + 1) / 2 ? (FI + 1) / 2 : (FO + 1) / 2; }
+template <typename P, int N, int FI, int FO>
+constexpr size_t xsplit_lds() {
+  return (size_t)(xsplit_teams<FI, FO>() * row_len<N>() + N) * sizeof(cpx<P>) +
+         (size_t)2 * ((FI + 1) / 2) * N * sizeof(P);
+}
+
+template <typename P, int N, int FI, int FO, class PROD, int E = EPT>
+__global__ void __launch_bounds__((xsplit_teams<FI, FO>() * (N / E)))
+k_xpass_split(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
+              const cpx<P> *__restrict__ tw, PROD prod) {
+  PSG_LDS_TWIDDLES(P, N, E);
+  constexpr int TPX = N / E;
+  constexpr bool WAVE = TPX <= 64;
+  constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2, NTEAM = xsplit_teams<FI, FO>();
+  constexpr int SW = row_swizzle<E>();
+  constexpr int ix = xband<N>();
+  const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
+  const unsigned row = blockIdx.x, nrm = (unsigned)nrow_m, ny = (unsigned)g.ny;
+  const bool valid = row < (unsigned long)nrows;
+  const unsigned m = valid ? row / nrm : 0, rr = valid ? row - m * nrm : 0, z = rr / ny, y = rr - z * ny;
+  cpx<P> *tl = lds + team * row_len<N>();
+  P *phys = reinterpret_cast<P *>(lds + NTEAM * row_len<N>());
+  const long ro = yoff(g, 0, (int)y, (int)z);
+  auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + ro; };
+  {
+    const int p = team < PI ? team : PI - 1, fa = 2 * p, fb = 2 * p + 1;
+    cpx<P> v[E];
+    XRow<P, N, E>::load(v, rowp(fa), rowp(fb < FI ? fb : fa), fb < FI, valid && team < PI, j, g);
+    fft_pencil<P, N, E, true, WAVE, SW>(v, tl, j, 1, 0, tw);
+    if (team < PI) {
+#pragma unroll
+      for (int mm = 0; mm < E; mm++) {
+        const int n = j + TPX * mm;
+        phys[fa * N + n] = v[mm].x;
+        phys[fb * N + n] = v[mm].y;
+      }
+    }
+  }
+  __syncthreads();
+  const int q = team < PO ? team : PO - 1;
+  cpx<P> v[E];
+#pragma unroll
+  for (int mm = 0; mm < E; mm++) {
+    const int n = j + TPX * mm;
+    P pin[2 * PI], pout[2 * PO];
+#pragma unroll
+    for (int f = 0; f < 2 * PI; f++) pin[f] = phys[f * N + n];
+#pragma unroll
+    for (int f = 0; f < 2 * PO; f++) pout[f] = 0;
+    prod(pin, pout);
+    v[mm] = cpx<P>{pout[2 * q], pout[2 * q + 1]};
+  }
+  fft_pencil<P, N, E, false, WAVE, SW>(v, tl, j, 1, 0, tw);
+  team_sync<WAVE>();
+#pragma unroll
+  for (int mm = 0; mm < E; mm++) {
+    const int k = j + TPX * mm;   // only the kept modes and their mirrors are read back
+    if (k < ix || k > N - ix) tl[lpos<SW>(k)] = v[mm];
+  }
+  team_sync<WAVE>();
+  if (valid && team < PO) {
+    cpx<P> *oa = out + ((long)m * FO + 2 * q) * g.ysz + (long)rr * g.xp;
+    cpx<P> *ob = out + ((long)m * FO + 2 * q + 1) * g.ysz + (long)rr * g.xp;
+    for (int k = j; k < ix; k += TPX) {
+      cpx<P> zk = tl[lpos<SW>(k)], zc = tl[lpos<SW>((N - k) & (N - 1))];
+      const P h = (P)0.5;
+      cpx<P> A{h * (zk.x + zc.x), h * (zk.y - zc.y)};
+      cpx<P> B{h * (zk.y + zc.y), h * (zc.x - zk.x)};
+      oa[k] = A;
+      if (2 * q + 1 < FO) ob[k] = B;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- products
 // 3D rotational form: (U,V,W) <- -(u x w)  (3D_NavierStokes/src/kernels.h:281-306)
 // inputs: 0..2 vorticity, 3..5 velocity
