@@ -200,6 +200,13 @@ int main(int argc, char **argv) {
   omp_set_dynamic(0);
   omp_set_max_active_levels(1);
   const bool quick = argc > 1 && strcmp(argv[1], "quick") == 0;
+  if (argc > 1 && strcmp(argv[1], "big2d") == 0) {
+    // 2D grids whose x-rows need more than one wavefront per team (N/8 > 64):
+    // the split-row x-pass then synchronises its teams with block barriers
+    ns2d_case("ns2d 1024^2 fp64", 1024, PSG_FP64, 1, 0, 1);
+    printf("%d passed, %d failed\n", g_pass, g_fail);
+    return g_fail ? 1 : 0;
+  }
   if (argc > 1 && strcmp(argv[1], "fp32") == 0) {
     // FP32-bulk forms only, at 16^3
     const int vs[] = {0, 1, 3, 8, 10};

@@ -124,6 +124,23 @@ def test_ns2d_parity_256(gpu):
         assert rel(s.get_state()[0], cur) < 1e-11
 
 
+@pytest.mark.parametrize("batch", [1, 2])
+def test_ns2d_large_grid_x_pass_forms(gpu, batch):
+    # 1024^2, one member: 1024 x-rows, the split-row x-pass with 128-thread
+    # teams (block barriers); two members: 2048 rows, the multi-row x-pass
+    n = 1024
+    g = O.grid2(n, n)
+    z0 = np.stack([O.ic_ns2d_cfg(g, 4711 + m, 1.0) for m in range(batch)])
+    orc = O.NS2D(n, v2=1 / 40, dt=0.005)
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / 40, dt=0.005, force="kolmogorov", kf=4.0, ampfor=-2.0,
+                   batch=batch) as s:
+        for m in range(batch):
+            s.set_state(z0[m][None], m)
+        s.run(2)
+        for m in range(batch):
+            assert rel(s.get_state(m)[0], orc.run(z0[m], 2)) < 1e-11, m
+
+
 def test_ns2d_reference_ic(gpu):
     n = 128
     g = O.grid2(n, n)
