@@ -52,6 +52,10 @@ static int load_record(const char *path, long irest, const psgh_grid *g, int nf,
   }
   const long n = 7 + 2L * nf * g->nkt;
   double *rec = malloc(sizeof(double) * n);
+  if (!rec) {
+    fclose(f);
+    return -4;
+  }
   int rc = 0;
   for (long i = 1; i <= irest && rc == 0; i++)
     if (psgh_rec_read(f, rec, (long)sizeof(double) * n) != (long)sizeof(double) * n) rc = -3;
@@ -113,6 +117,7 @@ static int run_ns2d(int nx, int ny, double alpha) {
     int hdr[3];
     if (psgh_rec_read(f, hdr, sizeof hdr) < 12) return die("restart: infile has no header record");
     rec = malloc(sizeof(double) * (8 + 2 * nkt));
+    if (!rec) return die("out of memory");
     for (int ntime = 1; ntime <= IREST; ntime++) {
       const int ns = (rt->type == 'B' && ntime == 1) ? 1 : rt->ns;
       const long want = 8L * ns + 16L * nkt;
@@ -152,6 +157,7 @@ static int run_ns2d(int nx, int ny, double alpha) {
   if (UPOflag == 1) {
     /* shift the final state back and compare with the initial one (main.F90:243-290) */
     double *zs = calloc(2 * nkt, sizeof(double)), nz2 = 0, nf2 = 0;
+    if (!zs) return die("out of memory");
     for (long i = 0; i < nkt; i++) {
       if (!L[i]) continue;
       const double ph = -(kx[i] * SHIFTXin + ky[i] * SHIFTYin), c = cos(ph), s = sin(ph);
@@ -255,6 +261,7 @@ int main(int argc, char **argv) {
       if (r == -1) return die("restart: cannot open infile");
       if (r == -2) return die("restart: infile header does not match the grid");
       if (r == -3) return die("restart: infile has fewer records than IREST");
+      if (r == -4) return die("out of memory");
     }
     psgh_ikf_ikn(&g, ikf, ikn);
     double TIME = 0.0, TSTART = 0.0;
