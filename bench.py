@@ -179,7 +179,7 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-4):
 
 
 def newton_case(P, n=512, t_spin=100.0, t_window=300.0, dt_sample=0.25, nguess=4, max_newton=20,
-                tmin=5.0, tmax=30.0):
+                tmin=5.0, tmax=30.0, save_orbit=None):
     """2D Kolmogorov flow n^2, Re=40, sin(4y) forcing, dt=0.005: spin up a seeded
     random-phase state, take the closest shift-minimised near-recurrences with
     T in [tmin, tmax] as guesses, and run Newton-GMRES-hookstep (Krylov 40,
@@ -216,6 +216,11 @@ def newton_case(P, n=512, t_spin=100.0, t_window=300.0, dt_sample=0.25, nguess=4
                                   max_newton=max_newton, dt0=0.005, verbose=1)
                 res["jv"] = name
                 res["guess"] = k
+                if save_orbit and jv == 0 and res["status_name"] == "CONVERGED" and not os.path.exists(save_orbit):
+                    L1 = L == 1
+                    np.savez_compressed(save_orbit, z_active=z[0][L1], T=res["T"], sx=res["sx"], ner=res["ner"],
+                                        n=n, re=40.0, dt0=0.005)
+                    log("bench: converged orbit saved to %s" % save_orbit)
                 log("bench: newton guess %d (T=%.3f) %s: %s ner=%.2e T=%.6f in %.1f s, %d iterations" %
                     (k, h["T"], name, res["status_name"], res["ner"], res["T"], res["seconds"], res["newton_iters"]))
                 rs.append(res)
@@ -268,7 +273,19 @@ def bous_ensemble(P, n=512, members=4, steps=6, warmup=2):
     return out
 
 
-def cpu_baseline(n, steps):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(n, steps, steps_1core=1):
+    """The FP64 C oracle on this host's cores (OpenMP, all threads it is given), and
+    the same on one thread, over a bounded sample of the workload's steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     cores = O.lib().or_get_threads()
@@ -278,8 +295,17 @@ def cpu_baseline(n, steps):
     t0 = time.perf_counter()
     orc.run(st0, steps)
     dt = time.perf_counter() - t0
+    O.lib().or_set_threads(1)
+    t0 = time.perf_counter()
+    orc.run(st0, steps_1core)
+    dt1 = time.perf_counter() - t0
+    O.lib().or_set_threads(cores)
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     return {"value": steps / dt, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": "%d Heun+CN steps of the %d^3 workload, FP64 C oracle, OpenMP %d threads" % (steps, n, cores)}
+            "value_1core": steps_1core / dt1, "cpu_model": cpu_model(), "nproc": nproc,
+            "cpu_count": os.cpu_count(), "fft": "project-written FP64 FFT (no FFTW on the host)",
+            "sample": "%d Heun+CN steps of the %d^3 workload, FP64 C oracle, OpenMP %d threads; %d step(s) on "
+                      "1 thread" % (steps, n, cores, steps_1core)}
 
 
 def main():
@@ -299,6 +325,8 @@ def main():
     ap.add_argument("--newton", type=int, default=1, metavar="NGUESS",
                     help="2D 512^2 Newton case (FD vs TL Jv) on NGUESS recurrence guesses (0: skip)")
     ap.add_argument("--newton-n", type=int, default=512)
+    ap.add_argument("--save-orbit", default=None, metavar="NPZ",
+                    help="write the first orbit the finite-difference Newton converges (active modes, T, shift)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1", file=sys.stderr)
@@ -354,7 +382,7 @@ def main():
     newton = None
     if a.newton:
         log("bench: newton 2D %d^2, %d guesses" % (a.newton_n, a.newton))
-        newton = newton_case(P, n=a.newton_n, nguess=a.newton)
+        newton = newton_case(P, n=a.newton_n, nguess=a.newton, save_orbit=a.save_orbit)
     cpu = None
     if not a.no_cpu:
         log("bench: cpu baseline (%d steps)" % a.cpu_steps)

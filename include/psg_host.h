@@ -105,6 +105,41 @@ int psgh_zk_record(FILE *f, const double *scalars, int nscalars, const double *f
 /* raw physical dump: NR doubles, x fastest (3D_NavierStokes/src/cuda.cu:72-93) */
 int psgh_raw_write(const char *path, const double *r, long nr);
 
+/* ---------------- restart records ("infile type") ----------------
+ * The reference's restart readers take record IREST of `infile` in one of
+ * several layouts chosen by the "infile type" letter:
+ *   PSGH_REC_NS2D  2D_NavierStokes/src_RK4/src_gmres/init.F90:41-106 (types A-L;
+ *                  Fortran sequential, header jw,IKTX,IKTY; B's first record
+ *                  holds the time only)
+ *   PSGH_REC_NS3D  3D_NavierStokes/src/init.F90:66-147 and src_gmres/init.F90:
+ *                  75-130 (A, B: C stream; C: sequential, fields only;
+ *                  D: sequential with header jw,IKTX,IKTY,IKTZ)
+ *   PSGH_REC_BOUS  3D_Boussinesq/src/init.F90:46-182 (A, B stream; C fields
+ *                  only; D, E, G, H with a header; F without; F-H carry Ri and
+ *                  alpha, H ignores its alpha)
+ *   PSGH_REC_EULER 3D_Euler/src/init.F90:73-118 (A stream with the time;
+ *                  B fields only; C with a header and the time)
+ * Values that a type does not carry are left as the caller set them (the
+ * reference keeps the params.txt value, or the previous record's). */
+enum { PSGH_REC_NS2D = 0, PSGH_REC_NS3D = 1, PSGH_REC_BOUS = 2, PSGH_REC_EULER = 3 };
+typedef struct {
+  double time, v2, ri, alpha, period, shiftx, shifty, shiftz, ner, ner2, tunder, rsym;
+  int has_v2, has_ri, has_alpha;   /* set by psgh_restart_read when the record carried them */
+} psgh_restart;
+/* 1 if `type` is a layout of `family`, else 0 */
+int psgh_restart_known(int family, const char *type);
+/* read record irest (1-based) into state (nf*nkt complex, the grid's storage
+ * order) and the scalars it carries into r; returns 0 or -1 with msg set */
+int psgh_restart_read(const char *path, int family, const char *type, long irest, int iktx, int ikty, int iktz,
+                      int nf, long nkt, double *state, psgh_restart *r, char *msg, int msglen);
+/* the same one record after the other: open (checks the type and the header
+ * against the grid; NULL with msg set on failure), next (0 or -1), close */
+typedef struct psgh_restart_file psgh_restart_file;
+psgh_restart_file *psgh_restart_open(const char *path, int family, const char *type, int iktx, int ikty, int iktz,
+                                     int nf, long nkt, char *msg, int msglen);
+int psgh_restart_next(psgh_restart_file *rf, double *state, psgh_restart *r, char *msg, int msglen);
+void psgh_restart_close(psgh_restart_file *rf);
+
 #ifdef __cplusplus
 }
 #endif

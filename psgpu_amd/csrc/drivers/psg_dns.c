@@ -41,40 +41,6 @@ static int die(const char *what) {
   return 1;
 }
 
-/* record irest of a Zk.out-layout file: nsc scalars then nf*nkt complex */
-static int load_record(const char *path, long irest, const psgh_grid *g, int nf, double *state) {
-  FILE *f = fopen(path, "rb");
-  if (!f) return -1;
-  int hdr[4] = {0, 0, 0, 0};
-  if (psgh_rec_read(f, hdr, sizeof hdr) < 16 || hdr[1] != g->iktx || hdr[2] != g->ikty || hdr[3] != g->iktz) {
-    fclose(f);
-    return -2;
-  }
-  const long n = 7 + 2L * nf * g->nkt;
-  double *rec = malloc(sizeof(double) * n);
-  if (!rec) {
-    fclose(f);
-    return -4;
-  }
-  int rc = 0;
-  for (long i = 1; i <= irest && rc == 0; i++)
-    if (psgh_rec_read(f, rec, (long)sizeof(double) * n) != (long)sizeof(double) * n) rc = -3;
-  if (rc == 0) memcpy(state, rec + 7, sizeof(double) * 2 * nf * g->nkt);
-  free(rec);
-  fclose(f);
-  return rc;
-}
-
-/* scalars ahead of the field in a 2D restart record, by infile type, and
- * where PERIODin, SHIFTXin, SHIFTYin, NERin and v2 sit among them (-1: not
- * present); 2D_NavierStokes/src_RK4/init.F90:66-106.  Type B's first record
- * holds the time only. */
-typedef struct { char type; int ns, period, shiftx, shifty, ner, v2; } rec2d_t;
-static const rec2d_t REC2D[] = {
-  {'A', 1, -1, -1, -1, -1, -1}, {'B', 5, 1, 2, -1, 3, -1}, {'C', 5, 1, 2, -1, 3, -1},
-  {'D', 6, 2, 3, -1, 4, 1},     {'E', 6, 2, 3, -1, 4, -1}, {'G', 5, 1, 2, 3, 4, -1},
-  {'H', 6, 2, 3, 4, 5, 1},      {'I', 6, 2, 3, 4, 5, -1},  {'J', 7, 2, 3, 4, 5, -1}};
-
 static int run_ns2d(int nx, int ny, double alpha) {
   char msg[256];
   static psgh_kv kv;
@@ -94,7 +60,7 @@ static int run_ns2d(int nx, int ny, double alpha) {
   const long nkt = g.nkt;
   double *kx = malloc(sizeof(double) * nkt), *ky = malloc(sizeof(double) * nkt);
   double *z = calloc(2 * nkt, sizeof(double)), *z0 = calloc(2 * nkt, sizeof(double));
-  double *zr = malloc(sizeof(double) * (long)nx * ny), *rec = NULL;
+  double *zr = malloc(sizeof(double) * (long)nx * ny);
   int *L = malloc(sizeof(int) * nkt), *ikf = malloc(sizeof(int) * (long)g.nx2 * ny), *ikn = malloc(sizeof(int) * nkt);
   if (!kx || !ky || !z || !z0 || !zr || !L || !ikf || !ikn) return die("out of memory");
   int IKTX = g.iktx, IKTY = g.ikty, KTY = g.kty, NX = nx, NY = ny;
@@ -108,31 +74,18 @@ static int run_ns2d(int nx, int ny, double alpha) {
   if (IREST == 0) {
     psgh_ic_ns2d_ref(&g, ISEED, AMPV, z);
   } else {
-    const rec2d_t *rt = NULL;
-    for (unsigned i = 0; i < sizeof REC2D / sizeof REC2D[0]; i++)
-      if (itype && itype[0] == REC2D[i].type && itype[1] == 0) rt = &REC2D[i];
-    if (!rt) return die("restart: infile type unknown");
-    FILE *f = infile ? fopen(infile, "rb") : NULL;
-    if (!f) return die("restart: cannot open infile");
-    int hdr[3];
-    if (psgh_rec_read(f, hdr, sizeof hdr) < 12) return die("restart: infile has no header record");
-    rec = malloc(sizeof(double) * (8 + 2 * nkt));
-    if (!rec) return die("out of memory");
-    for (int ntime = 1; ntime <= IREST; ntime++) {
-      const int ns = (rt->type == 'B' && ntime == 1) ? 1 : rt->ns;
-      const long want = 8L * ns + 16L * nkt;
-      if (psgh_rec_read(f, rec, want) != want) return die("restart: infile record does not match the grid or has fewer records than IREST");
-      if (ns > 1) {
-        PERIODin = rt->period >= 0 ? rec[rt->period] : PERIODin;
-        SHIFTXin = rt->shiftx >= 0 ? rec[rt->shiftx] : SHIFTXin;
-        SHIFTYin = rt->shifty >= 0 ? rec[rt->shifty] : SHIFTYin;
-        NERin = rt->ner >= 0 ? rec[rt->ner] : NERin;
-        if (rt->v2 >= 0) v2 = rec[rt->v2];
-      }
-      memcpy(z, rec + ns, sizeof(double) * 2 * nkt);
-      printf(" Just read time= %.15g\n", rec[0]);
-    }
-    fclose(f);
+    /* 2D_NavierStokes/src_RK4/init.F90:41-106: record IREST of the infile type's layout */
+    psgh_restart rr;
+    memset(&rr, 0, sizeof rr);
+    if (psgh_restart_read(infile ? infile : "", PSGH_REC_NS2D, itype, IREST, IKTX, IKTY, 0, 1, nkt, z, &rr, msg,
+                          sizeof msg))
+      return die(msg);
+    printf(" Just read time= %.15g\n", rr.time);
+    PERIODin = rr.period;
+    SHIFTXin = rr.shiftx;
+    SHIFTYin = rr.shifty;
+    NERin = rr.ner;
+    if (rr.has_v2) v2 = rr.v2;
     for (long i = 0; i < nkt; i++)
       if (L[i] != 1) z[2 * i] = z[2 * i + 1] = 0.0;
   }
@@ -177,7 +130,7 @@ static int run_ns2d(int nx, int ny, double alpha) {
   }
   fclose(fzk);
   printf(" Finished\n");
-  free(kx); free(ky); free(z); free(z0); free(zr); free(rec); free(L); free(ikf); free(ikn);
+  free(kx); free(ky); free(z); free(z0); free(zr); free(L); free(ikf); free(ikn);
   return 0;
 }
 
@@ -223,6 +176,7 @@ int main(int argc, char **argv) {
   if (NOUTV == 0) NOUTV = 1;
   int STATSFLAG = (int)psgh_kv_int(&kv, "STATSFLAG", 0), RCFLAG = (int)psgh_kv_int(&kv, "RCFLAG", 0);
   int ADAPTFLAG = (int)psgh_kv_int(&kv, "ADAPTflag", 0);
+  const int UPOflag = (int)psgh_kv_int(&kv, "UPOflag", 0);
   double FK = KF;
 
   psgh_grid g;
@@ -255,13 +209,29 @@ int main(int argc, char **argv) {
       else if (euler) psgh_ic_euler_ref(&g, ISEED, AMPV, xii, eta, zet);
       else psgh_ic_bous_ref(&g, ISEED, AMPV, xii, eta, zet, rho);
     } else {
-      if (!infile || !itype || strcmp(itype, "D")) return die("restart: only infile type D (Zk.out layout)");
-      snprintf(path, sizeof path, "%s/%s", simdir, infile);
-      const int r = load_record(path, irest, &g, nf, state);
-      if (r == -1) return die("restart: cannot open infile");
-      if (r == -2) return die("restart: infile header does not match the grid");
-      if (r == -3) return die("restart: infile has fewer records than IREST");
-      if (r == -4) return die("out of memory");
+      /* record IREST of the infile type's layout: 3D_NavierStokes/src/init.F90:66-147,
+       * 3D_Euler/src/init.F90:66-118, 3D_Boussinesq/src/init.F90:46-182; the record's
+       * v2 (and for the Boussinesq F-H types Ri and alpha) replace the params.txt values,
+       * and with UPOflag = 1 its period is the run length (main.F90:68-76) */
+      psgh_restart rr;
+      memset(&rr, 0, sizeof rr);
+      snprintf(path, sizeof path, "%s/%s", simdir, infile ? infile : "");
+      const int fam = ns3d ? PSGH_REC_NS3D : (euler ? PSGH_REC_EULER : PSGH_REC_BOUS);
+      if (psgh_restart_read(path, fam, itype, irest, IKTX, IKTY, IKTZ, nf, nkt, state, &rr, msg, sizeof msg))
+        return die(msg);
+      printf(" Just read time= %.15g\n", rr.time);
+      if (rr.has_v2 && !euler) v2 = rr.v2;
+      if (rr.has_ri) Ri = rr.ri;
+      if (rr.has_alpha && rr.alpha != alpha) {
+        alpha = rr.alpha;
+        if (psgh_grid3(&g, nx, ny, nz, alpha)) return die("bad grid");
+        psgh_tables(&g, kx, ky, kz, L);
+      }
+      if (ns3d) psgh_nutab(&g, v2, DELT, nu, nuzn, nu1);
+      if (UPOflag == 1) {
+        TSTEP = rr.period;
+        NSTOP = (int)(TSTEP / DELT);
+      }
     }
     psgh_ikf_ikn(&g, ikf, ikn);
     double TIME = 0.0, TSTART = 0.0;

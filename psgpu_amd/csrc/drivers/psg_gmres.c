@@ -10,22 +10,29 @@
  * paramsEXTRA.txt (src_gmres/parameters.F90:140-200).
  *
  * For each guess IGUESS = IREST..IGUESSEND the state, period and shifts are
- * read from record IGUESS of `infile` (infile type D: the Zk.out layout, a
- * header record 1,IKTX,IKTY[,IKTZ] then records t,v2,T,sx,sy,[sz,]NER,state),
- * the Newton solve runs on the GPU, and the driver writes
+ * read from record IGUESS of `infile` in the layout its "infile type" names
+ * (2D types A-L, 2D_NavierStokes/src_RK4/src_gmres/init.F90:66-106; 3D types
+ * A-D, 3D_NavierStokes/src_gmres/init.F90:77-130; psgh_restart_read).  As in
+ * the reference (2D main.F90:366-376, 3D main.F90:439-450): the 2D y shift
+ * enters with its sign flipped, a fixed period rescales the x shift, a
+ * record's v2 replaces the params.txt Re (types that carry it), and a period
+ * within 1e-10 of a whole number of steps is snapped to it (main.F90:678-683).
+ * The Newton solve runs on the GPU, and the driver writes
  *   UPOinfo.dat  one line per guess: guess, start time, Re, period, shifts and
  *                residual of the guess, status, UPO number, Re, period,
  *                shifts and residual reached, Newton/GMRES/hook counts;
  *   UPOs.out     header + one record per converged orbit;
  *   Zk.out       header + the guess and the final state of every guess.
  * The grid is compiled into the reference (global_params.F90); here it is given
- * on the command line.  Arc-length continuation, Arnoldi/Floquet analysis and
- * the other infile types are not part of this path and are rejected.
+ * on the command line.  Arc-length continuation and Arnoldi/Floquet analysis
+ * are not part of this path and are rejected, as is a type L record with
+ * RsymFLAG set (the reflection symmetry is not built).
  *
  * usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift]
  * (--jv tl: tangent-linear Jv; tl32: the same in FP32-bulk passes under the
  * FP64 residual; --noshift: the x shift is not an unknown, for equilibria and
- * other shift-free solutions)
+ * other shift-free solutions; --list: print each guess as read and adjusted,
+ * with no GPU work)
  */
 #include <math.h>
 #include <stdio.h>
@@ -61,10 +68,11 @@ static int die(const char *what) {
 
 int main(int argc, char **argv) {
   if (argc < 3)
-    return die("usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift]");
+    return die("usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift] "
+               "[--list]");
   const int is2d = strcmp(argv[1], "ns2d") == 0;
   if (!is2d && strcmp(argv[1], "ns3d") != 0) return die("flavour must be ns2d or ns3d");
-  int nx = atoi(argv[2]), ny = nx, nz = nx, ai = 3, jv = 0, maxnewton = -1, noshift = 0;
+  int nx = atoi(argv[2]), ny = nx, nz = nx, ai = 3, jv = 0, maxnewton = -1, noshift = 0, list = 0;
   double alpha = 1.0;
   const char *dir = ".";
   if (argc > 5 && argv[3][0] != '-') { ny = atoi(argv[3]); nz = atoi(argv[4]); alpha = atof(argv[5]); ai = 6; }
@@ -77,6 +85,7 @@ int main(int argc, char **argv) {
     }
     else if (!strcmp(argv[i], "--maxnewton") && i + 1 < argc) maxnewton = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--noshift")) noshift = 1;
+    else if (!strcmp(argv[i], "--list")) list = 1;
     else return die("unknown option");
   }
   char path[1024], msg[256];
@@ -91,7 +100,8 @@ int main(int argc, char **argv) {
   const long irest = psgh_kv_int(&kv, "IREST", 1);
   const long iguessend = psgh_kv_int(&kx, "IGUESSEND", irest);
   const char *infile = psgh_kv_str(&kv, "infile"), *itype = psgh_kv_str(&kv, "infile type");
-  if (!infile || !itype || strcmp(itype, "D") != 0) return die("only infile type D (Zk.out layout) is supported");
+  const int family = is2d ? PSGH_REC_NS2D : PSGH_REC_NS3D;
+  if (!infile || !psgh_restart_known(family, itype)) return die("infile type unknown for this flavour");
   if (psgh_kv_int(&kx, "arclcFLAG", 0) || psgh_kv_int(&kx, "ArnoldiFLAG", 0))
     return die("arc-length continuation and Arnoldi analysis are not part of this path");
   if (irest < 1) return die("IREST must name the first guess record (>= 1)");
@@ -112,8 +122,7 @@ int main(int argc, char **argv) {
   }
   long iktx, ikty, iktz, nkt, nact;
   if (psg_sizes(&c, &iktx, &ikty, &iktz, &nkt, &nact)) return die(psg_last_error());
-  psg_ctx *ctx;
-  if (psg_create(&c, &ctx)) return die(psg_last_error());
+  psg_ctx *ctx = NULL;
   psg_newton_opts o;
   psg_newton_defaults(c.flavour, &o);
   o.jv = jv;
@@ -121,19 +130,15 @@ int main(int argc, char **argv) {
   o.fixed_period = (int)psgh_kv_int(&kx, "Fixed Period", 0);
   if (maxnewton > 0) o.max_newton = maxnewton;
   if (noshift) o.nshift = 0;
-  const int nsc = is2d ? 6 : 7;      /* scalars per record */
+  const int nsc = is2d ? 6 : 7;      /* scalars per output record */
   const long ncpx = (is2d ? 1 : 3) * nkt;
 
   snprintf(path, sizeof path, "%s/%s", dir, infile);
-  FILE *fin = fopen(path, "rb");
-  if (!fin) return die("cannot open infile");
-  double *rec = malloc(sizeof(double) * (nsc + 2 * ncpx));
-  {
-    int hdr[4] = {0, 0, 0, 0};
-    const long n = psgh_rec_read(fin, hdr, sizeof hdr);
-    if (n < 12 || hdr[1] != iktx || hdr[2] != ikty || (!is2d && hdr[3] != iktz))
-      return die("infile header does not match the grid (IKTX, IKTY[, IKTZ])");
-  }
+  psgh_restart_file *fin = psgh_restart_open(path, family, itype, (int)iktx, (int)ikty, (int)iktz, is2d ? 1 : 3, nkt,
+                                             msg, sizeof msg);
+  if (!fin) return die(msg);
+  double *z = malloc(sizeof(double) * 2 * ncpx);
+  if (!z) return die("out of memory");
   snprintf(path, sizeof path, "%s/UPOinfo.dat", dir);
   FILE *finfo = fopen(path, "w");
   snprintf(path, sizeof path, "%s/UPOs.out", dir);
@@ -149,29 +154,56 @@ int main(int argc, char **argv) {
 
   int nupo = 0, rc = 0;
   long at = 0;   /* records consumed */
+  psgh_restart rr;
+  memset(&rr, 0, sizeof rr);
   for (long ig = irest; ig <= iguessend && rc == 0; ig++) {
     while (at < ig) {
-      if (psgh_rec_read(fin, rec, sizeof(double) * (nsc + 2 * ncpx)) != (long)(sizeof(double) * (nsc + 2 * ncpx))) {
-        fprintf(stderr, "psg_gmres: infile has no record %ld\n", ig);
+      rr.has_v2 = 0;
+      if (psgh_restart_next(fin, z, &rr, msg, sizeof msg)) {
+        fprintf(stderr, "psg_gmres: %s\n", msg);
         rc = 1;
         break;
       }
       at++;
     }
     if (rc) break;
-    const double tin = rec[0], v2in = rec[1], Tin = rec[2], sxin = rec[3], syin = rec[4];
-    const double szin = is2d ? 0.0 : rec[5], nerin = rec[nsc - 1];
-    double *z = rec + nsc;
-    double T = o.fixed_period ? psgh_kv_float(&kx, "Fixed Period Size", Tin) : Tin, sx = sxin, sz = szin;
-    o.sy = syin;
-    psgh_zk_record(fzk, rec, nsc, z, ncpx);
+    if (rr.rsym != 0.0) {
+      fprintf(stderr, "psg_gmres: record %ld asks for the reflection symmetry (RsymFLAG), which is not built\n", ig);
+      rc = 1;
+      break;
+    }
+    /* the guess's Re: the record's v2 where its type carries one (v2in = v2, main.F90:688-695) */
+    const double v2in = rr.has_v2 ? rr.v2 : 1.0 / Re;
+    if (!list && (!ctx || c.v2 != v2in)) {
+      if (ctx) psg_destroy(ctx);
+      c.v2 = v2in;
+      if (psg_create(&c, &ctx)) return die(psg_last_error());
+    }
+    const double tin = rr.time, Tin = rr.period, sxin = rr.shiftx, syin = rr.shifty, szin = rr.shiftz;
+    const double nerin = rr.ner;
+    double T = Tin, sx = sxin, sz = szin;
+    o.sy = is2d ? -syin : syin;            /* SNEWTY = -SNEWTYin in 2D, main.F90:370 */
+    if (o.fixed_period) {                  /* main.F90:373-376 */
+      const double fp = psgh_kv_float(&kx, "Fixed Period Size", Tin);
+      sx = Tin != 0.0 ? fp * sx / Tin : sx;
+      T = fp;
+    }
+    /* a guess period from the DNS is made exactly a whole number of steps (main.F90:678-683) */
+    if (fabs(T - nearbyint(T / delt) * delt) < 1e-10) T = nearbyint(T / delt) * delt;
+    if (list) {   /* the guess as the solver would start from it; no GPU work */
+      printf("guess %ld: t=%.17g v2=%.17g T=%.17g sx=%.17g sy=%.17g sz=%.17g NER=%.17g\n", ig, tin, v2in, T, sx, o.sy,
+             sz, nerin);
+      continue;
+    }
+    const double in_sc[7] = {tin, v2in, Tin, sxin, o.sy, is2d ? nerin : szin, nerin};
+    psgh_zk_record(fzk, in_sc, nsc, z, ncpx);
     psg_newton_result r;
     if (psg_newton_solve(ctx, z, &T, &sx, &sz, &o, &r)) {
       fprintf(stderr, "psg_gmres: %s\n", psg_last_error());
       rc = 1;
       break;
     }
-    double out[7] = {0.0, c.v2, T, sx, syin, 0.0, 0.0};
+    double out[7] = {0.0, c.v2, T, sx, o.sy, 0.0, 0.0};
     if (is2d) out[5] = r.ner;
     else { out[5] = sz; out[6] = r.ner; }
     psgh_zk_record(fzk, out, nsc, z, ncpx);
@@ -181,12 +213,12 @@ int main(int argc, char **argv) {
     fort_e(e3, 14, 5, r.ner);
     if (is2d)
       fprintf(finfo, "%5ld    %s%10.5f%12.5f%12.5f%12.5f%s : %s : %4d%12.5f%10.5f%10.5f%10.5f%s%6d%6d%6d\n", ig, e1,
-              1.0 / v2in, Tin, sxin, syin, e2, status_text[r.status], nupo + 1, Re, T, sx, syin, e3, r.newton_iters,
-              r.gmres_iters, r.hook_iters);
+              1.0 / v2in, Tin, sxin, syin, e2, status_text[r.status], nupo + 1, 1.0 / c.v2, T, sx, o.sy, e3,
+              r.newton_iters, r.gmres_iters, r.hook_iters);
     else
       fprintf(finfo, "%5ld    %s%10.5f%12.5f%12.5f%12.5f%12.5f%s : %s : %4d%12.5f%10.5f%10.5f%10.5f%10.5f%s%6d%6d%6d\n",
-              ig, e1, 1.0 / v2in, Tin, sxin, syin, szin, e2, status_text[r.status], nupo + 1, Re, T, sx, syin, sz, e3,
-              r.newton_iters, r.gmres_iters, r.hook_iters);
+              ig, e1, 1.0 / v2in, Tin, sxin, syin, szin, e2, status_text[r.status], nupo + 1, 1.0 / c.v2, T, sx, o.sy,
+              sz, e3, r.newton_iters, r.gmres_iters, r.hook_iters);
     fflush(finfo);
     if (r.status == PSG_NEWTON_CONVERGED) {
       psgh_zk_record(fupo, out, nsc, z, ncpx);
@@ -197,12 +229,12 @@ int main(int argc, char **argv) {
            T, sx, r.newton_iters, r.gmres_iters, r.seconds);
     fflush(stdout);
   }
-  fclose(fin);
+  psgh_restart_close(fin);
   fclose(finfo);
   fclose(fupo);
   fclose(fzk);
-  free(rec);
-  psg_destroy(ctx);
+  free(z);
+  if (ctx) psg_destroy(ctx);
   printf("%d orbit(s) written to UPOs.out\n", nupo);
   return rc;
 }

@@ -55,6 +55,7 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   if (!ctx) return;
   const long nkt = *NKT, nr = (long)*NX * *NY * *NZ;
   double *buf = malloc(sizeof(double) * 8 * nkt), *phys = NULL;
+  if (!buf) { lg_fail("host allocation"); return; }
   psg_double2 *f[4] = {xii, eta, zet, rho};
   lg_pack(buf, nkt, 4, f);
   if (psg_set_tables(ctx, NULL, NULL, LC) || psg_set_state(ctx, 0, buf)) { lg_fail("upload"); free(buf); return; }
@@ -65,6 +66,7 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
     int have_phys = 0;
     if (at_tout) {
       if (!phys) phys = malloc(sizeof(double) * 7 * nr);
+      if (!phys) { lg_fail("host allocation"); break; }
       if (psg_physical(ctx, 0, phys)) { lg_fail("physical"); break; }
       have_phys = 1;
       double umax = 0.0;
@@ -79,6 +81,7 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
     const int at_vout = *statsFLAG != 0 && nStop > 1 && lg_crossing(tme, dt, vout);
     if (at_vout && !have_phys) {
       if (!phys) phys = malloc(sizeof(double) * 7 * nr);
+      if (!phys) { lg_fail("host allocation"); break; }
       if (psg_physical(ctx, 0, phys)) { lg_fail("physical"); break; }
     }
     if (at_vout) {

@@ -37,6 +37,7 @@ void timestep_cuda_bous_gmres(psg_double2 *xii, psg_double2 *eta, psg_double2 *z
   if (!ctx) return;
   const long nkt = *NKT;
   double *buf = malloc(sizeof(double) * 8 * nkt);
+  if (!buf) { lg_fail("host allocation"); return; }
   psg_double2 *f[4] = {xii, eta, zet, rho};
   lg_pack(buf, nkt, 4, f);
   if (psg_set_tables(ctx, NULL, NULL, LC) || psg_set_state(ctx, 0, buf)) { lg_fail("upload"); free(buf); return; }
@@ -68,6 +69,7 @@ void timederiv_cuda_(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet, psg_d
   if (!ctx) return;
   const long nkt = *NKT;
   double *buf = malloc(sizeof(double) * 16 * nkt);
+  if (!buf) { lg_fail("host allocation"); return; }
   psg_double2 *f[4] = {xii, eta, zet, rho};
   lg_pack(buf, nkt, 4, f);
   if (psg_set_tables(ctx, NULL, NULL, LC) || psg_timederiv(ctx, 0, buf, buf + 8 * nkt)) lg_fail("timederiv");

@@ -20,6 +20,7 @@ void timestep_cuda_euler(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet, d
   if (!ctx) return;
   const long nkt = *NKT, nr = (long)*NX * *NY * *NZ;
   double *buf = malloc(sizeof(double) * 6 * nkt);
+  if (!buf) { lg_fail("host allocation"); return; }
   psg_double2 *f[3] = {xii, eta, zet};
   lg_pack(buf, nkt, 3, f);
   if (psg_set_tables(ctx, NULL, NULL, LC) || psg_set_state(ctx, 0, buf)) { lg_fail("upload"); free(buf); return; }
@@ -30,6 +31,7 @@ void timestep_cuda_euler(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet, d
   while (nt < nstop) {
     if (nt % nvort == 0 && nstop > 1) {
       if (!phys) phys = malloc(sizeof(double) * 6 * nr);
+      if (!phys) { lg_fail("host allocation"); break; }
       if (psg_physical(ctx, 0, phys)) { lg_fail("physical"); break; }
       memcpy(xir, phys, sizeof(double) * nr);
       memcpy(etr, phys + nr, sizeof(double) * nr);

@@ -64,6 +64,7 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   if (!ctx) return;
   const long nkt = *NKT, nr = (long)*NX * *NY * *NZ;
   double *buf = malloc(sizeof(double) * 6 * nkt);
+  if (!buf) { lg_fail("host allocation"); return; }
   psg_double2 *f[3] = {xii, eta, zet};
   lg_pack(buf, nkt, 3, f);
   if (psg_set_tables(ctx, nuzn, nu1, LC) || psg_set_state(ctx, 0, buf)) { lg_fail("upload"); free(buf); return; }
@@ -78,6 +79,13 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
     fe = fopen(p, "w");
     tavg = calloc(6 * nkt, sizeof(double));
     if (!fs || !fe) { fprintf(stderr, "psgpu legacy: cannot open JOB%d/stats.dat\n", *RANK); }
+    if (!tavg) {
+      lg_fail("host allocation");
+      if (fs) fclose(fs);
+      if (fe) fclose(fe);
+      free(buf);
+      return;
+    }
   }
   const int nout = *NOUT > 0 ? *NOUT : 1, nvort = *NOUTV > 0 ? *NOUTV : 1, nstop = *NSTOP;
   double *phys = NULL;
@@ -87,6 +95,7 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   while (nt < nstop) {
     if (nt % nvort == 0 && nstop > 1) {
       if (!phys) phys = malloc(sizeof(double) * 6 * nr);
+      if (!phys) { lg_fail("host allocation"); break; }
       if (psg_physical(ctx, 0, phys)) { lg_fail("physical"); break; }
       memcpy(xir, phys, sizeof(double) * nr);
       memcpy(etr, phys + nr, sizeof(double) * nr);

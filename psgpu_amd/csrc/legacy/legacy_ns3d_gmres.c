@@ -25,6 +25,7 @@ void timestep_cuda_ns3d_gmres(psg_double2 *xii, psg_double2 *eta, psg_double2 *z
   long nkt = 0;
   psg_sizes(&c, NULL, NULL, NULL, &nkt, NULL);
   double *buf = malloc(sizeof(double) * 6 * nkt);
+  if (!buf) { lg_fail("host allocation"); return; }
   psg_double2 *f[3] = {xii, eta, zet};
   lg_pack(buf, nkt, 3, f);
   if (psg_set_tables(ctx, nuzn, nu1, LC) || psg_set_state(ctx, 0, buf) || psg_run(ctx, *NSTOP) ||

@@ -23,12 +23,6 @@ void set_error(const char *fmt, ...) {
   va_end(ap);
 }
 
-struct PendProf {
-  int name;
-  long bytes;
-  int e0, e1;
-};
-
 static hipEvent_t get_event(Ctx *c) {
   if (c->ev_used >= (int)c->evs.size()) {
     hipEvent_t e;
@@ -38,23 +32,8 @@ static hipEvent_t get_event(Ctx *c) {
   return c->evs[c->ev_used++];
 }
 
-struct ProfState {
-  std::vector<PendProf> pend;
-  int cur_name = -1;
-  long cur_bytes = 0;
-  int cur_e0 = -1;
-};
-static ProfState *pstate(Ctx *c) {
-  static thread_local std::vector<std::pair<Ctx *, ProfState *>> tab;
-  for (auto &p : tab)
-    if (p.first == c) return p.second;
-  tab.push_back({c, new ProfState()});
-  return tab.back().second;
-}
-
 void prof_begin(Ctx *c, const char *name, long bytes) {
   if (!c->prof) return;
-  ProfState *s = pstate(c);
   int idx = -1;
   for (size_t i = 0; i < c->profs.size(); i++)
     if (c->profs[i].name == name) idx = (int)i;
@@ -67,27 +46,25 @@ void prof_begin(Ctx *c, const char *name, long bytes) {
   hipEvent_t e0 = get_event(c);
   if (!e0) return;
   hipEventRecord(e0, c->stream);
-  s->cur_name = idx;
-  s->cur_bytes = bytes;
-  s->cur_e0 = c->ev_used - 1;
+  c->prof_cur_name = idx;
+  c->prof_cur_bytes = bytes;
+  c->prof_cur_e0 = c->ev_used - 1;
 }
 
 void prof_end(Ctx *c) {
   if (!c->prof) return;
-  ProfState *s = pstate(c);
-  if (s->cur_name < 0) return;
+  if (c->prof_cur_name < 0) return;
   hipEvent_t e1 = get_event(c);
   if (!e1) return;
   hipEventRecord(e1, c->stream);
-  s->pend.push_back(PendProf{s->cur_name, s->cur_bytes, s->cur_e0, c->ev_used - 1});
-  s->cur_name = -1;
+  c->prof_pend.push_back(ProfPending{c->prof_cur_name, c->prof_cur_bytes, c->prof_cur_e0, c->ev_used - 1});
+  c->prof_cur_name = -1;
 }
 
 static void prof_collect(Ctx *c) {
-  ProfState *s = pstate(c);
-  if (s->pend.empty()) return;
+  if (c->prof_pend.empty()) return;
   hipStreamSynchronize(c->stream);
-  for (auto &p : s->pend) {
+  for (auto &p : c->prof_pend) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->evs[p.e0], c->evs[p.e1]) == hipSuccess) {
       ProfEntry &e = c->profs[p.name];
@@ -96,7 +73,7 @@ static void prof_collect(Ctx *c) {
       e.count += 1;
     }
   }
-  s->pend.clear();
+  c->prof_pend.clear();
   c->ev_used = 0;
 }
 
@@ -220,7 +197,7 @@ static void free_ctx(Ctx *c) {
     if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
   void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->st32, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
                   c->tl_rh, c->tl_bufA, c->tl_bufB, c->tl_bufC, c->bufA, c->bufB, c->bufC,
-                  c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred};
+                  c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred, c->io};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto e : c->evs) hipEventDestroy(e);
@@ -315,24 +292,41 @@ static int run_steps_in(Ctx *c, int kind, int nsteps) {
   return run_eager(c, kind, left, false);
 }
 
+void tl_free(Ctx *c) {
+  for (void **p : {(void **)&c->tl_state, (void **)&c->tl_stage, (void **)&c->tl_rh, &c->tl_bufA, &c->tl_bufB,
+                   &c->tl_bufC}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+// All-or-nothing: on any failure every tangent-linear buffer is released and
+// the pointers are null again, so a later call retries instead of running
+// kernels on a partial set.
 int tl_alloc(Ctx *c) {
-  if (c->tl_state) return 0;
+  if (c->tl_state && c->tl_stage && c->tl_rh && c->tl_bufB && c->tl_bufC && (c->g.dim == 2 || c->tl_bufA))
+    return 0;
+  tl_free(c);
   const Grid &g = c->g;
   const size_t sb = sizeof(cpx<double>) * c->nf * g.nkt * c->batch;
   const size_t b = c->cfg.precision == PSG_FP32BULK ? sizeof(cpx<float>) : sizeof(cpx<double>);
-  PSG_HIP(hipMalloc(&c->tl_state, sb));
-  PSG_HIP(hipMalloc(&c->tl_stage, sb));
-  PSG_HIP(hipMalloc(&c->tl_rh, 3 * sb));
-  PSG_HIP(hipMemsetAsync(c->tl_state, 0, sb, c->stream));
-  PSG_HIP(hipMemsetAsync(c->tl_stage, 0, sb, c->stream));
-  PSG_HIP(hipMemsetAsync(c->tl_rh, 0, 3 * sb, c->stream));
-  if (g.dim == 2) {
-    PSG_HIP(hipMalloc(&c->tl_bufB, 6 * g.ysz * c->batch * b));
-    PSG_HIP(hipMalloc(&c->tl_bufC, 4 * g.ysz * c->batch * b));
-  } else {
-    PSG_HIP(hipMalloc(&c->tl_bufA, 12 * g.zsz * c->batch * b));
-    PSG_HIP(hipMalloc(&c->tl_bufB, 12 * g.ysz * c->batch * b));
-    PSG_HIP(hipMalloc(&c->tl_bufC, 6 * g.ysz * c->batch * b));
+  bool ok = hipMalloc(&c->tl_state, sb) == hipSuccess && hipMalloc(&c->tl_stage, sb) == hipSuccess &&
+            hipMalloc(&c->tl_rh, 3 * sb) == hipSuccess;
+  if (ok && g.dim == 2)
+    ok = hipMalloc(&c->tl_bufB, 6 * g.ysz * c->batch * b) == hipSuccess &&
+         hipMalloc(&c->tl_bufC, 4 * g.ysz * c->batch * b) == hipSuccess;
+  else if (ok)
+    ok = hipMalloc(&c->tl_bufA, 12 * g.zsz * c->batch * b) == hipSuccess &&
+         hipMalloc(&c->tl_bufB, 12 * g.ysz * c->batch * b) == hipSuccess &&
+         hipMalloc(&c->tl_bufC, 6 * g.ysz * c->batch * b) == hipSuccess;
+  ok = ok && hipMemsetAsync(c->tl_state, 0, sb, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->tl_stage, 0, sb, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->tl_rh, 0, 3 * sb, c->stream) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    tl_free(c);
+    set_error("tangent-linear buffers: device allocation failed");
+    return PSG_ENOMEM;
   }
   return 0;
 }
@@ -549,12 +543,28 @@ int psg_diag(psg_ctx *c, int m, double *e, double *z, double *spec) {
   return diag_member(c, m, e, z, spec);
 }
 
+// Device buffer for the host-array entry points (psg_nonlin, psg_timederiv,
+// psg_physical), grown on demand and kept with the context, so repeated
+// calls (a legacy timederiv_cuda_ loop, DNS dumps) allocate nothing.
+static int io_buffer(Ctx *c, size_t bytes, void **p) {
+  if (c->io_n < bytes) {
+    PSG_HIP(hipStreamSynchronize(c->stream));
+    if (c->io) (void)hipFree(c->io);
+    c->io = nullptr;
+    c->io_n = 0;
+    PSG_HIP(hipMalloc(&c->io, bytes));
+    c->io_n = bytes;
+  }
+  *p = c->io;
+  return 0;
+}
+
 int psg_nonlin(psg_ctx *c, int m, const double *host_in, double *host_out) {
   if (int r = check_member(c, m)) return r;
   if (c->cfg.flavour == PSG_BOUS3D) { set_error("psg_nonlin: use psg_timederiv for the Boussinesq flavour"); return PSG_EINVAL; }
   const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
   cpx<double> *dout = nullptr;
-  PSG_HIP(hipMalloc(&dout, n));
+  if (int r = io_buffer(c, n, (void **)&dout)) return r;
   PSG_HIP(hipMemcpyAsync(c->tmp, host_in, n, hipMemcpyHostToDevice, c->stream));
   int r = c->g.dim == 2 ? ns2d_nonlin(c, 0, c->tmp, dout) : ns3d_nonlin(c, 0, c->tmp, dout);
   if (!r) {
@@ -562,7 +572,6 @@ int psg_nonlin(psg_ctx *c, int m, const double *host_in, double *host_out) {
         hipStreamSynchronize(c->stream) != hipSuccess) { set_error("nonlin copy"); r = PSG_EHIP; }
   }
   hipStreamSynchronize(c->stream);
-  hipFree(dout);
   return r;
 }
 
@@ -570,7 +579,7 @@ int psg_timederiv(psg_ctx *c, int m, const double *host_in, double *host_out) {
   if (int r = check_member(c, m)) return r;
   const size_t n = sizeof(cpx<double>) * c->nf * c->g.nkt;
   cpx<double> *dout = nullptr;
-  PSG_HIP(hipMalloc(&dout, n));
+  if (int r = io_buffer(c, n, (void **)&dout)) return r;
   PSG_HIP(hipMemcpyAsync(c->tmp, host_in, n, hipMemcpyHostToDevice, c->stream));
   int r = timederiv_dev(c, c->tmp, dout);
   if (!r) {
@@ -578,7 +587,6 @@ int psg_timederiv(psg_ctx *c, int m, const double *host_in, double *host_out) {
         hipStreamSynchronize(c->stream) != hipSuccess) { set_error("timederiv copy"); r = PSG_EHIP; }
   }
   hipStreamSynchronize(c->stream);
-  hipFree(dout);
   return r;
 }
 
@@ -588,14 +596,13 @@ int psg_physical(psg_ctx *c, int m, double *host_out) {
   const long nr = (long)c->g.nx * c->g.ny * c->g.nz;
   const int nfo = c->g.dim == 2 ? 3 : (bous ? 7 : 6);
   double *d = nullptr;
-  PSG_HIP(hipMalloc(&d, sizeof(double) * nr * nfo));
+  if (int r = io_buffer(c, sizeof(double) * nr * nfo, (void **)&d)) return r;
   int r = c->g.dim == 2 ? ns2d_physical(c, m, d) : (bous ? bous_physical(c, m, d) : ns3d_physical(c, m, d));
   if (!r) {
     if (hipMemcpyAsync(host_out, d, sizeof(double) * nr * nfo, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) { set_error("physical copy"); r = PSG_EHIP; }
   }
   hipStreamSynchronize(c->stream);
-  hipFree(d);
   return r;
 }
 
@@ -692,6 +699,9 @@ int psg_profile_enable(psg_ctx *c, int on) {
   if (!c) return PSG_EINVAL;
   hipStreamSynchronize(c->stream);
   c->prof = on != 0;
+  // pending brackets index profs and evs: drop them together with the table
+  c->prof_pend.clear();
+  c->prof_cur_name = -1;
   c->profs.clear();
   c->ev_used = 0;
   return 0;

@@ -607,3 +607,136 @@ int psgh_raw_write(const char *path, const double *r, long nr) {
   fclose(f);
   return w == (size_t)nr ? 0 : -1;
 }
+
+/* ---------------- restart records ----------------
+ * One layout per "infile type": the scalars ahead of the fields as a string of
+ * slot letters (T time, V v2, G Ri, A alpha, P period, X/Y/Z shifts, N NER,
+ * M NER2, U timeUNDER, R RsymFLAG, - a value the reference reads into
+ * `dummy`), whether the file is a C stream (no record markers) and whether a
+ * header record (jw, IKTX, IKTY[, IKTZ]) comes first. */
+typedef struct {
+  char type;
+  int stream, header;
+  const char *slots, *first;   /* first: slots of record 1 when they differ (2D type B) */
+} rec_layout;
+
+static const rec_layout REC_NS2D[] = {   /* 2D_NavierStokes/src_RK4/src_gmres/init.F90:66-106 */
+  {'A', 0, 1, "T", NULL},        {'B', 0, 1, "TPXNM", "T"},    {'C', 0, 1, "TPXNM", NULL},
+  {'D', 0, 1, "TVPXNM", NULL},   {'E', 0, 1, "T-PXNM", NULL},  {'G', 0, 1, "TPXYN", NULL},
+  {'H', 0, 1, "TVPXYN", NULL},   {'I', 0, 1, "T-PXYN", NULL},  {'J', 0, 1, "TVPXYNU", NULL},
+  {'K', 0, 1, "T-PXYNU", NULL},  {'L', 0, 1, "T-PXYRNU", NULL}, {0, 0, 0, NULL, NULL}};
+static const rec_layout REC_NS3D[] = {   /* 3D_NavierStokes/src/init.F90:76-129 */
+  {'A', 1, 0, "TVPXYZN", NULL},  {'B', 1, 0, "T-PXYZN", NULL}, {'C', 0, 0, "", NULL},
+  {'D', 0, 1, "TVPXYZN", NULL},  {0, 0, 0, NULL, NULL}};
+static const rec_layout REC_EULER[] = {  /* 3D_Euler/src/init.F90:73-118 */
+  {'A', 1, 0, "T", NULL},        {'B', 0, 0, "", NULL},          {'C', 0, 1, "T", NULL},
+  {0, 0, 0, NULL, NULL}};
+static const rec_layout REC_BOUS[] = {   /* 3D_Boussinesq/src/init.F90:46-182 */
+  {'A', 1, 0, "TVPXYZN", NULL},  {'B', 1, 0, "T-PXYZN", NULL},   {'C', 0, 0, "", NULL},
+  {'D', 0, 1, "TVPXYZN", NULL},  {'E', 0, 1, "T-PXYZN", NULL},   {'F', 0, 0, "TVGAPXYZN", NULL},
+  {'G', 0, 1, "TVGAPXYZN", NULL}, {'H', 0, 1, "TVG-PXYZN", NULL}, {0, 0, 0, NULL, NULL}};
+
+static const rec_layout *rec_find(int family, const char *type) {
+  const rec_layout *t = family == PSGH_REC_NS2D ? REC_NS2D : family == PSGH_REC_NS3D ? REC_NS3D :
+                        family == PSGH_REC_BOUS ? REC_BOUS : family == PSGH_REC_EULER ? REC_EULER : NULL;
+  if (!t || !type || !type[0] || type[1]) return NULL;
+  for (; t->type; t++)
+    if (t->type == type[0]) return t;
+  return NULL;
+}
+
+int psgh_restart_known(int family, const char *type) { return rec_find(family, type) != NULL; }
+
+static void rec_assign(const char *slots, const double *sc, psgh_restart *r) {
+  for (int i = 0; slots[i]; i++) {
+    const double v = sc[i];
+    switch (slots[i]) {
+      case 'T': r->time = v; break;
+      case 'V': r->v2 = v; r->has_v2 = 1; break;
+      case 'G': r->ri = v; r->has_ri = 1; break;
+      case 'A': r->alpha = v; r->has_alpha = 1; break;
+      case 'P': r->period = v; break;
+      case 'X': r->shiftx = v; break;
+      case 'Y': r->shifty = v; break;
+      case 'Z': r->shiftz = v; break;
+      case 'N': r->ner = v; break;
+      case 'M': r->ner2 = v; break;
+      case 'U': r->tunder = v; break;
+      case 'R': r->rsym = v; break;
+      default: break;
+    }
+  }
+}
+
+struct psgh_restart_file {
+  FILE *f;
+  const rec_layout *t;
+  int family, nf;
+  long nkt, nread;
+  double sc[16];
+};
+
+psgh_restart_file *psgh_restart_open(const char *path, int family, const char *type, int iktx, int ikty, int iktz,
+                                     int nf, long nkt, char *msg, int msglen) {
+  const rec_layout *t = rec_find(family, type);
+  if (!t) { snprintf(msg, (size_t)msglen, "infile type %s unknown", type ? type : "(none)"); return NULL; }
+  FILE *f = fopen(path, "rb");
+  if (!f) { snprintf(msg, (size_t)msglen, "cannot open %s", path); return NULL; }
+  if (t->header) {
+    int32_t h[4] = {0, 0, 0, 0};
+    const long want = family == PSGH_REC_NS2D ? 12 : 16;
+    if (psgh_rec_read(f, h, sizeof h) != want || h[1] != iktx || h[2] != ikty ||
+        (family != PSGH_REC_NS2D && h[3] != iktz)) {
+      snprintf(msg, (size_t)msglen, "infile header does not match the grid (IKTX, IKTY%s)",
+               family == PSGH_REC_NS2D ? "" : ", IKTZ");
+      fclose(f);
+      return NULL;
+    }
+  }
+  psgh_restart_file *rf = calloc(1, sizeof *rf);
+  if (!rf) { fclose(f); snprintf(msg, (size_t)msglen, "out of memory"); return NULL; }
+  rf->f = f; rf->t = t; rf->family = family; rf->nf = nf; rf->nkt = nkt;
+  return rf;
+}
+
+int psgh_restart_next(psgh_restart_file *rf, double *state, psgh_restart *r, char *msg, int msglen) {
+  const rec_layout *t = rf->t;
+  const char *slots = (rf->nread == 0 && t->first) ? t->first : t->slots;
+  const long ns = (long)strlen(slots), fbytes = 16L * rf->nf * rf->nkt, want = 8L * ns + fbytes;
+  int bad = 0;
+  if (t->stream) {
+    bad = (ns && fread(rf->sc, 8, (size_t)ns, rf->f) != (size_t)ns) ||
+          fread(state, 1, (size_t)fbytes, rf->f) != (size_t)fbytes;
+  } else {
+    /* one record: the scalars, then the fields, from the one payload */
+    int32_t m1 = 0, m2 = 0;
+    bad = fread(&m1, 4, 1, rf->f) != 1 || (long)m1 != want ||
+          (ns && fread(rf->sc, 8, (size_t)ns, rf->f) != (size_t)ns) ||
+          fread(state, 1, (size_t)fbytes, rf->f) != (size_t)fbytes || fread(&m2, 4, 1, rf->f) != 1 || m2 != m1;
+  }
+  if (bad) {
+    snprintf(msg, (size_t)msglen, "infile record %ld: missing, or its length does not match type %c on this grid",
+             rf->nread + 1, t->type);
+    return -1;
+  }
+  rec_assign(slots, rf->sc, r);
+  rf->nread++;
+  return 0;
+}
+
+void psgh_restart_close(psgh_restart_file *rf) {
+  if (!rf) return;
+  fclose(rf->f);
+  free(rf);
+}
+
+int psgh_restart_read(const char *path, int family, const char *type, long irest, int iktx, int ikty, int iktz,
+                      int nf, long nkt, double *state, psgh_restart *r, char *msg, int msglen) {
+  if (irest < 1) { snprintf(msg, (size_t)msglen, "IREST must be >= 1"); return -1; }
+  psgh_restart_file *rf = psgh_restart_open(path, family, type, iktx, ikty, iktz, nf, nkt, msg, msglen);
+  if (!rf) return -1;
+  int rc = 0;
+  for (long i = 1; i <= irest && rc == 0; i++) rc = psgh_restart_next(rf, state, r, msg, msglen);
+  psgh_restart_close(rf);
+  return rc;
+}

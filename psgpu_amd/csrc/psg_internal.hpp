@@ -35,6 +35,13 @@ struct ProfEntry {
   int count = 0;
 };
 
+// one launch bracketed by events, not yet read back
+struct ProfPending {
+  int name;
+  long bytes;
+  int e0, e1;
+};
+
 struct Ctx {
   psg_config cfg;
   Grid g;
@@ -78,11 +85,19 @@ struct Ctx {
   // diagnostics scratch
   double *dred = nullptr;
   size_t dred_n = 0;
+  // device side of the host-array entry points (nonlin, timederiv, physical)
+  void *io = nullptr;
+  size_t io_n = 0;
   // profiling
   bool prof = false;
   std::vector<ProfEntry> profs;
   std::vector<hipEvent_t> evs;
   int ev_used = 0;
+  // launches recorded since the last read-back, and the open bracket
+  std::vector<ProfPending> prof_pend;
+  int prof_cur_name = -1;
+  long prof_cur_bytes = 0;
+  int prof_cur_e0 = -1;
 };
 
 // error helpers (defined in psg_api.hip)

@@ -88,15 +88,89 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
     g = P.hgrid("ns2d", 32, 32)
     r = run(["ns2d", "32", "--jv", "exact"], d)
     assert r.returncode == 1 and "--jv" in r.stderr
-    setup_case(d, np.zeros(g.nkt, complex), main=main_2d(**{"infile type": "A"}))
+    setup_case(d, np.zeros(g.nkt, complex), main=main_2d(**{"infile type": "Z"}))
     r = run(["ns2d", "32"], d)
-    assert r.returncode == 1 and "infile type D" in r.stderr
+    assert r.returncode == 1 and "infile type unknown" in r.stderr
     setup_case(d, np.zeros(g.nkt, complex), extra=extra_2d(arclcFLAG="T"))
     r = run(["ns2d", "32"], d)
     assert r.returncode == 1 and "arc-length" in r.stderr
     os.remove(os.path.join(d, "paramsEXTRA.txt"))
     r = run(["ns2d", "32"], d)
     assert r.returncode == 1
+
+
+def listed(out):
+    rows = []
+    for line in out.splitlines():
+        if line.startswith("guess "):
+            rows.append({k: float(v) for k, v in (f.split("=") for f in line.split(": ", 1)[1].split())})
+    return rows
+
+
+# (infile type, scalars ahead of the field) of 2D_NavierStokes/src_RK4/src_gmres/init.F90:66-106
+# with t=1.5, v2=0.125, T=2.0, sx=0.25, sy=0.5, NER=0.75, NER2 or timeUNDER=0.875, RsymFLAG=0
+REC2D = {"C": (1.5, 2.0, 0.25, 0.75, 0.875), "D": (1.5, 0.125, 2.0, 0.25, 0.75, 0.875),
+         "E": (1.5, 9.0, 2.0, 0.25, 0.75, 0.875), "G": (1.5, 2.0, 0.25, 0.5, 0.75),
+         "H": (1.5, 0.125, 2.0, 0.25, 0.5, 0.75), "I": (1.5, 9.0, 2.0, 0.25, 0.5, 0.75),
+         "J": (1.5, 0.125, 2.0, 0.25, 0.5, 0.75, 0.875), "K": (1.5, 9.0, 2.0, 0.25, 0.5, 0.75, 0.875),
+         "L": (1.5, 9.0, 2.0, 0.25, 0.5, 0.0, 0.75, 0.875)}
+
+
+@pytest.mark.parametrize("itype", sorted(REC2D))
+def test_driver_reads_every_2d_record_type(tmp_path, itype):
+    # the scalars land where the reference's reader puts them: the y shift only from
+    # the types that carry one (negated, main.F90:370), v2 only from D, H, J, and a
+    # type-D record's NERin is not taken as a shift
+    d = str(tmp_path)
+    g = P.hgrid("ns2d", 32, 32)
+    write_params(os.path.join(d, "params_gmres.txt"), MAIN_2D, main_2d(**{"infile type": itype, "Re": "5.0"}))
+    write_params(os.path.join(d, "paramsEXTRA.txt"), EXTRA_2D, extra_2d(**{"Fixed Period": "F"}))
+    sc = REC2D[itype]
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<3i", 1, g.iktx, g.ikty)))
+        f.write(rec(struct.pack("<%dd" % len(sc), *sc) + np.zeros(g.nkt, np.complex128).tobytes()))
+    r = run(["ns2d", "32", "--list"], d)
+    assert r.returncode == 0, r.stderr
+    (row,) = listed(r.stdout)
+    has_y = itype in "GHIJKL"
+    assert row["t"] == 1.5 and row["T"] == 2.0 and row["sx"] == 0.25
+    assert row["sy"] == (-0.5 if has_y else 0.0)
+    assert row["NER"] == 0.75
+    assert row["v2"] == (0.125 if itype in "DHJ" else 0.2)
+
+
+def test_driver_fixed_period_rescales_shift_and_snaps_dns_periods(tmp_path):
+    d = str(tmp_path)
+    g = P.hgrid("ns2d", 32, 32)
+    write_params(os.path.join(d, "params_gmres.txt"), MAIN_2D, main_2d(**{"infile type": "H", "IREST": 1}))
+    write_params(os.path.join(d, "paramsEXTRA.txt"), EXTRA_2D, extra_2d(**{"IGUESSEND": 2, "Fixed Period": "F"}))
+    z = np.zeros(g.nkt, np.complex128).tobytes()
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<3i", 1, g.iktx, g.ikty)))
+        f.write(rec(struct.pack("<6d", 0.0, 0.2, 1.23 + 3e-11, 0.1, 0.0, 0.5) + z))     # a DNS period, round-off
+        f.write(rec(struct.pack("<6d", 0.0, 0.2, 1.2345, 0.1, 0.0, 0.5) + z))           # a Newton period
+    rows = listed(run(["ns2d", "32", "--list"], d).stdout)
+    assert rows[0]["T"] == round(1.23 / 0.01) * 0.01 and rows[1]["T"] == 1.2345
+    write_params(os.path.join(d, "paramsEXTRA.txt"), EXTRA_2D, extra_2d(**{"Fixed Period": "T",
+                                                                            "Fixed Period Size": "2.469"}))
+    (row,) = listed(run(["ns2d", "32", "--list"], d).stdout)[:1]
+    assert abs(row["sx"] - 2.469 * 0.1 / (1.23 + 3e-11)) < 1e-15 and row["T"] == 2.469
+
+
+def test_driver_rejects_reflection_records_and_bad_headers(tmp_path):
+    d = str(tmp_path)
+    g = P.hgrid("ns2d", 32, 32)
+    write_params(os.path.join(d, "params_gmres.txt"), MAIN_2D, main_2d(**{"infile type": "L"}))
+    write_params(os.path.join(d, "paramsEXTRA.txt"), EXTRA_2D, extra_2d())
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<3i", 1, g.iktx, g.ikty)))
+        f.write(rec(struct.pack("<8d", 0, 0, 1.0, 0, 0, 1.0, 0, 0) + np.zeros(g.nkt, np.complex128).tobytes()))
+    r = run(["ns2d", "32", "--list"], d)
+    assert r.returncode == 1 and "RsymFLAG" in r.stderr
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<3i", 1, g.iktx + 1, g.ikty)))
+    r = run(["ns2d", "32", "--list"], d)
+    assert r.returncode == 1 and "header" in r.stderr
 
 
 @pytest.mark.gpu
@@ -172,6 +246,34 @@ def test_dns_driver_ns3d_reference_run_matches_oracle(gpu, tmp_path):
     assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
     assert len(open(os.path.join(d, "JOB0", "stats.dat")).read().strip().splitlines()) == 3
     assert struct.unpack("<4i", read_recs(os.path.join(d, "JOB0", "Zk.out"))[0]) == (1, g.iktx, g.ikty, g.iktz)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("itype", ["A", "D"])
+def test_dns_driver_ns3d_restart_uses_the_record_v2(gpu, tmp_path, itype):
+    # 3D_NavierStokes/src/init.F90:79-125: types A (stream) and D (Zk.out layout) carry v2,
+    # and the CN tables are built from it; params.txt says Re = 100, the record v2 = 1/40
+    import oracle_lib as O
+    n = 32
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "JOB0"))
+    g = O.grid3(n, n, n)
+    st = O.ic_solenoidal(g, 99, 0.5)
+    body = struct.pack("<7d", 3.0, 1 / 40.0, 0.0, 0.0, 0.0, 0.0, 0.0) + st.astype(np.complex128).tobytes()
+    with open(os.path.join(d, "JOB0", "guess.out"), "wb") as f:
+        if itype == "A":
+            f.write(body)
+        else:
+            f.write(rec(struct.pack("<4i", 1, g.iktx, g.ikty, g.iktz)) + rec(body))
+    write_params(os.path.join(d, "JOB0", "params.txt"), MAIN_NS3D,
+                 ns3d_params(IREST=1, ISTOP=1, infile="guess.out", **{"infile type": itype, "STATSFLAG": 0}))
+    r = subprocess.run([DNS_EXE, "ns3d", str(n), "--dir", d], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    want = O.NS3D(n, v2=1 / 40.0, dt=0.005, force=1, kf=4.0).run(st, 20)
+    recs = read_recs(os.path.join(d, "JOB0", "final_001.out"))
+    assert struct.unpack("<7d", recs[1][:56])[1] == 1 / 40.0
+    got = np.frombuffer(recs[1][56:], np.complex128).reshape(3, g.nkt)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
 
 
 @pytest.mark.gpu
