@@ -83,6 +83,10 @@ int psg_get_state_device(psg_ctx *ctx, int member, void *dev);
 int psg_run(psg_ctx *ctx, int nsteps);
 /* change dt (rebuilds the CN tables, as src_gmres/main.F90:759-768 does per period) */
 int psg_set_dt(psg_ctx *ctx, double dt);
+/* change the viscosity v2 = 1/Re (rebuilds the CN tables, as the reference's
+ * arc-length continuation does at every change of Re,
+ * 2D_NavierStokes/src_RK4/src_gmres/main.F90:588-599) */
+int psg_set_v2(psg_ctx *ctx, double v2);
 /* replace the CN tables (NUZN, NU1; nkt doubles each, 3D only) and the mask L
  * (nkt ints) with caller-built ones, as the reference receives them from its
  * Fortran driver (3D_NavierStokes/src/cuda.cu:24-25,39).  Any pointer may be NULL. */
@@ -132,11 +136,18 @@ int psg_device_count(void);
  * variants 0..1:
  *   0  the three fields' inputs front-loaded
  *   1  each field loaded just before its transform (fewer registers)
+ * y-inverse, x-pass and y-forward of the 3D Navier-Stokes and Euler step as
+ * slabs of z planes (PSG_PASS_SLAB), variants 0..5:
+ *   0  each pass over the whole grid in one launch
+ *   v  the three passes over 128 >> (v-1) planes at a time (128, 64, 32, 16, 8),
+ *      so that a slab's intermediate buffers are read back from the device's
+ *      last-level cache instead of HBM
  * psg_autotune times `steps` steps of every variant of each pass in turn on the
  * context's own state (restored afterwards), keeps the fastest of each and
  * writes a text report. */
 #define PSG_PASS_XPASS 0
 #define PSG_PASS_ZFWD 1
+#define PSG_PASS_SLAB 2
 /* Step graphs: psg_run launches its first step directly, then replays a
  * captured HIP graph of `steps` steps (default 16) as often as fits, then
  * launches the remainder directly; results are bitwise those of direct

@@ -71,6 +71,25 @@ void psg_newton_defaults(int flavour, psg_newton_opts *o);
 /* z (nf*nkt complex, host) in: guess, out: solution; T, sx, sz in/out */
 int psg_newton_solve(psg_ctx *ctx, double *z, double *T, double *sx, double *sz, const psg_newton_opts *o,
                      psg_newton_result *res);
+/* Pseudo-arclength continuation in v2 = 1/Re (the reference's arclcFLAG2
+ * Newton branches, 2D_NavierStokes/src_RK4/src_gmres/main.F90:805-812,
+ * 1049-1062, 1123-1155, 1350-1366, 1441-1530, 1678, 1720-1726): v2 is a
+ * further unknown and <dX/dr, X - X_r> = ds a further row, with
+ * X = (z, T, sx, v2) and the inner product of the Newton vectors.  The
+ * caller (the continuation driver) supplies the tangent dX/dr and the last
+ * point X_r of the branch; Jv in v2 is the finite difference of the
+ * reference (jv = 0) or, with the tangent-linear Jv, a central difference
+ * S dPhi/dv2 taken once per Newton step.  Acceptance compares
+ * |b+| = sqrt(|F+|^2 + Narc+^2) with sqrt(|F|^2 + Narc^2) (main.F90:1539). */
+typedef struct {
+  const double *dzdr;  /* host, nf*nkt complex: dZr/dr */
+  double dTdr, dsdr, dv2dr;
+  const double *zr;    /* host, nf*nkt complex: the branch point Zr */
+  double Tr, sr, v2r;
+  double ds;           /* arclength step Ndelta_r */
+} psg_alc;
+int psg_newton_solve_alc(psg_ctx *ctx, double *z, double *T, double *sx, double *v2, const psg_alc *alc,
+                         const psg_newton_opts *o, psg_newton_result *res);
 /* Phi_T with the solver's time policy: z (host, in/out) advanced by T */
 int psg_flow_map(psg_ctx *ctx, double *z, double T, double dt0, int time_policy);
 
@@ -79,12 +98,64 @@ int psg_flow_map(psg_ctx *ctx, double *z, double T, double dt0, int time_policy)
 int psg_tangent(psg_ctx *ctx, const double *z, const double *dz0, double T, double dt0, int time_policy, double *z1,
                 double *dz1);
 
+/* ---------- Floquet (stability) analysis of a periodic orbit ----------
+ * Leading eigenvalues mu of the monodromy operator A q = S_s dPhi_T(z) q
+ * (the Jacobian action of the Newton residual without its -q term), by
+ * Arnoldi iteration on the GPU with FP64 masked real inner products.
+ * Replaces the reference's ARPACK dnaupd reverse-communication loop over the
+ * GMRES Jv code (2D_NavierStokes/src_RK4/src_gmres/main.F90:381-400 setup,
+ * 859-1037 iteration and output, 1084-1100 the Jv it feeds; defaults nev = 10,
+ * ncv = 200, TOL = 1e-7, which = 'LR', main.F90:63-67,388).  The reference
+ * restarts implicitly (ARPACK's IRAM, exact shifts); here a cycle of ncv
+ * Arnoldi steps is restarted explicitly from the wanted Ritz vectors, and Jv
+ * may come from the tangent-linear stepper.  Ritz values come from the
+ * Hessenberg matrix by Francis double-shift QR (psg_hess_eig), Ritz vectors
+ * by inverse iteration; a pair converges when |h_{m+1,m}| |y_m| <= tol |mu|.
+ * Floquet exponents are log|mu|/T and atan2(mu_im, mu_re)/T
+ * (main.F90:929-930). */
+enum { PSG_FLOQUET_LR = 0, PSG_FLOQUET_LM = 1 };
+typedef struct {
+  int nev;            /* wanted multipliers */
+  int ncv;            /* Arnoldi steps per cycle (nev + 2 .. 400) */
+  int max_restarts;   /* cycles after the first */
+  double tol;         /* relative Ritz residual tolerance */
+  int which;          /* PSG_FLOQUET_LR: largest real part; PSG_FLOQUET_LM: largest magnitude */
+  int jv;             /* 0 finite difference (eps), 1 tangent linear, 2 tangent linear in FP32-bulk passes */
+  double eps;         /* finite-difference epsilon, 1e-7 */
+  int time_policy;    /* PSG_TIME_* */
+  double dt0;
+  double sy;          /* fixed y shift */
+  unsigned long long seed;  /* start vector */
+  int verbose;
+} psg_floquet_opts;
+
+typedef struct {
+  int nconv;          /* wanted pairs that met tol */
+  int matvecs;        /* Jacobian actions (flows over T) */
+  int cycles;
+  double seconds;
+} psg_floquet_result;
+
+void psg_floquet_defaults(psg_floquet_opts *o);
+/* z (host, nf*nkt complex): the orbit; T, sx, sz its period and shifts.
+ * mu_re, mu_im, err: nev entries, ordered by `which`, err = |h_{m+1,m}| |y_m|
+ * (absolute Ritz residual).  vecs (may be NULL): nev host vectors of nf*nkt
+ * complex, the Ritz vector of a real mu, and for a complex pair (mu, conj mu)
+ * its real part then its imaginary part (the dneupd convention). */
+int psg_floquet(psg_ctx *ctx, const double *z, double T, double sx, double sz, const psg_floquet_opts *o,
+                double *mu_re, double *mu_im, double *err, double *vecs, psg_floquet_result *res);
+
 /* Dense helpers exposed for tests: one-sided Jacobi SVD of a column-major
  * m x n (m >= n) matrix A: U (m x n), s (n, descending), V (n x n); and the
  * hookstep trust-region search on (s, p): x_i = p_i s_i/(mu + s_i^2) with
  * Delta/sqrt2 < |x| < sqrt2 Delta (main.F90:1283-1333). Returns mu. */
 int psg_svd(int m, int n, const double *A, double *U, double *s, double *V);
 double psg_hook_mu(int k, const double *s, const double *p, double delta, double *x);
+/* eigenvalues (wr + i wi) of the n x n upper Hessenberg matrix H (column-major,
+ * entries below the subdiagonal ignored) by Francis double-shift QR; complex
+ * pairs are adjacent, positive imaginary part first.  0, or -1 if the
+ * iteration does not converge. */
+int psg_hess_eig(int n, const double *H, double *wr, double *wi);
 
 #ifdef __cplusplus
 }

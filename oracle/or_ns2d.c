@@ -178,19 +178,22 @@ void or2_nonlin(or2_ctx *c, const orc *z, orc *nz_) {
 }
 
 void or2_diag(const or2_ctx *c, const orc *z, double *energy, double *enstrophy, double *spec) {
-  double e = 0.0, v = 0.0;
-  if (spec) memset(spec, 0, sizeof(double) * (c->g.kty + 2));
+  /* out.F90's sums in compensated arithmetic */
+  or_ksum e = {0, 0}, v = {0, 0};
+  const int nb = c->g.kty + 2;
+  or_ksum *bins = (or_ksum *)calloc((size_t)nb, sizeof(or_ksum));
   for (long ik = 0; ik < c->g.nkt; ik++) {
     if (!c->L[ik]) continue;
     double wk = c->kx[ik] * c->kx[ik] + c->ky[ik] * c->ky[ik];
     double vz = z[ik].re * z[ik].re + z[ik].im * z[ik].im;
-    e += vz / wk;
-    v += vz;
-    if (spec) {
-      int j = (int)(sqrt(wk) + 0.5);
-      if (j >= 0 && j <= c->g.kty + 1) spec[j] += vz / wk;
-    }
+    or_kadd(&e, vz / wk);
+    or_kadd(&v, vz);
+    int j = (int)(sqrt(wk) + 0.5);
+    if (j >= 0 && j < nb) or_kadd(&bins[j], vz / wk);
   }
-  *energy = e;
-  *enstrophy = 2.0 * v;
+  if (spec)
+    for (int j = 0; j < nb; j++) spec[j] = or_kval(&bins[j]);
+  free(bins);
+  *energy = or_kval(&e);
+  *enstrophy = 2.0 * or_kval(&v);
 }

@@ -300,8 +300,10 @@ void or3_nonlin(or3_ctx *c, const orc *xii, const orc *eta, const orc *zet,
 
 void or3_diag(const or3_ctx *c, const orc *xii, const orc *eta, const orc *zet,
               double *energy, double *enstrophy, double *spec) {
-  double e = 0.0, z = 0.0;
-  if (spec) memset(spec, 0, sizeof(double) * (c->g.kty + 2));
+  /* the sums of computeStats in compensated arithmetic */
+  or_ksum e = {0, 0}, z = {0, 0};
+  const int nb = c->g.kty + 2;
+  or_ksum *bins = (or_ksum *)calloc((size_t)nb, sizeof(or_ksum));
   for (long ik = 0; ik < c->g.nkt; ik++) {
     if (c->L[ik] != 1) continue;
     double kk = c->kx[ik] * c->kx[ik] + c->ky[ik] * c->ky[ik] + c->kz[ik] * c->kz[ik];
@@ -309,13 +311,14 @@ void or3_diag(const or3_ctx *c, const orc *xii, const orc *eta, const orc *zet,
     double vz = xii[ik].re * xii[ik].re + xii[ik].im * xii[ik].im +
                 eta[ik].re * eta[ik].re + eta[ik].im * eta[ik].im +
                 zet[ik].re * zet[ik].re + zet[ik].im * zet[ik].im;
-    z += vz;
-    e += vz / wk;
-    if (spec) {
-      int j = (int)(sqrt(kk) + 0.5);
-      if (j >= 0 && j <= c->g.kty + 1) spec[j] += vz / wk;
-    }
+    or_kadd(&z, vz);
+    or_kadd(&e, vz / wk);
+    int j = (int)(sqrt(kk) + 0.5);
+    if (j >= 0 && j < nb) or_kadd(&bins[j], vz / wk);
   }
-  *energy = e;
-  *enstrophy = z;
+  if (spec)
+    for (int j = 0; j < nb; j++) spec[j] = or_kval(&bins[j]);
+  free(bins);
+  *energy = or_kval(&e);
+  *enstrophy = or_kval(&z);
 }

@@ -32,6 +32,19 @@ extern "C" {
 
 typedef struct { double re, im; } orc;
 
+/* Neumaier (Kahan-Babuska) compensated sum.  The diagnostics add up to
+ * ~2e7 terms per field; a plain running sum loses up to n*eps of relative
+ * accuracy there (1e-13 at 256^3), which would swamp a 1e-12 comparison. */
+typedef struct { double s, c; } or_ksum;
+static inline void or_kadd(or_ksum *k, double x) {
+  const double t = k->s + x;
+  const double as = k->s < 0 ? -k->s : k->s, ax = x < 0 ? -x : x;
+  if (as >= ax) k->c += (k->s - t) + x;
+  else k->c += (x - t) + k->s;
+  k->s = t;
+}
+static inline double or_kval(const or_ksum *k) { return k->s + k->c; }
+
 /* ---------------- FFT (or_fft.c) ---------------- */
 /* In-place complex FFT of length n (power of two), unnormalised.
  * sign = -1: forward  X_k = sum x_j e^{-2 pi i jk/n}

@@ -59,6 +59,16 @@ class NewtonResult(C.Structure):
                 ("hook_iters", C.c_int), ("status", C.c_int), ("seconds", C.c_double), ("phi_evals", C.c_long)]
 
 
+class FloquetOpts(C.Structure):
+    _fields_ = [("nev", C.c_int), ("ncv", C.c_int), ("max_restarts", C.c_int), ("tol", C.c_double),
+                ("which", C.c_int), ("jv", C.c_int), ("eps", C.c_double), ("time_policy", C.c_int),
+                ("dt0", C.c_double), ("sy", C.c_double), ("seed", C.c_ulonglong), ("verbose", C.c_int)]
+
+
+class FloquetResult(C.Structure):
+    _fields_ = [("nconv", C.c_int), ("matvecs", C.c_int), ("cycles", C.c_int), ("seconds", C.c_double)]
+
+
 NEWTON_STATUS = {0: "CONVERGED", 1: "GMRES FAILED", 2: "H-STEP FAILED", 3: "T-STEP FAILED", 4: "NEWTON STALLED",
                  5: "PERIOD TOO SMALL", 6: "NEWTON FAILED"}
 
@@ -139,6 +149,9 @@ def lib():
     L.psg_svd.argtypes = [_I, _I, _P, _P, _P, _P]
     L.psg_hook_mu.argtypes = [_I, _P, _P, _D, _P]
     L.psg_hook_mu.restype = _D
+    L.psg_hess_eig.argtypes = [_I, _P, _P, _P]
+    L.psg_floquet.argtypes = [_P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P]
+    L.psg_floquet_defaults.argtypes = [_P]
     L.psg_newton_defaults.argtypes = [_I, C.POINTER(NewtonOpts)]
     L.psg_newton_solve.argtypes = [_P, _P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(NewtonOpts),
                                    C.POINTER(NewtonResult)]
@@ -408,6 +421,20 @@ class Stepper:
         res = {f: getattr(r, f) for f, _ in NewtonResult._fields_}
         res["status_name"] = NEWTON_STATUS.get(r.status, "?")
         return z, res
+
+    def floquet(self, orbit, T, sx=0.0, sz=0.0, vectors=False, **opts):
+        """Leading Floquet multipliers of S_s Phi_T at the orbit: (mu, err, vecs or None, result dict)."""
+        o = FloquetOpts()
+        lib().psg_floquet_defaults(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        z = np.ascontiguousarray(orbit, dtype=np.complex128).reshape(self.nf, self.nkt)
+        mr, mi, er = np.zeros(o.nev), np.zeros(o.nev), np.zeros(o.nev)
+        vecs = np.zeros((o.nev, self.nf, self.nkt), np.complex128) if vectors else None
+        r = FloquetResult()
+        _check(lib().psg_floquet(self.h, _ptr(z), T, sx, sz, C.byref(o), _ptr(mr), _ptr(mi), _ptr(er),
+                                 _ptr(vecs) if vectors else None, C.byref(r)), "psg_floquet")
+        return mr + 1j * mi, er, vecs, {f: getattr(r, f) for f, _ in FloquetResult._fields_}
 
     def set_graph(self, steps):
         """Steps per captured HIP graph in run() (0: direct launches only)."""

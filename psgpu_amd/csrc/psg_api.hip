@@ -85,7 +85,14 @@ __global__ void k_diag(const cpx<double> *__restrict__ st, long fs, int nf, cons
   double *bins = sh + 2 * blockDim.x;
   for (int i = threadIdx.x; i < nbins; i += blockDim.x) bins[i] = 0.0;
   __syncthreads();
-  double e = 0.0, z = 0.0;
+  // per-thread sums compensated (Neumaier), so that the total over up to 2e7
+  // modes carries tree-reduction rounding only
+  double e = 0.0, z = 0.0, ec = 0.0, zc = 0.0;
+  auto kadd = [](double &s, double &c, double x) {
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  };
   for (long ik = blockIdx.x * (long)blockDim.x + threadIdx.x; ik < g.nkt; ik += (long)gridDim.x * blockDim.x) {
     if (L[ik] != 1) continue;
     const int ikx = (int)(ik % g.iktx);
@@ -100,13 +107,13 @@ __global__ void k_diag(const cpx<double> *__restrict__ st, long fs, int nf, cons
       cpx<double> a = st[f * fs + ik];
       vz += a.x * a.x + a.y * a.y;
     }
-    z += vz;
-    e += vz / wk;
+    kadd(z, zc, vz);
+    kadd(e, ec, vz / wk);
     int j = (int)(sqrt(kk) + 0.5);
     if (j < nbins) atomicAdd(&bins[j], vz / wk);
   }
-  sh[threadIdx.x] = e;
-  sh[blockDim.x + threadIdx.x] = z;
+  sh[threadIdx.x] = e + ec;
+  sh[blockDim.x + threadIdx.x] = z + zc;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
@@ -120,31 +127,48 @@ __global__ void k_diag(const cpx<double> *__restrict__ st, long fs, int nf, cons
   for (int i = threadIdx.x; i < nbins; i += blockDim.x) o[2 + i] = bins[i];
 }
 
+// blocks of k_diag and its launch: partial rows of 2 + KTY + 2 doubles into part
+static int diag_blocks(const Ctx *c) { return (int)std::min<long>(256, (c->g.nkt + 255) / 256); }
+static int launch_diag(Ctx *c, int member, double *part) {
+  const Grid &g = c->g;
+  const int nbins = g.kty + 2;
+  const int nfv = c->cfg.flavour == PSG_BOUS3D ? 3 : (c->g.dim == 3 ? 3 : 1);
+  size_t sh = (2 * 256 + nbins) * sizeof(double);
+  hipLaunchKernelGGL(k_diag, dim3(diag_blocks(c)), dim3(256), sh, c->stream,
+                     c->state + (long)member * c->nf * g.nkt, g.nkt, nfv, c->mask, g, nbins, part);
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
 int diag_member(Ctx *c, int member, double *e, double *z, double *spec) {
   const Grid &g = c->g;
   const int nbins = g.kty + 2;
-  const int blocks = (int)std::min<long>(256, (g.nkt + 255) / 256);
+  const int blocks = diag_blocks(c);
   const size_t need = (size_t)blocks * (2 + nbins);
   if (c->dred_n < need) {
     if (c->dred) hipFree(c->dred);
     PSG_HIP(hipMalloc(&c->dred, need * sizeof(double)));
     c->dred_n = need;
   }
-  const int nfv = c->cfg.flavour == PSG_BOUS3D ? 3 : (c->g.dim == 3 ? 3 : 1);
-  size_t sh = (2 * 256 + nbins) * sizeof(double);
-  hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), sh, c->stream, c->state + (long)member * c->nf * g.nkt,
-                     g.nkt, nfv, c->mask, g, nbins, c->dred);
-  PSG_HIP(hipGetLastError());
+  if (int r = launch_diag(c, member, c->dred)) return r;
   std::vector<double> h(need);
   PSG_HIP(hipMemcpyAsync(h.data(), c->dred, need * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PSG_HIP(hipStreamSynchronize(c->stream));
-  double E = 0, Z = 0;
-  std::vector<double> sp(nbins, 0.0);
+  double E = 0, Z = 0, Ec = 0, Zc = 0;
+  auto kadd = [](double &s, double &c, double x) {
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  };
+  std::vector<double> sp(nbins, 0.0), spc(nbins, 0.0);
   for (int b = 0; b < blocks; b++) {
-    E += h[(size_t)b * (2 + nbins)];
-    Z += h[(size_t)b * (2 + nbins) + 1];
-    for (int i = 0; i < nbins; i++) sp[i] += h[(size_t)b * (2 + nbins) + 2 + i];
+    kadd(E, Ec, h[(size_t)b * (2 + nbins)]);
+    kadd(Z, Zc, h[(size_t)b * (2 + nbins) + 1]);
+    for (int i = 0; i < nbins; i++) kadd(sp[i], spc[i], h[(size_t)b * (2 + nbins) + 2 + i]);
   }
+  E += Ec;
+  Z += Zc;
+  for (int i = 0; i < nbins; i++) sp[i] += spc[i];
   if (g.dim == 2) Z *= 2.0;
   if (e) *e = E;
   if (z) *z = Z;
@@ -517,6 +541,14 @@ int psg_set_dt(psg_ctx *c, double dt) {
   return upload_tables(c);
 }
 
+int psg_set_v2(psg_ctx *c, double v2) {
+  if (!c || !(v2 > 0.0)) { set_error("invalid v2"); return PSG_EINVAL; }
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  c->cfg.v2 = v2;
+  c->graph_gen++;   // the 2D update holds v2 by value
+  return upload_tables(c);
+}
+
 int psg_set_tables(psg_ctx *c, const double *nuzn, const double *nu1, const int *L) {
   if (!c) { set_error("null context"); return PSG_EINVAL; }
   const long n = c->g.nkt;
@@ -610,12 +642,14 @@ void *psg_stream(psg_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 int psg_set_variant(psg_ctx *c, int pass, int variant) {
   if (!c) return PSG_EINVAL;
-  const int nv = pass == PSG_PASS_XPASS ? PSG_XPASS_VARIANTS : (pass == PSG_PASS_ZFWD ? PSG_ZFWD_VARIANTS : 0);
+  const int nv = pass == PSG_PASS_XPASS ? PSG_XPASS_VARIANTS
+                 : pass == PSG_PASS_ZFWD ? PSG_ZFWD_VARIANTS
+                 : pass == PSG_PASS_SLAB ? PSG_SLAB_VARIANTS : 0;
   if (variant < 0 || variant >= nv) {
     set_error("psg_set_variant: unknown pass %d or variant %d", pass, variant);
     return PSG_EINVAL;
   }
-  (pass == PSG_PASS_XPASS ? c->xvar : c->zvar) = variant;
+  (pass == PSG_PASS_XPASS ? c->xvar : pass == PSG_PASS_ZFWD ? c->zvar : c->slvar) = variant;
   c->graph_gen++;
   return 0;
 }
@@ -629,8 +663,8 @@ int psg_set_graph(psg_ctx *c, int steps) {
 }
 
 int psg_get_variant(psg_ctx *c, int pass) {
-  if (!c || (pass != PSG_PASS_XPASS && pass != PSG_PASS_ZFWD)) return PSG_EINVAL;
-  return pass == PSG_PASS_XPASS ? c->xvar : c->zvar;
+  if (!c || (pass != PSG_PASS_XPASS && pass != PSG_PASS_ZFWD && pass != PSG_PASS_SLAB)) return PSG_EINVAL;
+  return pass == PSG_PASS_XPASS ? c->xvar : pass == PSG_PASS_ZFWD ? c->zvar : c->slvar;
 }
 
 int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
@@ -653,7 +687,8 @@ int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
     int rc = 0;
     // passes are tuned one after the other, each with the others' choices fixed
     struct { const char *name; int *var; int n; } passes[] = {{"xpass", &c->xvar, PSG_XPASS_VARIANTS},
-                                                                {"zfwd", &c->zvar, PSG_ZFWD_VARIANTS}};
+                                                                {"zfwd", &c->zvar, PSG_ZFWD_VARIANTS},
+                                                                {"slab", &c->slvar, PSG_SLAB_VARIANTS}};
     for (auto &ps : passes) {
       double best = 1e300;
       int bestv = *ps.var;

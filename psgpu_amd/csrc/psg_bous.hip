@@ -77,7 +77,7 @@ k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__r
     if (valid) {
       cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz;
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) o[zoff(g, ikx, iky, j + TP * mm)] = v[mm];
+      for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_ZINV>::st>(&o[zoff<P>(g, ikx, iky, j + TP * mm)], v[mm]);
     }
   }
 }
@@ -210,12 +210,12 @@ static int launch_zinv_b(Ctx *c, const cpx<double> *st, long st_ms, int umask, i
   const Grid &g = c->g;
   cpx<P> *A = (cpx<P> *)c->bufA;
   PSG_DISPATCH_N(g.nz, {
-    constexpr int T = Col<N>::T;
+    constexpr int T = Col<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_zinv_bous<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, A, 9 * g.zsz, g,
+    hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, g.nkt, A, 9 * g.zsz, g,
                        twp<P>(c, 2), c->mask, umask, nout);
   });
   PSG_HIP(hipGetLastError());
@@ -223,12 +223,12 @@ static int launch_zinv_b(Ctx *c, const cpx<double> *st, long st_ms, int umask, i
 }
 
 // generic column/row passes shared with the Navier-Stokes path
-template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
-template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
+template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
+template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
 template <typename P> int launch_xphys(Ctx *c, const cpx<P> *in, double *out, int nf);
 
 template <typename P, int FI, int FO, class PROD>
-static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
+static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod, int zs0 = 0) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nx, {
     if (g.iktx != xband<N>()) {
@@ -246,7 +246,7 @@ static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, in
     auto k = k_xpass<P, N, TE, FI, FO, PROD>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
-                       prod);
+                       prod, zs0);
   });
   PSG_HIP(hipGetLastError());
   return 0;
@@ -255,12 +255,12 @@ static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, in
 template <typename P, class UPD> static int launch_zfwd_b(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nz, {
-    constexpr int T = Col<N>::T;
+    constexpr int T = Col<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_zfwd<P, N, T, 4, UPD>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);
+    hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);
   });
   PSG_HIP(hipGetLastError());
   return 0;

@@ -27,6 +27,7 @@ struct Grid {
 enum { PREC64 = 0, PREC32 = 1 };
 constexpr int PSG_XPASS_VARIANTS = 12;
 constexpr int PSG_ZFWD_VARIANTS = 2;
+constexpr int PSG_SLAB_VARIANTS = 6;
 
 struct ProfEntry {
   std::string name;
@@ -82,6 +83,7 @@ struct Ctx {
   // kernel shape variants (psg_autotune / psg_set_variant)
   int xvar = 0;    // Navier-Stokes x-pass: 0..PSG_XPASS_VARIANTS-1
   int zvar = 0;    // Navier-Stokes z-forward/update: 0..PSG_ZFWD_VARIANTS-1
+  int slvar = 0;   // Navier-Stokes passes 2-4 by z slabs: 0 whole grid, v >= 1: 128 >> (v-1) planes
   // diagnostics scratch
   double *dred = nullptr;
   size_t dred_n = 0;
@@ -99,6 +101,12 @@ struct Ctx {
   long prof_cur_bytes = 0;
   int prof_cur_e0 = -1;
 };
+
+// z planes per slab of passes 2-4 (the whole grid when slabs are off)
+inline int slab_planes(const Ctx *c) {
+  const int s = c->slvar <= 0 ? c->g.nz : (128 >> (c->slvar - 1));
+  return s < c->g.nz ? s : c->g.nz;
+}
 
 // error helpers (defined in psg_api.hip)
 void set_error(const char *fmt, ...);

@@ -6,11 +6,18 @@
 
 namespace psg {
 
-// column tile: T kx columns x N/8 threads per column
-template <int N> struct Col {
+// column tile: T kx columns x N/8 threads per column; at least 8 columns
+// (128 B of FP64 per pencil element row), and in FP32 at least TMIN32
+// columns, capped at 1024 threads
+#ifndef PSG_COLT32
+#define PSG_COLT32 8
+#endif
+template <int N, typename P = double> struct Col {
   static constexpr int TP = N / EPT;
   static constexpr int T0 = 256 / TP;
-  static constexpr int T = T0 < 8 ? 8 : (T0 > 32 ? 32 : T0);
+  static constexpr int TMIN = sizeof(P) == 4 ? PSG_COLT32 : 8;
+  static constexpr int T1 = T0 < TMIN ? TMIN : (T0 > 32 ? 32 : T0);
+  static constexpr int T = T1 * TP > 1024 ? 1024 / TP : T1;
   static constexpr int THREADS = T * TP;
 };
 // x-pass row teams

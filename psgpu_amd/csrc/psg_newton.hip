@@ -6,7 +6,9 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
 #include <chrono>
+#include <complex>
 #include <vector>
 #include "psg_launch.hpp"
 #include "../../include/psg_newton.h"
@@ -27,6 +29,31 @@ __global__ void k_dot(const cpx<double> *__restrict__ a, const cpx<double> *__re
   double s = 0.0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     if (L[i % nkt]) s += a[i].x * b[i].x + a[i].y * b[i].y;
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// masked real product over the modes with kx^2 + ky^2 + kz^2 <= kmax2 only
+// (the low-wavenumber residual NER2, 2D_NavierStokes/src_RK4/src_gmres/main.F90:737-751)
+__global__ void k_dot_low(const cpx<double> *__restrict__ a, const cpx<double> *__restrict__ b, Grid g,
+                          double kmax2, const int8_t *__restrict__ L, long n, double *__restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long ik = i % g.nkt;
+    if (!L[ik]) continue;
+    const int ikx = (int)(ik % g.iktx);
+    const long t = ik / g.iktx;
+    const int iky = (int)(t % g.ikty), ikz = (int)(t / g.ikty);
+    const double kx = ikx * g.alpha, ky = iky - g.kty, kz = g.dim == 3 ? (double)(ikz - g.ktz) : 0.0;
+    if (kx * kx + ky * ky + kz * kz > kmax2) continue;
+    s += a[i].x * b[i].x + a[i].y * b[i].y;
+  }
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int k = blockDim.x / 2; k > 0; k >>= 1) {
@@ -94,6 +121,14 @@ struct VecOps {
   ~VecOps() { if (dpart) hipFree(dpart); }
   double dot(const cpx<double> *a, const cpx<double> *b) {
     hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(256), 0, c->stream, a, b, c->mask, c->g.nkt, n, dpart);
+    hipMemcpyAsync(hpart.data(), dpart, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    double s = 0.0;
+    for (int i = 0; i < RED_BLOCKS; i++) s += hpart[i];
+    return s;
+  }
+  double dot_low(const cpx<double> *a, const cpx<double> *b, double kmax2) {
+    hipLaunchKernelGGL(k_dot_low, dim3(RED_BLOCKS), dim3(256), 0, c->stream, a, b, c->g, kmax2, c->mask, n, dpart);
     hipMemcpyAsync(hpart.data(), dpart, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, c->stream);
     hipStreamSynchronize(c->stream);
     double s = 0.0;
@@ -232,7 +267,183 @@ static double hook_mu(int k, const double *S, const double *P, double delta, dou
 
 using namespace psg;
 
+// ------------------------------------------------------------------ Hessenberg eigenvalues
+// Householder reflector P = I - beta v v^T with v[0] = 1 mapping x (length m,
+// 2 or 3) onto a multiple of e_1 (Golub & Van Loan, Matrix Computations,
+// Alg. 5.1.1).
+static void house(const double *x, int m, double *v, double &beta) {
+  double sigma = 0.0;
+  for (int i = 1; i < m; i++) sigma += x[i] * x[i];
+  v[0] = 1.0;
+  for (int i = 1; i < m; i++) v[i] = x[i];
+  if (sigma == 0.0) { beta = 0.0; return; }
+  const double mu = sqrt(x[0] * x[0] + sigma);
+  const double v0 = x[0] <= 0.0 ? x[0] - mu : -sigma / (x[0] + mu);
+  beta = 2.0 * v0 * v0 / (sigma + v0 * v0);
+  for (int i = 1; i < m; i++) v[i] /= v0;
+}
+
+// Eigenvalues of an n x n upper Hessenberg matrix h (row-major, overwritten)
+// by Francis double-shift QR steps with bulge chasing (Golub & Van Loan,
+// Alg. 7.5.1-7.5.2), deflating at negligible subdiagonal entries and taking
+// the exceptional shifts of EISPACK's HQR after 10 and 20 fruitless steps.
+// Complex pairs are stored adjacently, positive imaginary part first.
+static int hess_eig(int n, double *h, double *wr, double *wi) {
+  auto H = [&](int i, int j) -> double & { return h[(size_t)i * n + j]; };
+  double anorm = 0.0;
+  for (int i = 0; i < n; i++)
+    for (int j = (i > 0 ? i - 1 : 0); j < n; j++) anorm += fabs(H(i, j));
+  const double eps = 2.220446049250313e-16;
+  if (!(anorm == anorm) || anorm > 1e300) return -1;   // NaN or overflow in the input
+  double tshift = 0.0;   // accumulated exceptional shifts, added back to every later eigenvalue
+  int hi = n - 1, iter = 0, total = 0;
+  const int itmax = 30 * (n > 10 ? n : 10);               // the whole reduction's budget, as LAPACK's dlahqr
+  while (hi >= 0) {
+    int lo = hi;
+    for (; lo > 0; lo--) {
+      double s = fabs(H(lo - 1, lo - 1)) + fabs(H(lo, lo));
+      if (s == 0.0) s = anorm;
+      if (fabs(H(lo, lo - 1)) <= eps * s) { H(lo, lo - 1) = 0.0; break; }
+    }
+    if (lo == hi) {                       // 1 x 1 block
+      wr[hi] = H(hi, hi) + tshift;
+      wi[hi] = 0.0;
+      hi--;
+      iter = 0;
+      continue;
+    }
+    if (lo == hi - 1) {                   // 2 x 2 block: its two roots, without cancellation
+      const double a = H(hi - 1, hi - 1), b = H(hi - 1, hi), c = H(hi, hi - 1), d = H(hi, hi);
+      const double p = 0.5 * (a - d), bc = b * c, disc = p * p + bc;
+      if (disc >= 0.0) {
+        const double z = p + copysign(sqrt(disc), p);
+        wr[hi - 1] = d + z + tshift;
+        wr[hi] = (z != 0.0 ? d - bc / z : d) + tshift;
+        wi[hi - 1] = wi[hi] = 0.0;
+      } else {
+        wr[hi - 1] = wr[hi] = d + p + tshift;
+        wi[hi - 1] = sqrt(-disc);
+        wi[hi] = -wi[hi - 1];
+      }
+      hi -= 2;
+      iter = 0;
+      continue;
+    }
+    if (++total > itmax) return -1;
+    // the double shift as the sum s and product t of the two shifts; an
+    // exceptional one after every 10 steps without a deflation
+    double s, t;
+    if (iter > 0 && iter % 10 == 0) {
+      const double x = H(hi, hi);
+      tshift += x;
+      for (int i = 0; i <= hi; i++) H(i, i) -= x;
+      const double e = fabs(H(hi, hi - 1)) + fabs(H(hi - 1, hi - 2));
+      s = 1.5 * e;
+      t = e * e;
+    } else {
+      s = H(hi - 1, hi - 1) + H(hi, hi);
+      t = H(hi - 1, hi - 1) * H(hi, hi) - H(hi - 1, hi) * H(hi, hi - 1);
+    }
+    iter++;
+    // first column of (H - s1)(H - s2) on the active block, then chase the bulge
+    double x[3] = {H(lo, lo) * H(lo, lo) + H(lo, lo + 1) * H(lo + 1, lo) - s * H(lo, lo) + t,
+                   H(lo + 1, lo) * (H(lo, lo) + H(lo + 1, lo + 1) - s), H(lo + 1, lo) * H(lo + 2, lo + 1)};
+    for (int k = lo; k <= hi - 2; k++) {
+      double v[3], beta;
+      house(x, 3, v, beta);
+      const int c0 = k > lo ? k - 1 : lo;
+      for (int j = c0; j <= hi; j++) {
+        const double dd = beta * (H(k, j) + v[1] * H(k + 1, j) + v[2] * H(k + 2, j));
+        H(k, j) -= dd;
+        H(k + 1, j) -= dd * v[1];
+        H(k + 2, j) -= dd * v[2];
+      }
+      const int r1 = k + 3 < hi ? k + 3 : hi;
+      for (int i = lo; i <= r1; i++) {
+        const double dd = beta * (H(i, k) + v[1] * H(i, k + 1) + v[2] * H(i, k + 2));
+        H(i, k) -= dd;
+        H(i, k + 1) -= dd * v[1];
+        H(i, k + 2) -= dd * v[2];
+      }
+      x[0] = H(k + 1, k);
+      x[1] = H(k + 2, k);
+      x[2] = k < hi - 2 ? H(k + 3, k) : 0.0;
+    }
+    {
+      double v[2], beta;
+      house(x, 2, v, beta);
+      for (int j = hi - 2; j <= hi; j++) {
+        const double dd = beta * (H(hi - 1, j) + v[1] * H(hi, j));
+        H(hi - 1, j) -= dd;
+        H(hi, j) -= dd * v[1];
+      }
+      for (int i = lo; i <= hi; i++) {
+        const double dd = beta * (H(i, hi - 1) + v[1] * H(i, hi));
+        H(i, hi - 1) -= dd;
+        H(i, hi) -= dd * v[1];
+      }
+    }
+    // the reflectors zero the bulge exactly in exact arithmetic: keep the form exact
+    for (int i = lo + 2; i <= hi; i++)
+      for (int j = lo; j < i - 1; j++) H(i, j) = 0.0;
+  }
+  return 0;
+}
+
+// Eigenvector y (H y = mu y, |y| = 1) of the n x n Hessenberg matrix h
+// (row-major) for the eigenvalue mu, by inverse iteration on H - (mu + d) I
+// with complex LU and partial pivoting.
+static void hess_evec(int n, const double *h, std::complex<double> mu, std::vector<std::complex<double>> &y) {
+  typedef std::complex<double> Z;
+  double hn = 0.0;
+  for (int i = 0; i < n * n; i++) hn = std::max(hn, fabs(h[i]));
+  const Z sh = mu + Z(1e-10 * (std::abs(mu) + hn), 0.0);
+  std::vector<Z> A((size_t)n * n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) A[(size_t)i * n + j] = Z(h[(size_t)i * n + j], 0.0) - (i == j ? sh : Z(0.0, 0.0));
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    for (int i = k + 1; i < n; i++)
+      if (std::abs(A[(size_t)i * n + k]) > std::abs(A[(size_t)p * n + k])) p = i;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; j++) std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]);
+    Z d = A[(size_t)k * n + k];
+    if (std::abs(d) == 0.0) d = A[(size_t)k * n + k] = Z(1e-300, 0.0);
+    for (int i = k + 1; i < n; i++) {
+      const Z f = A[(size_t)i * n + k] / d;
+      A[(size_t)i * n + k] = f;
+      if (f != Z(0.0, 0.0))
+        for (int j = k + 1; j < n; j++) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+    }
+  }
+  y.assign(n, Z(1.0, 0.0));
+  for (int it = 0; it < 3; it++) {
+    for (int k = 0; k < n; k++) std::swap(y[k], y[piv[k]]);
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < i; j++) y[i] -= A[(size_t)i * n + j] * y[j];
+    for (int i = n - 1; i >= 0; i--) {
+      for (int j = i + 1; j < n; j++) y[i] -= A[(size_t)i * n + j] * y[j];
+      y[i] /= A[(size_t)i * n + i];
+    }
+    double nr = 0.0;
+    for (auto &a : y) nr += std::norm(a);
+    nr = sqrt(nr);
+    for (auto &a : y) a /= nr;
+  }
+}
+
 extern "C" {
+
+int psg_hess_eig(int n, const double *Hc, double *wr, double *wi) {
+  if (n < 1 || !Hc || !wr || !wi) { set_error("psg_hess_eig: bad argument"); return PSG_EINVAL; }
+  std::vector<double> h((size_t)n * n, 0.0);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i <= std::min(j + 1, n - 1); i++) h[(size_t)i * n + j] = Hc[(size_t)j * n + i];
+  if (hess_eig(n, h.data(), wr, wi)) { set_error("psg_hess_eig: QR iteration did not converge"); return -1; }
+  return 0;
+}
 
 int psg_svd(int m, int n, const double *A, double *U, double *s, double *V) {
   if (m < n || n < 1) return PSG_EINVAL;
@@ -310,9 +521,14 @@ struct TlPrecision {
   }
 };
 
-int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szio, const psg_newton_opts *o,
-                     psg_newton_result *res) {
+// The solver; with alc, v2 is a further unknown and the pseudo-arclength
+// condition <dX/dr, X - X_r> = ds a further row (the reference's arclcFLAG2
+// branches, 2D_NavierStokes/src_RK4/src_gmres/main.F90:805-812, 1049-1062,
+// 1123-1155, 1350-1366, 1441-1530, 1678, 1720-1726).
+static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szio, double *v2io,
+                       const psg_alc *alc, const psg_newton_opts *o, psg_newton_result *res) {
   if (!c || !zh || !Tio || !o || !res) { set_error("null argument"); return PSG_EINVAL; }
+  if (alc && (!v2io || !alc->dzdr || !alc->zr || !(*v2io > 0.0))) { set_error("newton: continuation needs v2, dzdr and zr"); return PSG_EINVAL; }
   if (c->batch != 1) { set_error("psg_newton_solve needs a batch-1 context"); return PSG_EINVAL; }
   if (o->jv < 0 || o->jv > 2) { set_error("newton: jv %d is not 0, 1 or 2", o->jv); return PSG_EINVAL; }
   if (o->jv == 2 && c->cfg.flavour != PSG_NS2D && c->cfg.flavour != PSG_NS3D) {
@@ -339,10 +555,12 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
   };
   cpx<double> *ZNEWT, *ZN, *ZNN, *ZNSH, *FN, *dZNdt, *dzdt, *dZNdS[2], *dzdS[2], *v, *DIFF, *Zplus, *FNplus, *Zplus2,
       *ZN2, *FNplus2, *work, *Q;
+  cpx<double> *dZrdr = nullptr, *Zr = nullptr, *dFdv = nullptr, *work2 = nullptr;
   bool ok = dalloc(&ZNEWT) && dalloc(&ZN) && dalloc(&ZNN) && dalloc(&ZNSH) && dalloc(&FN) && dalloc(&dZNdt) &&
             dalloc(&dzdt) && dalloc(&dZNdS[0]) && dalloc(&dZNdS[1]) && dalloc(&dzdS[0]) && dalloc(&dzdS[1]) &&
             dalloc(&v) && dalloc(&DIFF) && dalloc(&Zplus) && dalloc(&FNplus) && dalloc(&Zplus2) && dalloc(&ZN2) &&
             dalloc(&FNplus2) && dalloc(&work);
+  if (ok && alc) ok = dalloc(&dZrdr) && dalloc(&Zr) && dalloc(&dFdv) && dalloc(&work2);
   if (ok) {
     if (hipMalloc(&Q, bytes * (size_t)(K + 1)) != hipSuccess) ok = false;
     else bufs.push_back(Q);
@@ -353,8 +571,9 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
   };
   if (!ok) { cleanup(); set_error("newton: device allocation"); return PSG_ENOMEM; }
   auto q = [&](int i) { return Q + (size_t)i * n; };
-  std::vector<double> qN((size_t)(K + 1) * 3, 0.0);   // scalar parts: [T, sx, sz]
-  auto qn = [&](int i, int a) -> double & { return qN[(size_t)i * 3 + a]; };
+  constexpr int NQ = 4;   // scalar parts of a Krylov vector: [T, sx, sz, v2]
+  std::vector<double> qN((size_t)(K + 1) * NQ, 0.0);
+  auto qn = [&](int i, int a) -> double & { return qN[(size_t)i * NQ + a]; };
   std::vector<double> H((size_t)(K + 1) * K, 0.0);   // column-major (K+1) x K
   auto h = [&](int i, int j) -> double & { return H[(size_t)j * (K + 1) + i]; };
 
@@ -375,9 +594,26 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
     V.shift(in, out, sx, sy, ns > 1 ? sz : 0.0);
   };
   auto norm = [&](const cpx<double> *a) { return sqrt(V.dot(a, a)); };
+  // continuation: the context's viscosity follows the flow being computed
+  double v2N = v2io ? *v2io : c->cfg.v2;
+  auto at_v2 = [&](double vv) { return vv == c->cfg.v2 ? 0 : psg_set_v2(static_cast<psg_ctx *>(c), vv); };
+  auto narc_of = [&](const cpx<double> *Zz, double Tz, double Sz, double vz) {
+    V.copy(Zz, work2);
+    V.axpby(-1.0, Zr, 1.0, work2);
+    double r = V.dot(dZrdr, work2) + alc->dsdr * (Sz - alc->sr) + alc->dv2dr * (vz - alc->v2r) - alc->ds;
+    if (!o->fixed_period) r += alc->dTdr * (Tz - alc->Tr);
+    return r;
+  };
 
   NTRY(hipMemcpyAsync(work, zh, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess ? 0 : PSG_EHIP);
   V.copy(work, ZNEWT);
+  if (alc) {
+    NTRY(hipMemcpyAsync(work, alc->dzdr, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess ? 0 : PSG_EHIP);
+    V.copy(work, dZrdr);
+    NTRY(hipMemcpyAsync(work, alc->zr, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess ? 0 : PSG_EHIP);
+    V.copy(work, Zr);
+  }
+  NTRY(at_v2(v2N));
   NTRY(flow(c, ZNEWT, ZN, T, dt0, o->time_policy));
   phi++;
   V.copy(ZN, ZNN);
@@ -387,16 +623,37 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
   V.axpby(-1.0, ZNEWT, 1.0, FN);
   double NORMFN = norm(FN);
   double NER = NORMFN / NORMZ;
+  // second residual over |k|^2 <= KYmax2 = ((IKTY-KTY-1)^2)/4 (2D main.F90:124-125); the 3D
+  // solvers define none
+  const double kymax2 = (double)c->g.kty * (double)c->g.kty / 4.0;
+  auto ner2_of = [&](const cpx<double> *F) { return c->g.dim == 2 ? sqrt(V.dot_low(F, F, kymax2)) / NORMZ : -1.0; };
+  double NER2 = ner2_of(FN);
   double Delta = o->delta0, NERold = 0.0;
   int NERcount = 0;
   int status = PSG_NEWTON_FAILED, INE = 0;
   long gm_total = 0, hook_total = 0;
   if (o->verbose) { fprintf(stderr, "newton: start T=%.10g sx=%.6g NER=%.3e\n", T, sh[0], NER); fflush(stderr); }
   if (NER <= o->netol) status = PSG_NEWTON_CONVERGED;
+  double Narc = alc ? narc_of(ZNEWT, T, sh[0], v2N) : 0.0;
   for (INE = 1; INE <= o->max_newton && status != PSG_NEWTON_CONVERGED; INE++) {
-    const double NORMb = NORMFN;
+    const double NORMb = alc ? sqrt(NORMFN * NORMFN + Narc * Narc) : NORMFN;
     V.axpby(-1.0 / NORMb, FN, 0.0, q(0));
-    for (int a = 0; a < 3; a++) qn(0, a) = 0.0;
+    for (int a = 0; a < NQ; a++) qn(0, a) = 0.0;
+    if (alc) qn(0, 3) = -Narc / NORMb;
+    NTRY(at_v2(v2N));
+    if (alc && o->jv >= 1) {
+      // S dPhi/dv2 for the tangent-linear Jv, by a central difference in v2
+      const double hv = 1e-4 * v2N;
+      NTRY(at_v2(v2N + hv));
+      NTRY(flow(c, ZNEWT, dFdv, T, dt0, o->time_policy));
+      NTRY(at_v2(v2N - hv));
+      NTRY(flow(c, ZNEWT, work2, T, dt0, o->time_policy));
+      phi += 2;
+      V.axpby(-1.0, work2, 1.0, dFdv);
+      shift_of(dFdv, dFdv, sh[0], sh[1]);
+      V.axpby(0.0, dFdv, 0.5 / hv, dFdv);
+      NTRY(at_v2(v2N));
+    }
     if (!o->fixed_period) {
       NTRY(timederiv_dev(c, ZNSH, dZNdt));
       NTRY(timederiv_dev(c, ZNEWT, dzdt));
@@ -415,16 +672,25 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
       if (o->jv >= 1) {
         if ((r = tl_flow_map(c, ZNEWT, dz, work, out, T, dt0, o->time_policy, true))) return r;
         shift_of(out, out, sh[0], sh[1]);   // S dPhi dz
+        if (alc && dsc[3] != 0.0) V.axpby(dsc[3], dFdv, 1.0, out);
       } else {
         V.copy(ZNEWT, work);
         V.axpby(o->eps, dz, 1.0, work);
+        // the finite difference perturbs v2 along with z (main.F90:1049-1062)
+        if (alc && (r = at_v2(v2N + o->eps * dsc[3]))) return r;
         if ((r = flow(c, work, out, T, dt0, o->time_policy))) return r;
+        if (alc && (r = at_v2(v2N))) return r;
         V.axpby(-1.0, ZNN, 1.0, out);
         shift_of(out, out, sh[0], sh[1]);
         V.axpby(0.0, out, 1.0 / o->eps, out);
       }
       phi++;
       V.axpby(-1.0, dz, 1.0, out);
+      vn[3] = 0.0;
+      if (alc) {   // the arclength row (main.F90:1123-1126)
+        vn[3] = V.dot(dZrdr, dz) + alc->dsdr * dsc[1] + alc->dv2dr * dsc[3];
+        if (!o->fixed_period) vn[3] += alc->dTdr * dsc[0];
+      }
       int a = 0;
       if (!o->fixed_period) { V.axpby(dsc[0], dZNdt, 1.0, out); vn[0] = V.dot(dzdt, dz); a = 1; }
       else vn[0] = 0.0;
@@ -436,24 +702,26 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
       return 0;
     };
     for (k = 1; k <= K; k++) {
-      double vn[3] = {0, 0, 0};
-      double dsc[3] = {qn(k - 1, 0), qn(k - 1, 1), qn(k - 1, 2)};
+      double vn[NQ] = {0, 0, 0, 0};
+      double dsc[NQ] = {qn(k - 1, 0), qn(k - 1, 1), qn(k - 1, 2), qn(k - 1, 3)};
       NTRY(apply_J(q(k - 1), dsc, v, vn));
       for (int i = 0; i < k; i++) {
         double hij = V.dot(q(i), v);
         if (!o->fixed_period) hij += qn(i, 0) * vn[0];
         for (int s = 0; s < ns; s++) hij += qn(i, 1 + s) * vn[1 + s];
+        if (alc) hij += qn(i, 3) * vn[3];
         h(i, k - 1) = hij;
         V.axpby(-hij, q(i), 1.0, v);
-        for (int a = 0; a < 3; a++) vn[a] -= hij * qn(i, a);
+        for (int a = 0; a < NQ; a++) vn[a] -= hij * qn(i, a);
       }
       double hn = V.dot(v, v);
       if (!o->fixed_period) hn += vn[0] * vn[0];
       for (int s = 0; s < ns; s++) hn += vn[1 + s] * vn[1 + s];
+      if (alc) hn += vn[3] * vn[3];
       hn = sqrt(hn);
       h(k, k - 1) = hn;
       V.axpby(1.0 / hn, v, 0.0, q(k));
-      for (int a = 0; a < 3; a++) qn(k, a) = vn[a] / hn;
+      for (int a = 0; a < NQ; a++) qn(k, a) = vn[a] / hn;
       // SVD of H(0:k, 0:k-1)
       std::vector<double> Hk((size_t)(k + 1) * k);
       for (int j = 0; j < k; j++)
@@ -489,7 +757,7 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
     int IHOOK = 0;
     bool halve = false, dbl = false, hook_failed = false;
     double NORMFNplus = 0, NORMFNpred = 0, NORMFNplus2 = 0, Tplus = T, Tplus2 = T, Splus[2] = {sh[0], sh[1]},
-           Splus2[2] = {sh[0], sh[1]};
+           Splus2[2] = {sh[0], sh[1]}, v2plus = v2N, v2plus2 = v2N;
     for (;;) {
       IHOOK++;
       if (IHOOK > o->max_hook) { hook_failed = true; break; }
@@ -500,29 +768,39 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
         Y[i] = t;
       }
       V.axpby(0.0, DIFF, 0.0, DIFF);
-      double dsc[3] = {0, 0, 0};
+      double dsc[NQ] = {0, 0, 0, 0};
       for (int i = 0; i < k; i++) {
         V.axpby(Y[i], q(i), 1.0, DIFF);
-        for (int a = 0; a < 3; a++) dsc[a] += qn(i, a) * Y[i];
+        for (int a = 0; a < NQ; a++) dsc[a] += qn(i, a) * Y[i];
       }
       if (o->fixed_period) dsc[0] = 0.0;
+      if (!alc) dsc[3] = 0.0;
       // predicted residual F + J DIFF
-      double vn[3];
+      double vn[NQ];
       NTRY(apply_J(DIFF, dsc, v, vn));
       V.axpby(1.0, FN, 1.0, v);
       NORMFNpred = norm(v);
-      const double NORMbpredC = NORMFN;   // const = 0 in the reference (main.F90:135)
+      // const = 0 in the reference (main.F90:135), so the bound is |b| itself (main.F90:1435-1442)
+      const double NORMbpredC = alc ? sqrt(NORMFN * NORMFN + Narc * Narc) : NORMFN;
       // evaluate the candidate
       V.copy(ZNEWT, Zplus);
       V.axpby(1.0, DIFF, 1.0, Zplus);
       Tplus = T + dsc[0];
       for (int s = 0; s < ns; s++) Splus[s] = sh[s] + dsc[1 + s];
+      v2plus = v2N + dsc[3];
+      NTRY(at_v2(v2plus));
       NTRY(flow(c, Zplus, ZN, Tplus, dt0, o->time_policy));
+      NTRY(at_v2(v2N));
       phi++;
       shift_of(ZN, ZNSH, Splus[0], Splus[1]);
       V.copy(ZNSH, FNplus);
       V.axpby(-1.0, Zplus, 1.0, FNplus);
       NORMFNplus = norm(FNplus);
+      double NORMbplus = NORMFNplus;
+      if (alc) {   // main.F90:1511-1518
+        const double na = narc_of(Zplus, Tplus, Splus[0], v2plus);
+        NORMbplus = sqrt(NORMFNplus * NORMFNplus + na * na);
+      }
       if (o->verbose > 1) {
         fprintf(stderr, "  hook %d mu=%.3e Delta=%.3e |F+|=%.3e pred=%.3e |F|=%.3e\n", IHOOK, mu, Delta,
                 NORMFNplus, NORMFNpred, NORMFN);
@@ -533,9 +811,9 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
         halve = true;
         continue;
       }
-      if (NORMFNplus >= NORMbpredC) {
+      if (NORMbplus >= NORMbpredC) {
         if (dbl) {
-          V.copy(Zplus2, Zplus); Tplus = Tplus2; Splus[0] = Splus2[0]; Splus[1] = Splus2[1];
+          V.copy(Zplus2, Zplus); Tplus = Tplus2; Splus[0] = Splus2[0]; Splus[1] = Splus2[1]; v2plus = v2plus2;
           V.copy(ZN2, ZN); V.copy(FNplus2, FNplus); NORMFNplus = NORMFNplus2;
           Delta /= 2.0;
           break;
@@ -545,7 +823,7 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
       } else {
         if (halve) break;
         if (dbl && NORMFNplus > NORMFNplus2) {
-          V.copy(Zplus2, Zplus); Tplus = Tplus2; Splus[0] = Splus2[0]; Splus[1] = Splus2[1];
+          V.copy(Zplus2, Zplus); Tplus = Tplus2; Splus[0] = Splus2[0]; Splus[1] = Splus2[1]; v2plus = v2plus2;
           V.copy(ZN2, ZN); V.copy(FNplus2, FNplus); NORMFNplus = NORMFNplus2;
           Delta /= 2.0;
           break;
@@ -554,7 +832,7 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
         if (fabs(NORMFNpred - NORMFNplus) < 0.1 * fabs(NORMFNplus - NORMFN) || NORMFNplus < NORMFNpred) {
           Delta *= 2.0;
           dbl = true;
-          V.copy(Zplus, Zplus2); Tplus2 = Tplus; Splus2[0] = Splus[0]; Splus2[1] = Splus[1];
+          V.copy(Zplus, Zplus2); Tplus2 = Tplus; Splus2[0] = Splus[0]; Splus2[1] = Splus[1]; v2plus2 = v2plus;
           V.copy(ZN, ZN2); V.copy(FNplus, FNplus2); NORMFNplus2 = NORMFNplus;
         } else {
           break;
@@ -576,11 +854,15 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
     T = Tplus;
     sh[0] = Splus[0];
     sh[1] = Splus[1];
+    v2N = v2plus;
+    NTRY(at_v2(v2N));
     shift_of(ZNN, ZNSH, sh[0], sh[1]);
     NORMZ = norm(ZNN);
     V.copy(FNplus, FN);
     NORMFN = NORMFNplus;
     NER = NORMFN / NORMZ;
+    if (alc) Narc = narc_of(ZNEWT, T, sh[0], v2N);   // main.F90:1720-1726
+    NER2 = ner2_of(FN);   // main.F90:1700-1712
     if (o->verbose) { fprintf(stderr, "newton %d: NER=%.3e T=%.10g sx=%.6g gmres=%d hook=%d\n", INE, NER, T, sh[0], k, IHOOK); fflush(stderr); }
     const double relch = fabs((NER - NERold) / NER);
     if (relch < 0.001) {
@@ -592,15 +874,15 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
     if (NER <= o->netol) { status = PSG_NEWTON_CONVERGED; break; }
     if (T < 0.1) { status = PSG_NEWTON_PERIOD_SMALL; break; }
   }
-  // low-wavenumber residual NER2 is not recomputed here (diagnostic only)
   NTRY(hipMemcpyAsync(zh, ZNEWT, bytes, hipMemcpyDeviceToHost, c->stream) == hipSuccess ? 0 : PSG_EHIP);
   hipStreamSynchronize(c->stream);
   *Tio = T;
   if (sxio) *sxio = sh[0];
   if (szio) *szio = sh[1];
+  if (v2io) *v2io = v2N;
   res->T = T; res->sx = sh[0]; res->sz = sh[1];
   res->ner = NER;
-  res->ner2 = -1.0;
+  res->ner2 = NER2;
   res->newton_iters = INE > o->max_newton ? o->max_newton : INE;
   res->gmres_iters = (int)gm_total;
   res->hook_iters = (int)hook_total;
@@ -610,6 +892,191 @@ int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *
   cleanup();
 #undef NTRY
   return 0;
+}
+
+int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szio, const psg_newton_opts *o,
+                     psg_newton_result *res) {
+  return newton_core(c, zh, Tio, sxio, szio, nullptr, nullptr, o, res);
+}
+
+int psg_newton_solve_alc(psg_ctx *c, double *zh, double *Tio, double *sxio, double *v2io, const psg_alc *alc,
+                         const psg_newton_opts *o, psg_newton_result *res) {
+  if (!alc) { set_error("null argument"); return PSG_EINVAL; }
+  if (o && o->nshift > 1) { set_error("newton: continuation takes the x shift only"); return PSG_EINVAL; }
+  return newton_core(c, zh, Tio, sxio, nullptr, v2io, alc, o, res);
+}
+
+void psg_floquet_defaults(psg_floquet_opts *o) {
+  memset(o, 0, sizeof(*o));
+  o->nev = 10;            // main.F90:63-67
+  o->ncv = 200;
+  o->max_restarts = 20;
+  o->tol = 1e-7;
+  o->which = PSG_FLOQUET_LR;   // main.F90:388
+  o->jv = 1;
+  o->eps = 1e-7;
+  o->time_policy = PSG_TIME_UNIFORM;
+  o->dt0 = 0.005;
+  o->seed = 1;
+}
+
+int psg_floquet(psg_ctx *c, const double *zh, double T, double sx, double sz, const psg_floquet_opts *o,
+                double *mu_re, double *mu_im, double *err, double *vecs, psg_floquet_result *res) {
+  typedef std::complex<double> Z;
+  if (!c || !zh || !o || !mu_re || !mu_im || !err || !res) { set_error("null argument"); return PSG_EINVAL; }
+  if (c->batch != 1) { set_error("psg_floquet needs a batch-1 context"); return PSG_EINVAL; }
+  const int nev = o->nev, m = o->ncv;
+  if (nev < 1 || m < nev + 2 || m > 400) { set_error("psg_floquet: need 1 <= nev and nev + 2 <= ncv <= 400"); return PSG_EINVAL; }
+  if (o->jv < 0 || o->jv > 2) { set_error("psg_floquet: jv %d is not 0, 1 or 2", o->jv); return PSG_EINVAL; }
+  if (o->jv >= 1 && c->cfg.flavour != PSG_NS2D && c->cfg.flavour != PSG_NS3D) {
+    set_error("psg_floquet: tangent-linear Jv is 2D/3D Navier-Stokes only");
+    return PSG_EINVAL;
+  }
+  TlPrecision tlp(c, o->jv == 2);
+  const auto t_start = std::chrono::steady_clock::now();
+  memset(res, 0, sizeof(*res));
+  VecOps V(c);
+  if (!V.dpart) { set_error("psg_floquet: reduction buffer"); return PSG_ENOMEM; }
+  const long n = V.n;
+  const size_t bytes = sizeof(cpx<double>) * n;
+  cpx<double> *buf = nullptr;
+  // Q (m + 1 basis vectors), the orbit, its image, and three work vectors
+  if (hipMalloc(&buf, bytes * (size_t)(m + 6)) != hipSuccess) { set_error("psg_floquet: device allocation"); return PSG_ENOMEM; }
+  auto q = [&](int i) { return buf + (size_t)i * n; };
+  cpx<double> *Z0 = q(m + 1), *ZN = q(m + 2), *w = q(m + 3), *t1 = q(m + 4), *t2 = q(m + 5);
+  int rc = 0;
+  auto done = [&](int r) {
+    hipStreamSynchronize(c->stream);
+    hipFree(buf);
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return r;
+  };
+  if (hipMemcpyAsync(t1, zh, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) { set_error("copy"); return done(PSG_EHIP); }
+  V.copy(t1, Z0);
+  if (o->jv == 0) {
+    if ((rc = flow(c, Z0, ZN, T, o->dt0, o->time_policy))) return done(rc);
+  }
+  // A q = S_s dPhi_T(z) q
+  auto apply_A = [&](const cpx<double> *in, cpx<double> *out) -> int {
+    int r;
+    if (o->jv >= 1) {
+      if ((r = tl_flow_map(c, Z0, in, t2, out, T, o->dt0, o->time_policy, true))) return r;
+    } else {
+      V.copy(Z0, t2);
+      V.axpby(o->eps, in, 1.0, t2);
+      if ((r = flow(c, t2, out, T, o->dt0, o->time_policy))) return r;
+      V.axpby(-1.0, ZN, 1.0, out);
+      V.axpby(0.0, out, 1.0 / o->eps, out);
+    }
+    V.shift(out, out, sx, o->sy, sz);
+    res->matvecs++;
+    return 0;
+  };
+  // seeded start vector on the active modes
+  {
+    std::vector<cpx<double>> h0(n);
+    unsigned long long st = o->seed * 0x9E3779B97F4A7C15ull + 12345u;
+    auto rnd = [&]() {
+      st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+      return (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    };
+    for (long i = 0; i < n; i++) h0[i] = cpx<double>{rnd(), rnd()};
+    if (hipMemcpyAsync(t1, h0.data(), bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) { set_error("copy"); return done(PSG_EHIP); }
+  }
+  const double n0 = sqrt(V.dot(t1, t1));
+  V.axpby(1.0 / n0, t1, 0.0, q(0));
+  std::vector<double> Hm((size_t)(m + 1) * m);
+  auto h = [&](int i, int j) -> double & { return Hm[(size_t)i * m + j]; };
+  std::vector<int> order;
+  std::vector<Z> mus;
+  std::vector<std::vector<Z>> ys;
+  std::vector<double> errs;
+  int nconv = 0, kdim = m;
+  for (int cyc = 0; cyc <= o->max_restarts; cyc++) {
+    res->cycles = cyc + 1;
+    std::fill(Hm.begin(), Hm.end(), 0.0);
+    kdim = m;
+    for (int j = 0; j < m; j++) {
+      if ((rc = apply_A(q(j), w))) return done(rc);
+      const double wn0 = sqrt(V.dot(w, w));
+      if (!(wn0 == wn0)) { set_error("psg_floquet: the Jacobian action is not finite"); return done(PSG_EINVAL); }
+      // modified Gram-Schmidt, twice
+      for (int pass = 0; pass < 2; pass++)
+        for (int i = 0; i <= j; i++) {
+          const double hij = V.dot(q(i), w);
+          h(i, j) += hij;
+          V.axpby(-hij, q(i), 1.0, w);
+        }
+      const double hn = sqrt(V.dot(w, w));
+      h(j + 1, j) = hn;
+      // invariant subspace: what is left after orthogonalisation is rounding
+      // (operators with repeated eigenvalues, e.g. symmetry pairs, reach it
+      // before the space is exhausted)
+      if (!(hn > 1e-12 * wn0)) { h(j + 1, j) = 0.0; kdim = j + 1; break; }
+      V.axpby(1.0 / hn, w, 0.0, q(j + 1));
+    }
+    // Ritz values of the kdim x kdim Hessenberg matrix
+    std::vector<double> hh((size_t)kdim * kdim), wr(kdim), wi(kdim);
+    for (int i = 0; i < kdim; i++)
+      for (int j = 0; j < kdim; j++) hh[(size_t)i * kdim + j] = h(i, j);
+    std::vector<double> hq = hh;
+    if (hess_eig(kdim, hq.data(), wr.data(), wi.data())) { set_error("psg_floquet: Hessenberg QR failed"); return done(-1); }
+    order.resize(kdim);
+    for (int i = 0; i < kdim; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      const double ka = o->which == PSG_FLOQUET_LM ? std::hypot(wr[a], wi[a]) : wr[a];
+      const double kb = o->which == PSG_FLOQUET_LM ? std::hypot(wr[b], wi[b]) : wr[b];
+      if (ka != kb) return ka > kb;
+      return wi[a] > wi[b];   // conjugate pairs: positive imaginary part first
+    });
+    const int nw = std::min(nev, kdim);
+    mus.assign(nw, Z(0, 0));
+    ys.assign(nw, {});
+    errs.assign(nw, 0.0);
+    nconv = 0;
+    const double beta = kdim < m || h(m, m - 1) == 0.0 ? 0.0 : h(m, m - 1);
+    for (int i = 0; i < nw; i++) {
+      mus[i] = Z(wr[order[i]], wi[order[i]]);
+      hess_evec(kdim, hh.data(), mus[i], ys[i]);
+      errs[i] = (kdim < m ? 0.0 : beta) * std::abs(ys[i][kdim - 1]);
+      if (errs[i] <= o->tol * std::max(std::abs(mus[i]), 1e-300)) nconv++;
+    }
+    if (o->verbose) {
+      fprintf(stderr, "floquet cycle %d: %d/%d converged, leading mu = %.10g%+.10gi (err %.2e)\n", cyc, nconv, nw,
+              mus[0].real(), mus[0].imag(), errs[0]);
+      fflush(stderr);
+    }
+    if (nconv >= nw || cyc == o->max_restarts) break;
+    // explicit restart from the sum of the wanted Ritz vectors' real and imaginary parts
+    V.axpby(0.0, w, 0.0, w);
+    for (int i = 0; i < nw; i++)
+      for (int j = 0; j < kdim; j++) V.axpby(ys[i][j].real() + ys[i][j].imag(), q(j), 1.0, w);
+    const double wn = sqrt(V.dot(w, w));
+    V.axpby(1.0 / wn, w, 0.0, q(0));
+  }
+  const int nw = (int)mus.size();
+  for (int i = 0; i < nev; i++) {
+    mu_re[i] = i < nw ? mus[i].real() : 0.0;
+    mu_im[i] = i < nw ? mus[i].imag() : 0.0;
+    err[i] = i < nw ? errs[i] : -1.0;
+  }
+  res->nconv = nconv;
+  if (vecs) {
+    // Ritz vectors; a conjugate pair gives its real then its imaginary part
+    for (int i = 0; i < nw; i++) {
+      const bool second = i > 0 && mus[i].imag() < 0.0 && mus[i - 1].imag() > 0.0 &&
+                          mus[i] == std::conj(mus[i - 1]);
+      const std::vector<Z> &y = second ? ys[i - 1] : ys[i];
+      V.axpby(0.0, w, 0.0, w);
+      for (int j = 0; j < kdim; j++) V.axpby(second ? y[j].imag() : y[j].real(), q(j), 1.0, w);
+      if (hipMemcpyAsync(vecs + (size_t)i * 2 * n, w, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
+        set_error("copy");
+        return done(PSG_EHIP);
+      }
+    }
+  }
+  return done(0);
 }
 
 }  // extern "C"

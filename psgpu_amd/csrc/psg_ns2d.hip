@@ -15,8 +15,8 @@ namespace psg {
 
 // 2D y-pass column tiles: 4 columns per block from N = 512 on (the launches
 // have few blocks; half the LDS per block lets them spread over more CUs)
-template <int N> struct Col2 {
-  static constexpr int T = N >= 512 ? 4 : Col<N>::T;
+template <int N, typename P> struct Col2 {
+  static constexpr int T = N >= 512 ? 4 : Col<N, P>::T;
   static constexpr int THREADS = T * (N / EPT);
 };
 
@@ -87,12 +87,12 @@ __global__ void k2_setz0(const cpx<double> *__restrict__ z, cpx<double> *__restr
 template <typename P> static int launch_yinv2(Ctx *c, const cpx<double> *st, cpx<P> *out, int nm) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.ny, {
-    constexpr int T = Col2<N>::T;
+    constexpr int T = Col2<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, 3, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k2_yinv<P, N, T, 1, true>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col2<N>::THREADS), lds, c->stream, st, g.nkt, out, 3 * g.ysz, g,
+    hipLaunchKernelGGL(k, grid, dim3(Col2<N, P>::THREADS), lds, c->stream, st, g.nkt, out, 3 * g.ysz, g,
                        twp<P>(c, 1), c->mask, (const cpx<double> *)nullptr);
   });
   PSG_HIP(hipGetLastError());
@@ -102,12 +102,12 @@ template <typename P> static int launch_yinv2(Ctx *c, const cpx<double> *st, cpx
 template <typename P, class UPD> static int launch_yfwd2(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.ny, {
-    constexpr int T = Col2<N>::T;
+    constexpr int T = Col2<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, 1, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k2_yfwd<P, N, T, UPD>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col2<N>::THREADS), lds, c->stream, in, 2 * g.ysz, g, twp<P>(c, 1), upd);
+    hipLaunchKernelGGL(k, grid, dim3(Col2<N, P>::THREADS), lds, c->stream, in, 2 * g.ysz, g, twp<P>(c, 1), upd);
   });
   PSG_HIP(hipGetLastError());
   return 0;
@@ -142,7 +142,7 @@ static int launch_xpass2(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int
       auto k = k_xpass<P, N, TE, FI, FO, PROD>;
       PSG_HIP(lds_attr(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g,
-                         twp<P>(c, 0), prod);
+                         twp<P>(c, 0), prod, 0);
     }
   });
   PSG_HIP(hipGetLastError());
@@ -219,15 +219,15 @@ k2_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, long nrow, Gri
   const long row = (long)blockIdx.x * TE + team;
   const bool valid = row < nrow;
   const int f = blockIdx.y;
-  const cpx<P> *ra = in + (long)f * g.ysz + yrow(g, valid ? row : 0);
+  const cpx<P> *ra = in + (long)f * g.ysz + yrow<P>(g, valid ? row : 0);
   cpx<P> v[EPT];
 #pragma unroll
   for (int mm = 0; mm < EPT; mm++) {
     const int k = j + TPX * mm;
     cpx<P> a{0, 0};
     if (valid) {
-      if (k < g.iktx) { a = ra[ycol(g, k)]; if (k == 0) a.y = 0; }
-      else if (k > N - g.iktx) { a = ra[ycol(g, N - k)]; a.y = -a.y; }
+      if (k < g.iktx) { a = ra[ycol<P>(g, k)]; if (k == 0) a.y = 0; }
+      else if (k > N - g.iktx) { a = ra[ycol<P>(g, N - k)]; a.y = -a.y; }
     }
     v[mm] = a;
   }
@@ -323,23 +323,23 @@ template <typename P, class UPD> static int rhs2_tl(Ctx *c, UPD upd, int nm) {
   const Grid &g = c->g;
   cpx<P> *B = (cpx<P> *)c->tl_bufB, *Cb = (cpx<P> *)c->tl_bufC;
   PSG_DISPATCH_N(g.ny, {
-    constexpr int T = Col2<N>::T;
+    constexpr int T = Col2<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, 6, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k2_yinv<P, N, T, 2, true>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col2<N>::THREADS), lds, c->stream, c->stage, g.nkt, B, 6 * g.ysz, g,
+    hipLaunchKernelGGL(k, grid, dim3(Col2<N, P>::THREADS), lds, c->stream, c->stage, g.nkt, B, 6 * g.ysz, g,
                        twp<P>(c, 1), c->mask, c->tl_stage);
   });
   PSG_HIP(hipGetLastError());
   PSG_TRY2((launch_xpass2<P, 6, 4>(c, B, Cb, g.ny, nm, Prod2DTL<P>{})));
   PSG_DISPATCH_N(g.ny, {
-    constexpr int T = Col2<N>::T;
+    constexpr int T = Col2<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, 2, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k2_yfwd<P, N, T, UPD, 4, 2>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col2<N>::THREADS), lds, c->stream, Cb, 4 * g.ysz, g, twp<P>(c, 1), upd);
+    hipLaunchKernelGGL(k, grid, dim3(Col2<N, P>::THREADS), lds, c->stream, Cb, 4 * g.ysz, g, twp<P>(c, 1), upd);
   });
   PSG_HIP(hipGetLastError());
   return 0;

@@ -90,68 +90,73 @@ struct UpdStoreNL {
 };
 
 // generic column passes (also used by the Boussinesq and tangent-linear units)
-template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf) {
+// planes z0 .. z0+nzs-1 of every field (nzs = 0: all planes)
+template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0, int nzs) {
   const Grid &g = c->g;
+  if (nzs <= 0) z0 = 0, nzs = g.nz;
+  if (z0 < 0 || z0 + nzs > g.nz) { set_error("y-inverse: planes %d+%d outside %d", z0, nzs, g.nz); return PSG_EINVAL; }
   PSG_DISPATCH_N(g.ny, {
-    constexpr int T = Col<N>::T;
-    dim3 grid((g.iktx + T - 1) / T, g.nz, nmf);
+    constexpr int T = Col<N, P>::T;
+    dim3 grid((g.iktx + T - 1) / T, nzs, nmf);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_yinv<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1), z0);
   });
   PSG_HIP(hipGetLastError());
   return 0;
 }
 
-template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf) {
+template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0, int nzs) {
   const Grid &g = c->g;
+  if (nzs <= 0) z0 = 0, nzs = g.nz;
+  if (z0 < 0 || z0 + nzs > g.nz) { set_error("y-forward: planes %d+%d outside %d", z0, nzs, g.nz); return PSG_EINVAL; }
   PSG_DISPATCH_N(g.ny, {
-    constexpr int T = Col<N>::T;
-    dim3 grid((g.iktx + T - 1) / T, g.nz, nmf);
+    constexpr int T = Col<N, P>::T;
+    dim3 grid((g.iktx + T - 1) / T, nzs, nmf);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_yfwd<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1));
+    hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1), z0);
   });
   PSG_HIP(hipGetLastError());
   return 0;
 }
 
-template int launch_yinv_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int);
-template int launch_yinv_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int);
-template int launch_yfwd_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int);
-template int launch_yfwd_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int);
+template int launch_yinv_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int, int, int);
+template int launch_yinv_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int, int, int);
+template int launch_yfwd_g<float>(Ctx *, const cpx<float> *, cpx<float> *, int, int, int);
+template int launch_yfwd_g<double>(Ctx *, const cpx<double> *, cpx<double> *, int, int, int);
 
 // x-pass shapes of the Navier-Stokes product, chosen per context (psg_autotune):
 // points per thread, input front-loading, per-pair product re-evaluation
-template <typename P> static int xpass_ns(Ctx *c, const cpx<P> *B, cpx<P> *Cb, int nm) {
-  const long nr = (long)c->g.ny * c->g.nz;
+template <typename P> static int xpass_ns(Ctx *c, const cpx<P> *B, cpx<P> *Cb, int nm, int z0, int nzs) {
+  const long nr = (long)c->g.ny * nzs;
   switch (c->xvar) {
-    case 1: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true>(c, B, Cb, nr, nm, ProdNS<P>{});
-    case 2: return launch_xpass<P, 6, 3, ProdNS<P>, 8, false, true>(c, B, Cb, nr, nm, ProdNS<P>{});
-    case 3: return launch_xpass<P, 6, 3, ProdNS<P>, 4, true, false>(c, B, Cb, nr, nm, ProdNS<P>{});
-    case 4: return launch_xpass<P, 6, 3, ProdNS<P>, 4, false, true>(c, B, Cb, nr, nm, ProdNS<P>{});
-    case 5: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 2>(c, B, Cb, nr, nm, ProdNS<P>{});
-    case 6: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 4>(c, B, Cb, nr, nm, ProdNS<P>{});
-    case 7: return launch_xpass<P, 6, 3, ProdNS<P>, 4, true, true, 1>(c, B, Cb, nr, nm, ProdNS<P>{});
+    case 1: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
+    case 2: return launch_xpass<P, 6, 3, ProdNS<P>, 8, false, true>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
+    case 3: return launch_xpass<P, 6, 3, ProdNS<P>, 4, true, false>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
+    case 4: return launch_xpass<P, 6, 3, ProdNS<P>, 4, false, true>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
+    case 5: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 2>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
+    case 6: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 4>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
+    case 7: return launch_xpass<P, 6, 3, ProdNS<P>, 4, true, true, 1>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
     case 8:
     case 9:
       // 16 points per thread (radix-16 stages, one LDS exchange per 256-point
       // transform): FP32 only, the FP64 registers would not fit
       if constexpr (std::is_same<P, float>::value) {
-        if (c->xvar == 8) return launch_xpass<P, 6, 3, ProdNS<P>, 16, false, true>(c, B, Cb, nr, nm, ProdNS<P>{});
-        return launch_xpass<P, 6, 3, ProdNS<P>, 16, false, true, 4>(c, B, Cb, nr, nm, ProdNS<P>{});
+        if (c->xvar == 8) return launch_xpass<P, 6, 3, ProdNS<P>, 16, false, true>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
+        return launch_xpass<P, 6, 3, ProdNS<P>, 16, false, true, 4>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
       }
-      return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true>(c, B, Cb, nr, nm, ProdNS<P>{});
+      return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
     // register caps for occupancy: FP64 4 or 5 waves per SIMD, FP32 5 or 6
     case 10:
       return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 0, sizeof(P) == 8 ? 4 : 5>(c, B, Cb, nr, nm,
-                                                                                       ProdNS<P>{});
+                                                                                       ProdNS<P>{}, z0);
     case 11:
       return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, true, 0, sizeof(P) == 8 ? 5 : 6>(c, B, Cb, nr, nm,
-                                                                                       ProdNS<P>{});
-    default: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, false>(c, B, Cb, nr, nm, ProdNS<P>{});
+                                                                                       ProdNS<P>{}, z0);
+    default: return launch_xpass<P, 6, 3, ProdNS<P>, 8, true, false>(c, B, Cb, nr, nm, ProdNS<P>{}, z0);
   }
 }
 
@@ -165,15 +170,23 @@ static int rhs3(Ctx *c, const cpx<S> *st, long st_ms, int umask, UPD upd, long s
   prof_begin(c, "zinv", pb[0] * nm);
   PSG_TRY(launch_zinv<P>(c, st, st_ms, A, 6 * g.zsz, true, umask, nm));
   prof_end(c);
-  prof_begin(c, "yinv", pb[1] * nm);
-  PSG_TRY(launch_yinv_g<P>(c, A, B, 6 * nm));
-  prof_end(c);
-  prof_begin(c, "xpass", pb[2] * nm);
-  PSG_TRY(xpass_ns<P>(c, B, Cb, nm));
-  prof_end(c);
-  prof_begin(c, "yfwd", pb[3] * nm);
-  PSG_TRY(launch_yfwd_g<P>(c, Cb, A, 3 * nm));
-  prof_end(c);
+  // passes 2-4 run slab by slab of z planes when the context asks for it, so
+  // that the slab's Y and C blocks are read back while they are still held
+  // by the device's last-level cache; A' of a slab only overwrites A planes
+  // of the same slab, which its y-inverse has already read
+  const int SL = slab_planes(c);
+  for (int z0 = 0; z0 < g.nz; z0 += SL) {
+    const long fr = SL;   // bytes of a slab: the planes' share of the pass
+    prof_begin(c, "yinv", pb[1] * nm / g.nz * fr);
+    PSG_TRY(launch_yinv_g<P>(c, A, B, 6 * nm, z0, SL));
+    prof_end(c);
+    prof_begin(c, "xpass", pb[2] * nm / g.nz * fr);
+    PSG_TRY(xpass_ns<P>(c, B, Cb, nm, z0, SL));
+    prof_end(c);
+    prof_begin(c, "yfwd", pb[3] * nm / g.nz * fr);
+    PSG_TRY(launch_yfwd_g<P>(c, Cb, A, 3 * nm, z0, SL));
+    prof_end(c);
+  }
   prof_begin(c, "zfwd_update", (pb[4] + state_bytes) * nm);
   if (c->zvar == 1) PSG_TRY((launch_zfwd<P, 3, UPD, false>(c, A, upd, nm)));
   else PSG_TRY((launch_zfwd<P, 3, UPD, true>(c, A, upd, nm)));
@@ -298,15 +311,15 @@ k_xphys(const cpx<P> *__restrict__ in, double *__restrict__ out, int nf, long nr
   const long row = (long)blockIdx.x * TE + team;
   const bool valid = row < nrow;
   const int f = blockIdx.y;
-  const cpx<P> *ra = in + (long)f * g.ysz + yrow(g, valid ? row : 0);
+  const cpx<P> *ra = in + (long)f * g.ysz + yrow<P>(g, valid ? row : 0);
   cpx<P> v[EPT];
 #pragma unroll
   for (int mm = 0; mm < EPT; mm++) {
     const int k = j + TPX * mm;
     cpx<P> a{0, 0};
     if (valid) {
-      if (k < g.iktx) { a = ra[ycol(g, k)]; if (k == 0) a.y = 0; }
-      else if (k > N - g.iktx) { a = ra[ycol(g, N - k)]; a.y = -a.y; }
+      if (k < g.iktx) { a = ra[ycol<P>(g, k)]; if (k == 0) a.y = 0; }
+      else if (k > N - g.iktx) { a = ra[ycol<P>(g, N - k)]; a.y = -a.y; }
     }
     v[mm] = a;
   }

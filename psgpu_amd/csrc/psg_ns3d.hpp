@@ -37,26 +37,26 @@ __device__ __forceinline__ void add_forcing(const Force &F, double kx, double ky
 
 
 // ------------------------------------------------------------------ launches
-template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
-template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf);
+template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
+template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
 
 template <typename P, typename S = double>
 static int launch_zinv(Ctx *c, const cpx<S> *st, long st_ms, cpx<P> *out, long out_ms, bool bs, int umask,
                        int nm) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nz, {
-    constexpr int T = Col<N>::T;
+    constexpr int T = Col<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     if (bs) {
       auto k = k_zinv<P, N, T, true, S>;
       PSG_HIP(lds_attr(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
+      hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
                          twp<P>(c, 2), c->mask, umask);
     } else {
       auto k = k_zinv<P, N, T, false, S>;
       PSG_HIP(lds_attr(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
+      hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
                          twp<P>(c, 2), c->mask, umask);
     }
   });
@@ -68,7 +68,8 @@ static int launch_zinv(Ctx *c, const cpx<S> *st, long st_ms, cpx<P> *out, long o
 // TEAMS_MAX caps the rows per block (0: as many as fit in 256 threads)
 template <typename P, int FI, int FO, class PROD, int E = EPT, bool FRONT = true, bool LAZY = false,
           int TEAMS_MAX = 0, int WPE = 0>
-static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod) {
+// nrow_m rows per member from plane zs0 on (nrow_m = NY x planes)
+static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod, int zs0 = 0) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nx, {
     if (g.iktx != xband<N>()) {
@@ -87,7 +88,8 @@ static int launch_xpass(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int 
     size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
     auto k = k_xpass<P, N, TE, FI, FO, PROD, E, FRONT, LAZY, WPE>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(TE * TPX), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0), prod);
+    hipLaunchKernelGGL(k, grid, dim3(TE * TPX), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0), prod,
+                       zs0);
   });
   PSG_HIP(hipGetLastError());
   return 0;
@@ -97,12 +99,12 @@ template <typename P, int FO, class UPD, bool FRONT = (FO <= 4)>
 static int launch_zfwd(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nz, {
-    constexpr int T = Col<N>::T;
+    constexpr int T = Col<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_zfwd<P, N, T, FO, UPD, FRONT>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);
+    hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);
   });
   PSG_HIP(hipGetLastError());
   return 0;

@@ -90,13 +90,49 @@ template <typename P> __device__ __forceinline__ cpx<P> to(cpx<float> a) { retur
 __device__ __forceinline__ cpx<double> up(cpx<double> a) { return a; }
 __device__ __forceinline__ cpx<double> up(cpx<float> a) { return {(double)a.x, (double)a.y}; }
 
+// Pass-buffer element access.  Every pass-buffer element is written once by
+// one pass and read once by the next, so these accesses may bypass the caches
+// as non-temporal loads and stores.  Which passes gain was measured on MI355X
+// (256^3, per-kernel HIP event times): in FP64 streaming the stores of the
+// z-inverse, y-inverse and x-pass took 8% off a step and streaming the loads
+// of the y-inverse and z-forward a little more, while the y-forward and the
+// x-pass's loads ran slower streamed; in FP32 the z-inverse's stores did.
+enum { PASS_ZINV, PASS_YINV, PASS_XPASS, PASS_YFWD, PASS_ZFWD };
+template <typename P, int PASS> struct NT {
+  static constexpr bool f64 = sizeof(P) == 8;
+  static constexpr bool st = PASS == PASS_YINV || PASS == PASS_XPASS || (PASS == PASS_ZINV && f64) ||
+                             (PASS == PASS_YFWD && !f64);
+  static constexpr bool ld = PASS == PASS_YINV || PASS == PASS_ZFWD;
+};
+template <bool STREAM, typename P> __device__ __forceinline__ void st_pass(cpx<P> *p, cpx<P> v) {
+#if defined(__clang__)
+  if constexpr (STREAM) {
+    typedef P vec2 __attribute__((ext_vector_type(2)));
+    vec2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<vec2 *>(p));
+    return;
+  }
+#endif
+  *p = v;
+}
+template <bool STREAM, typename P> __device__ __forceinline__ cpx<P> ld_pass(const cpx<P> *p) {
+#if defined(__clang__)
+  if constexpr (STREAM) {
+    typedef P vec2 __attribute__((ext_vector_type(2)));
+    const vec2 t = __builtin_nontemporal_load(reinterpret_cast<const vec2 *>(p));
+    return cpx<P>{t.x, t.y};
+  }
+#endif
+  return *p;
+}
+
 // kz (or ky) of padded position p, or INT_MIN when p is in the zero band
 __device__ __forceinline__ int wrapk(int p, int n, int kt) {
   return p <= kt ? p : (p >= n - kt ? p - n : -0x40000000);
 }
 #define PSG_NOK (-0x40000000)
 
-// Pass-buffer layouts, per field.  kx columns are grouped in tiles of LW = 8
+// Pass-buffer layouts, per field.  kx columns are grouped in tiles of LW<P>
 // (xp/LW tiles).  A column pass's workgroup owns one tile of one plane, so
 // the y passes read and write whole tiles contiguously (21.9 KB of ky and
 // 32 KB of y at 256^3 FP64) instead of 128-byte pieces of rows; the z passes
@@ -105,16 +141,33 @@ __device__ __forceinline__ int wrapk(int p, int n, int kt) {
 //   Z, Z' (z-inverse -> y-inverse, y-forward -> z-forward):  [z][tile][iky][LW]
 //   Y     (y-inverse -> x-pass):                            [z][tile][y][LW]
 //   C     (x-pass -> y-forward): row-major [z][y][xp] (the x-pass writes rows)
-constexpr int LW = 8;
-__device__ __forceinline__ long zoff(const Grid &g, int ikx, int iky, int z) {
-  return ((long)z * (g.xp / LW) + (ikx >> 3)) * ((long)g.ikty * LW) + iky * LW + (ikx & (LW - 1));
+// The tile width is LW<P>: 8 points (128 B) in FP64; in FP32 PSG_LW32 points
+// (8: 64 B, 16: a whole 128-byte line).  XP is a multiple of 16, so either
+// tiles the padded kx range.
+#ifndef PSG_LW32
+#define PSG_LW32 8
+#endif
+template <typename P> struct Lay {
+  static constexpr int W = sizeof(P) == 4 ? PSG_LW32 : 8;
+  static constexpr int SH = W == 16 ? 4 : 3;
+  static_assert(W == 8 || W == 16, "tile width");
+};
+template <typename P> __device__ __forceinline__ long zoff(const Grid &g, int ikx, int iky, int z) {
+  constexpr int W = Lay<P>::W, SH = Lay<P>::SH;
+  return ((long)z * (g.xp / W) + (ikx >> SH)) * ((long)g.ikty * W) + iky * W + (ikx & (W - 1));
 }
-__device__ __forceinline__ long yoff(const Grid &g, int ikx, int y, int z) {
-  return ((long)z * (g.xp / LW) + (ikx >> 3)) * ((long)g.ny * LW) + y * LW + (ikx & (LW - 1));
+template <typename P> __device__ __forceinline__ long yoff(const Grid &g, int ikx, int y, int z) {
+  constexpr int W = Lay<P>::W, SH = Lay<P>::SH;
+  return ((long)z * (g.xp / W) + (ikx >> SH)) * ((long)g.ny * W) + y * W + (ikx & (W - 1));
 }
 // x row rr = z*ny + y of a Y buffer: element k sits at yrow(rr)[ycol(k)]
-__device__ __forceinline__ long yrow(const Grid &g, long rr) { return yoff(g, 0, (int)(rr % g.ny), (int)(rr / g.ny)); }
-__device__ __forceinline__ int ycol(const Grid &g, int k) { return (k >> 3) * (g.ny * LW) + (k & (LW - 1)); }
+template <typename P> __device__ __forceinline__ long yrow(const Grid &g, long rr) {
+  return yoff<P>(g, 0, (int)(rr % g.ny), (int)(rr / g.ny));
+}
+template <typename P> __device__ __forceinline__ int ycol(const Grid &g, int k) {
+  constexpr int W = Lay<P>::W, SH = Lay<P>::SH;
+  return (k >> SH) * (g.ny * W) + (k & (W - 1));
+}
 
 // Biot-Savart u = i k x w / max(|k|^2,1e-3): 3D_NavierStokes/src/kernels.h:187-201
 __device__ __forceinline__ void biot_savart(double kx, double ky, double kz, cpx<double> xi, cpx<double> et,
@@ -208,7 +261,7 @@ k_zinv(const cpx<S> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict
     if (valid) {
       cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz;
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) o[zoff(g, ikx, iky, j + TP * mm)] = v[mm];
+      for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_ZINV>::st>(&o[zoff<P>(g, ikx, iky, j + TP * mm)], v[mm]);
     }
   }
 }
@@ -216,12 +269,12 @@ k_zinv(const cpx<S> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict
 // ---------------------------------------------------------------- pass 2 (3D)
 template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
-k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw) {
+k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw, int z0) {
   PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
-  const int ikx = tid.x * T + col, z = tid.y, mf = tid.z;
+  const int ikx = tid.x * T + col, z = z0 + tid.y, mf = tid.z;
   const bool valid = ikx < g.iktx;
   const cpx<P> *src = in + (long)mf * g.zsz;
   cpx<P> v[EPT];
@@ -229,37 +282,37 @@ k_yinv(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
   for (int mm = 0; mm < EPT; mm++) {
     const int y = j + TP * mm;
     const int ky = wrapk(y, g.ny, g.kty);
-    v[mm] = (valid && ky != PSG_NOK) ? src[zoff(g, ikx, ky + g.kty, z)] : cpx<P>{0, 0};
+    v[mm] = (valid && ky != PSG_NOK) ? ld_pass<NT<P, PASS_YINV>::ld>(&src[zoff<P>(g, ikx, ky + g.kty, z)]) : cpx<P>{0, 0};
   }
   fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
   if (valid) {
     cpx<P> *o = out + (long)mf * g.ysz;
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) o[yoff(g, ikx, j + TP * mm, z)] = v[mm];
+    for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_YINV>::st>(&o[yoff<P>(g, ikx, j + TP * mm, z)], v[mm]);
   }
 }
 
 // ---------------------------------------------------------------- pass 4 (3D)
 template <typename P, int N, int T>
 __global__ void __launch_bounds__(T *(N / EPT))
-k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw) {
+k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cpx<P> *__restrict__ tw, int z0) {
   PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
-  const int ikx = tid.x * T + col, z = tid.y, mf = tid.z;
+  const int ikx = tid.x * T + col, z = z0 + tid.y, mf = tid.z;
   const bool valid = ikx < g.iktx;
   const cpx<P> *src = in + (long)mf * g.ysz + (long)z * g.ny * g.xp + ikx;
   cpx<P> v[EPT];
 #pragma unroll
-  for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[(long)(j + TP * mm) * g.xp] : cpx<P>{0, 0};
+  for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? ld_pass<NT<P, PASS_YFWD>::ld>(&src[(long)(j + TP * mm) * g.xp]) : cpx<P>{0, 0};
   fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
   if (valid) {
     cpx<P> *o = out + (long)mf * g.zsz;
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
       const int ky = wrapk(j + TP * mm, g.ny, g.kty);
-      if (ky != PSG_NOK) o[zoff(g, ikx, ky + g.kty, z)] = v[mm];
+      if (ky != PSG_NOK) st_pass<NT<P, PASS_YFWD>::st>(&o[zoff<P>(g, ikx, ky + g.kty, z)], v[mm]);
     }
   }
 }
@@ -285,7 +338,7 @@ k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD
   auto load = [&](int f, cpx<P>(&v)[EPT]) {
     const cpx<P> *src = in + ((long)m * FO + f) * g.zsz;
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? src[zoff(g, ikx, iky, j + TP * mm)] : cpx<P>{0, 0};
+    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? ld_pass<NT<P, PASS_ZFWD>::ld>(&src[zoff<P>(g, ikx, iky, j + TP * mm)]) : cpx<P>{0, 0};
   };
   if constexpr (FRONT) {
 #pragma unroll
@@ -337,7 +390,7 @@ template <int N> constexpr int xband() { return N / 3 + 1; }
 
 template <typename P, int N, int E> struct XRow {
   // pair p of a row: z = a + i b from the half spectra of fields fa, fb, whose
-  // row starts ra, rb hold element k at ycol(g, k)
+  // row starts ra, rb hold element k at ycol<P>(g, k)
   __device__ __forceinline__ static void load(cpx<P> (&v)[E], const cpx<P> *ra, const cpx<P> *rb, bool hasb,
                                               bool valid, int j, const Grid &g) {
     constexpr int TPX = N / E;
@@ -349,13 +402,13 @@ template <typename P, int N, int E> struct XRow {
       cpx<P> z{0, 0};
       if (valid) {
         if (k < ix) {
-          a = ra[ycol(g, k)];
-          if (hasb) b = rb[ycol(g, k)];
+          a = ld_pass<NT<P, PASS_XPASS>::ld>(&ra[ycol<P>(g, k)]);
+          if (hasb) b = ld_pass<NT<P, PASS_XPASS>::ld>(&rb[ycol<P>(g, k)]);
           if (k == 0) { a.y = 0; b.y = 0; }
           z = cpx<P>{a.x - b.y, a.y + b.x};
         } else if (k > N - ix) {
-          a = ra[ycol(g, N - k)];
-          if (hasb) b = rb[ycol(g, N - k)];
+          a = ld_pass<NT<P, PASS_XPASS>::ld>(&ra[ycol<P>(g, N - k)]);
+          if (hasb) b = ld_pass<NT<P, PASS_XPASS>::ld>(&rb[ycol<P>(g, N - k)]);
           z = cpx<P>{a.x + b.y, b.x - a.y};
         }
       }
@@ -372,14 +425,15 @@ template <typename P, int N, int TEAMS, int FI, int FO, class PROD, int E = EPT,
           bool LAZY = false, int WPE = 0>
 __global__ void __launch_bounds__(TEAMS *(N / E)) PSG_WAVES_PER_EU(WPE > 0 ? WPE : 1)
 k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, long nrows, Grid g,
-        const cpx<P> *__restrict__ tw, PROD prod) {
+        const cpx<P> *__restrict__ tw, PROD prod, int zs0) {
   PSG_LDS_TWIDDLES(P, N, E);
   constexpr int TPX = N / E;
   constexpr bool WAVE = TPX <= 64;
   constexpr int PI = (FI + 1) / 2, PO = (FO + 1) / 2;
   constexpr int NV = FRONT ? PI : 1;
   const int team = threadIdx.x / TPX, j = threadIdx.x % TPX;
-  // row = (m, z, y) with y fastest: the block's first row is split once per
+  // row = (m, z, y) with y fastest over the launch's planes zs0 .. zs0 +
+  // nrow_m/ny - 1 of each member: the block's first row is split once per
   // block (uniform), each team steps y from it (launches keep nrows < 2^31)
   const unsigned row0 = blockIdx.x * (unsigned)TEAMS, nrm = (unsigned)nrow_m, ny = (unsigned)g.ny;
   const unsigned m0 = row0 / nrm, rr0 = row0 - m0 * nrm, z0 = rr0 / ny, nzr = nrm / ny;
@@ -388,11 +442,12 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
   while (z >= nzr) { z -= nzr; m++; }
   const bool valid = row0 + team < (unsigned long)nrows;
   if (!valid) m = 0, z = 0, y = 0;
+  z += (unsigned)zs0;
   const long rr = (long)z * ny + y;
   cpx<P> *tl = lds + team * row_len<N>();
   constexpr int SW = row_swizzle<E>();
   constexpr int ix = xband<N>();
-  const long ro = yoff(g, 0, (int)y, (int)z);
+  const long ro = yoff<P>(g, 0, (int)y, (int)z);
   auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + ro; };
   cpx<P> vin[NV][E];
   if constexpr (FRONT) {
@@ -465,8 +520,8 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
         const P h = (P)0.5;
         cpx<P> A{h * (zk.x + zc.x), h * (zk.y - zc.y)};
         cpx<P> B{h * (zk.y + zc.y), h * (zc.x - zk.x)};
-        oa[k] = A;
-        if (2 * q + 1 < FO) ob[k] = B;
+        st_pass<NT<P, PASS_XPASS>::st>(&oa[k], A);
+        if (2 * q + 1 < FO) st_pass<NT<P, PASS_XPASS>::st>(&ob[k], B);
       }
     }
   }
@@ -503,7 +558,7 @@ k_xpass_split(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow
   const unsigned m = valid ? row / nrm : 0, rr = valid ? row - m * nrm : 0, z = rr / ny, y = rr - z * ny;
   cpx<P> *tl = lds + team * row_len<N>();
   P *phys = reinterpret_cast<P *>(lds + NTEAM * row_len<N>());
-  const long ro = yoff(g, 0, (int)y, (int)z);
+  const long ro = yoff<P>(g, 0, (int)y, (int)z);
   auto rowp = [&](int f) { return in + ((long)m * FI + f) * g.ysz + ro; };
   {
     const int p = team < PI ? team : PI - 1, fa = 2 * p, fb = 2 * p + 1;
@@ -549,8 +604,8 @@ k_xpass_split(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow
       const P h = (P)0.5;
       cpx<P> A{h * (zk.x + zc.x), h * (zk.y - zc.y)};
       cpx<P> B{h * (zk.y + zc.y), h * (zc.x - zk.x)};
-      oa[k] = A;
-      if (2 * q + 1 < FO) ob[k] = B;
+      st_pass<NT<P, PASS_XPASS>::st>(&oa[k], A);
+      if (2 * q + 1 < FO) st_pass<NT<P, PASS_XPASS>::st>(&ob[k], B);
     }
   }
 }
@@ -649,7 +704,7 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
       if (valid) {
         cpx<P> *o = out + (long)m * out_ms + (long)(3 * src + f) * g.ysz;
 #pragma unroll
-        for (int mm = 0; mm < EPT; mm++) o[yoff(g, ikx, j + TP * mm, 0)] = v[mm];
+        for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_YINV>::st>(&o[yoff<P>(g, ikx, j + TP * mm, 0)], v[mm]);
       }
     }
   }

@@ -101,7 +101,7 @@ k_zinv_curl(const cpx<double> *__restrict__ st, long fs, cpx<P> *__restrict__ ou
     if (valid) {
       cpx<P> *o = out + (long)f * g.zsz;
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) o[zoff(g, ikx, iky, j + TP * mm)] = v[mm];
+      for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_ZINV>::st>(&o[zoff<P>(g, ikx, iky, j + TP * mm)], v[mm]);
     }
   }
 }
@@ -232,12 +232,12 @@ static int rhs_adj(Ctx *c, const cpx<double> *w, const cpx<double> *v, const dou
   prof_begin(c, "adj_zinv", pb[0] + 3 * g.nkt * 16);
   PSG_TRY(launch_zinv<P>(c, w, 3 * g.nkt, A, 9 * g.zsz, true, 0, 1));
   PSG_DISPATCH_N(g.nz, {
-    constexpr int T = Col<N>::T;
+    constexpr int T = Col<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, 1);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_zinv_curl<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N>::THREADS), lds, c->stream, v, g.nkt, A + 6 * g.zsz, g, twp<P>(c, 2),
+    hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, v, g.nkt, A + 6 * g.zsz, g, twp<P>(c, 2),
                        tab, fac);
   });
   PSG_HIP(hipGetLastError());

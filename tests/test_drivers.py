@@ -91,9 +91,9 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
     setup_case(d, np.zeros(g.nkt, complex), main=main_2d(**{"infile type": "Z"}))
     r = run(["ns2d", "32"], d)
     assert r.returncode == 1 and "infile type unknown" in r.stderr
-    setup_case(d, np.zeros(g.nkt, complex), extra=extra_2d(arclcFLAG="T"))
+    setup_case(d, np.zeros(g.nkt, complex), extra=extra_2d(arclcFLAG="T", ArnoldiFLAG="T"))
     r = run(["ns2d", "32"], d)
-    assert r.returncode == 1 and "arc-length" in r.stderr
+    assert r.returncode == 1 and "exclusive" in r.stderr
     os.remove(os.path.join(d, "paramsEXTRA.txt"))
     r = run(["ns2d", "32"], d)
     assert r.returncode == 1
@@ -195,6 +195,76 @@ def test_driver_converges_laminar_state_and_writes_outputs(gpu, tmp_path, jv):
     assert np.abs((z - lam)[L == 1]).max() < 1e-9
     zk = read_recs(os.path.join(d, "Zk.out"))
     assert len(zk) == 3 and np.array_equal(np.frombuffer(zk[1][48:], np.complex128), z0)
+
+
+@pytest.mark.gpu
+def test_driver_arnoldi_flag_writes_floquet_exponents(gpu, tmp_path):
+    # ArnoldiFLAG: the guess (here the laminar Kolmogorov state at Re = 40, a fixed
+    # point, unstable) is analysed instead of solved; eigenvalues.dat receives the
+    # 2D layout of 2D_NavierStokes/src_RK4/src_gmres/main.F90:933-948
+    d = str(tmp_path)
+    g = P.hgrid("ns2d", 32, 32)
+    kx, ky, _, L = P.tables(g)
+    lam = np.zeros(g.nkt, complex)
+    lam[(kx == 0) & (ky == 4)] = -2.0 / (16 / 40.0)
+    write_params(os.path.join(d, "params_gmres.txt"), MAIN_2D, main_2d(Re="40.0"))
+    write_params(os.path.join(d, "paramsEXTRA.txt"), EXTRA_2D, extra_2d(ArnoldiFLAG="T"))
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<3i", 1, g.iktx, g.ikty)))
+        f.write(rec(struct.pack("<6d", 0.0, 1 / 40.0, 0.2, 0.0, 0.0, 0.0) + lam.tobytes()))
+    r = run(["ns2d", "32", "--jv", "tl"], d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(os.path.join(d, "eigenvalues.dat")).read().splitlines()
+    assert lines[0].split() == ["UPO", "1"]
+    rows = [tuple(float(x) for x in ln.split()) for ln in lines[1:]]
+    assert rows[-1][0] == 40.0
+    exps = [a for a, _ in rows[:-1]]
+    assert exps and min(exps) > 0 and rows[-1][1] == max(exps)
+    with P.Stepper(flavour="ns2d", n=32, v2=1 / 40.0, dt=0.01, force="kolmogorov", kf=4.0, ampfor=-2.0) as s:
+        mu, err, _, res = s.floquet(lam[None], 0.2, nev=10, ncv=200, tol=1e-7, jv=1, dt0=0.01)
+    lead = np.log(np.abs(mu[0])) / 0.2
+    assert abs(rows[-1][1] - lead) <= 1e-4 * lead, (rows[-1], lead)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("jv", ["fd", "tl"])
+def test_driver_arclength_continuation_traces_laminar_branch(gpu, tmp_path, jv):
+    # arclcFLAG: from the laminar Kolmogorov state at Re = 5 (a fixed point, so the
+    # period is fixed) the continuation steps Re by delta_v2, then follows the
+    # branch z(v2) = -F/(v2 |k|^2) at (0, 4) by pseudo-arclength steps in (z, v2)
+    # (2D_NavierStokes/src_RK4/src_gmres/main.F90:408-604, 1776-1803)
+    d = str(tmp_path)
+    g = P.hgrid("ns2d", 32, 32)
+    kx, ky, _, L = P.tables(g)
+    sel = (kx == 0) & (ky == 4)
+    lam = np.zeros(g.nkt, complex)
+    lam[sel] = -2.0 / (16 * 0.2)
+    write_params(os.path.join(d, "params_gmres.txt"), MAIN_2D, main_2d())
+    write_params(os.path.join(d, "paramsEXTRA.txt"), EXTRA_2D, extra_2d(arclcFLAG="T", IGUESSEND=4, delta_v2="0.02"))
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<3i", 1, g.iktx, g.ikty)))
+        f.write(rec(struct.pack("<6d", 0.0, 0.2, 0.2, 0.0, 0.0, 0.0) + lam.tobytes()))
+    r = run(["ns2d", "32", "--noshift", "--jv", jv], d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    upos = read_recs(os.path.join(d, "UPOs.out"))[1:]
+    assert len(upos) == 5, r.stdout
+    pts = []
+    for u in upos:
+        sc = struct.unpack("<6d", u[:48])
+        z = np.frombuffer(u[48:], np.complex128)
+        v2 = sc[1]
+        assert sc[5] <= 1e-12
+        assert abs(z[sel][0] - (-2.0 / (16 * v2))) <= 1e-9 * abs(z[sel][0])     # on the laminar branch
+        assert np.abs(z[~sel & (L == 1)]).max() <= 1e-9
+        pts.append((v2, z[sel][0].real))
+    v2s = [p[0] for p in pts]
+    assert abs(v2s[0] - 0.18) < 1e-14 and all(b < a for a, b in zip(v2s, v2s[1:]))   # Re increases
+    # arclength step: the second point lies twice the first secant's length along
+    # that secant (main.F90:467-471 and the arclength row)
+    t = np.array([pts[0][0] - 0.2, pts[0][1] - lam[sel][0].real])
+    d0 = np.hypot(*t)
+    proj = np.dot(np.array([pts[1][0] - pts[0][0], pts[1][1] - pts[0][1]]), t / d0)
+    assert abs(proj - 2 * d0) <= 1e-6 * d0, (d0, proj)
 
 
 # ---------------------------------------------------------------- DNS driver

@@ -69,3 +69,25 @@ def test_newton_converged_orbit_rechecks(gpu):
         zT = s.flow_map(z, res["T"], 0.01, 0)
     ner = np.sqrt((np.abs(zT - z) ** 2).sum()) / np.sqrt((np.abs(zT) ** 2).sum())
     assert abs(ner - res["ner"]) <= 1e-3 * max(res["ner"], 1e-16) + 1e-15
+
+
+def test_newton_ner2_low_wavenumber_residual(gpu):
+    # NER2 (2D_NavierStokes/src_RK4/src_gmres/main.F90:124-125, 737-751): the
+    # residual over the modes with |k|^2 <= KYmax2 = (IKTY-KTY-1)^2/4, relative
+    # to ||Phi_T(z)||; recomputed on the host from the state the solver returns
+    n, re_ = 32, 5.0
+    g = O.grid2(n, n)
+    kx, ky, _, L = O.tables(g)
+    lam = np.zeros(g.nkt, complex)
+    lam[np.where((kx == 0) & (ky == 4))[0][0]] = -re_ / 8
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / re_, dt=0.01, kf=4.0, ampfor=-2.0) as s:
+        z, res = s.newton((lam + masked_random(g, 11, 0.2))[None], T=1.5, nshift=0, fixed_period=1, dt0=0.01,
+                          max_newton=1)
+        zT = s.flow_map(z, res["T"], 0.01, 0)[0]
+    F = (zT - z[0]) * (L == 1)
+    low = (kx ** 2 + ky ** 2) <= g.kty ** 2 / 4.0
+    ner = np.sqrt((np.abs(F) ** 2).sum()) / np.sqrt((np.abs(zT) ** 2).sum())
+    ner2 = np.sqrt((np.abs(F[low]) ** 2).sum()) / np.sqrt((np.abs(zT) ** 2).sum())
+    print("NER %.6e (host %.6e)  NER2 %.6e (host %.6e)" % (res["ner"], ner, res["ner2"], ner2), flush=True)
+    assert 0.0 < res["ner2"] <= res["ner"]
+    assert abs(res["ner2"] - ner2) <= 1e-9 * ner2 and abs(res["ner"] - ner) <= 1e-9 * ner

@@ -53,3 +53,63 @@ def test_hookstep_is_levenberg_marquardt():
             assert delta / np.sqrt(2) < nx < np.sqrt(2) * delta
             # (H^T H + mu I) y = H^T b : the Levenberg-Marquardt / trust-region step
             assert np.abs((H.T @ H + mu * np.eye(k)) @ y - H.T @ b).max() < 1e-10
+
+
+def hess_eig(H):
+    n = H.shape[0]
+    a = np.asfortranarray(H, dtype=np.float64)
+    wr, wi = np.zeros(n), np.zeros(n)
+    assert P.lib().psg_hess_eig(n, a.ctypes.data_as(C.c_void_p), wr.ctypes.data_as(C.c_void_p),
+                                wi.ctypes.data_as(C.c_void_p)) == 0
+    return wr + 1j * wi
+
+
+def match(got, want, tol):
+    """every eigenvalue of want paired with a distinct one of got"""
+    got = list(got)
+    for w in want:
+        i = int(np.argmin([abs(g - w) for g in got]))
+        assert abs(got[i] - w) <= tol, (w, got[i])
+        got.pop(i)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 17, 40, 97])
+def test_hess_eig_vs_numpy_random(n):
+    # Hessenberg QR eigenvalues against LAPACK's (numpy.linalg.eigvals) on
+    # random Hessenberg matrices; conjugate pairs adjacent, positive part first
+    rng = np.random.default_rng(100 + n)
+    H = np.triu(rng.standard_normal((n, n)), -1)
+    ev = hess_eig(H)
+    scale = max(1.0, np.abs(H).max())
+    match(ev, np.linalg.eigvals(H), 1e-10 * scale * n)
+    for i in range(n - 1):
+        if ev[i].imag > 0:
+            assert ev[i + 1] == np.conj(ev[i])
+
+
+def test_hess_eig_structured():
+    # a Floquet-like spectrum: a few leading multipliers over a decaying bulk,
+    # similarity-transformed to Hessenberg form by numpy's QR of a random basis
+    rng = np.random.default_rng(7)
+    n = 60
+    lam = np.concatenate([[1.7, -0.9, 0.5 + 0.6j, 0.5 - 0.6j], 0.05 * rng.random(n - 4)])
+    D = np.zeros((n, n))
+    i = 0
+    while i < n:
+        if abs(lam[i].imag) > 0:
+            a, b = lam[i].real, lam[i].imag
+            D[i:i + 2, i:i + 2] = [[a, b], [-b, a]]
+            i += 2
+        else:
+            D[i, i] = lam[i].real
+            i += 1
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    import scipy.linalg as sl
+    Hh = sl.hessenberg(Q @ D @ Q.T)
+    ev = hess_eig(Hh)
+    match(ev, lam, 1e-10)
+    # zero subdiagonal (already split) and a Jordan-like defective 2 x 2 block
+    H2 = np.diag([3.0, 2.0, 1.0]) + np.diag([0.0, 0.0], -1) + np.triu(np.ones((3, 3)), 1)
+    match(hess_eig(H2), [3.0, 2.0, 1.0], 1e-14)
+    J = np.array([[2.0, 1.0], [0.0, 2.0]])
+    match(hess_eig(J), [2.0, 2.0], 1e-7)

@@ -73,6 +73,30 @@ def test_diag_3d_parseval():
     assert np.abs(ph[3] - up[0]).max() <= 1e-12 * np.abs(up[0]).max()
 
 
+def test_diag_sums_exactly_rounded_at_128():
+    # the diagnostics add ~1.5e5 active modes per field; compensated sums stay
+    # within a few ulps of the exactly rounded total (math.fsum), where a plain
+    # running sum drifts by ~n*eps
+    import math
+    g = O.grid3(128, 128, 128)
+    st = O.ic_solenoidal(g, 4141, 0.5)
+    kx, ky, kz, L = O.tables(g)
+    act = L == 1
+    kk = kx ** 2 + ky ** 2 + kz ** 2
+    wk = np.where(kk > 0.001, kk, 0.001)
+    vz = (np.abs(st) ** 2).sum(axis=0)
+    e_ex = math.fsum((vz / wk)[act])
+    z_ex = math.fsum(vz[act])
+    e, z, spec = O.NS3D(128).diag(st)
+    assert abs(e - e_ex) <= 4e-16 * e_ex, (e, e_ex)
+    assert abs(z - z_ex) <= 4e-16 * z_ex, (z, z_ex)
+    j = (np.sqrt(kk) + 0.5).astype(int)
+    for b in range(len(spec)):
+        sel = act & (j == b)
+        want = math.fsum((vz / wk)[sel])
+        assert abs(spec[b] - want) <= 4e-16 * max(want, 1e-300), b
+
+
 def test_diag_2d_parseval():
     g = O.grid2(64, 64)
     z = O.NS2D(64).run(O.ic_ns2d_cfg(g, 77, 1.0), 20)
