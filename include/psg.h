@@ -96,6 +96,16 @@ int psg_sync(psg_ctx *ctx);
  * (2D: 2*sum_L|z|^2 as out.F90), spectrum[j] = sum over shell j=INT(|k|+0.5) of |w|^2/|k|^2
  * (spec may be NULL, else length KTY+2).  3D_NavierStokes/src/functions.h:128-161 */
 int psg_diag(psg_ctx *ctx, int member, double *energy, double *enstrophy, double *spec);
+/* psg_diag without any device allocation: the per-block partial sums live in a
+ * fixed device array of the library (at most 257 x 346 doubles, 711 KB, held
+ * by each device the library runs on until the process ends) instead of the
+ * context's lazily allocated reduction buffer.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * between captured graph launches, tight serving loops, harnesses that forbid
+ * hipMalloc).  The context's device must be current.  Calls that overlap on
+ * one GPU share the array and race.  Results equal psg_diag's.  Grids whose
+ * spectrum needs more than 344 bins are rejected with PSG_EINVAL. */
+int psg_diag_noalloc(psg_ctx *ctx, int member, double *energy, double *enstrophy, double *spec);
 /* spectral nonlinear term dz/dt = N(z)+F (no viscous term), masked, for one member,
  * written into out (nf*nkt complex); cf. TIMEDERIV src_gmres/timederiv.F90:2-44 */
 int psg_nonlin(psg_ctx *ctx, int member, const double *host_in, double *host_out);

@@ -106,6 +106,7 @@ def lib():
     L.psg_set_dt.argtypes = [_P, _D]
     L.psg_sync.argtypes = [_P]
     L.psg_diag.argtypes = [_P, _I, C.POINTER(_D), C.POINTER(_D), _P]
+    L.psg_diag_noalloc.argtypes = [_P, _I, C.POINTER(_D), C.POINTER(_D), _P]
     L.psg_nonlin.argtypes = [_P, _I, _P, _P]
     L.psg_physical.argtypes = [_P, _I, _P]
     L.psg_timederiv.argtypes = [_P, _I, _P, _P]

@@ -9,6 +9,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <math.h>
+#include <mutex>
 #include "psg_launch.hpp"
 #include "../../include/psg_host.h"
 
@@ -174,6 +175,55 @@ int diag_member(Ctx *c, int member, double *e, double *z, double *spec) {
   if (z) *z = Z;
   if (spec) memcpy(spec, sp.data(), sizeof(double) * nbins);
   return 0;
+}
+
+// ------------------------------------------------------------------ diag without allocation
+// psg_diag_noalloc keeps its partial sums in this fixed device array instead of
+// the context's lazily allocated reduction buffer: one row of 2 + bins
+// partials per k_diag block, then one row of totals.  Each device holds its
+// own copy at its own address; host code reaches it only through the address
+// hipGetSymbolAddress returns for that device (na_scratch).  Every row a call
+// reads is written earlier in the same call, so nothing needs clearing; calls
+// that overlap on one device share the array and race.
+constexpr int NA_BLOCKS = 256;
+constexpr int NA_MAX_BINS = 344;   // KTY + 2 of the largest supported grid (1024 points) is 343
+constexpr int NA_ROW = 2 + NA_MAX_BINS;
+static __device__ __align__(256) double scratch[(NA_BLOCKS + 1) * NA_ROW];
+constexpr int NA_MAX_DEV = 64;
+static std::mutex g_na_mu;
+static double *g_na_ptr[NA_MAX_DEV];
+
+// the calling thread's current device must be dev
+static int na_scratch(int dev, double **p) {
+  if (dev < 0 || dev >= NA_MAX_DEV) { set_error("psg_diag_noalloc: device %d out of range", dev); return PSG_EINVAL; }
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("psg_diag_noalloc: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+      return PSG_EHIP;
+    }
+    g_na_ptr[dev] = (double *)ptr;
+  }
+  *p = g_na_ptr[dev];
+  return 0;
+}
+
+// totals of the partial rows, one thread per column, in the block order and
+// compensated arithmetic of diag_member's host sum (so both give the same bits)
+__global__ void k_diag_total(double *__restrict__ part, int blocks, int ncol, double zscale) {
+  for (int col = threadIdx.x; col < ncol; col += blockDim.x) {
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < blocks; b++) {
+      const double x = part[(long)b * ncol + col];
+      const double t = s + x;
+      c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+      s = t;
+    }
+    s += c;
+    part[(long)blocks * ncol + col] = col == 1 ? s * zscale : s;
+  }
 }
 
 static int upload_tables(Ctx *c) {
@@ -573,6 +623,32 @@ int psg_sync(psg_ctx *c) {
 int psg_diag(psg_ctx *c, int m, double *e, double *z, double *spec) {
   if (int r = check_member(c, m)) return r;
   return diag_member(c, m, e, z, spec);
+}
+
+int psg_diag_noalloc(psg_ctx *c, int m, double *e, double *z, double *spec) {
+  if (int r = check_member(c, m)) return r;
+  const int nbins = c->g.kty + 2, ncol = 2 + nbins, blocks = diag_blocks(c);
+  if (nbins > NA_MAX_BINS || blocks > NA_BLOCKS) {
+    set_error("psg_diag_noalloc: %d spectrum bins exceed the fixed scratch of %d", nbins, NA_MAX_BINS);
+    return PSG_EINVAL;
+  }
+  int cur = -1;
+  PSG_HIP(hipGetDevice(&cur));
+  if (cur != c->device) {
+    set_error("psg_diag_noalloc: the context's device %d is not current (%d)", c->device, cur);
+    return PSG_EINVAL;
+  }
+  double *part = nullptr;
+  if (int r = na_scratch(cur, &part)) return r;
+  if (int r = launch_diag(c, m, part)) return r;
+  hipLaunchKernelGGL(k_diag_total, dim3(1), dim3(256), 0, c->stream, part, blocks, ncol, c->g.dim == 2 ? 2.0 : 1.0);
+  PSG_HIP(hipGetLastError());
+  const double *tot = part + (long)blocks * ncol;
+  if (e) PSG_HIP(hipMemcpyAsync(e, tot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (z) PSG_HIP(hipMemcpyAsync(z, tot + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (spec) PSG_HIP(hipMemcpyAsync(spec, tot + 2, sizeof(double) * nbins, hipMemcpyDeviceToHost, c->stream));
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 // Device buffer for the host-array entry points (psg_nonlin, psg_timederiv,

@@ -32,6 +32,8 @@
 #define PSG_WAVES_PER_EU(...)
 #define __shared__ static thread_local
 #define PSG_DYN_SMEM(name) char *name = ::emu::dyn_smem
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {
   unsigned x, y, z;
@@ -121,6 +123,8 @@ inline hipError_t hipMalloc(void **p, size_t n) {
 }
 template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void **)p, n); }
 inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+// a file-scope device array is ordinary host memory here: its address is the symbol's
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { memmove(d, s, n); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t st = nullptr) {
   if (st && st->capturing) {
