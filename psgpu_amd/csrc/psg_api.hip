@@ -244,7 +244,7 @@ static int upload_tables(Ctx *c) {
     PSG_HIP(hipMemcpy(c->nuzn, nuzn.data(), sizeof(double) * g.nkt, hipMemcpyHostToDevice));
     PSG_HIP(hipMemcpy(c->nu1, nu1.data(), sizeof(double) * g.nkt, hipMemcpyHostToDevice));
   }
-  return 0;
+  return ns3d_tables_w(c);
 }
 
 static int upload_twiddles(Ctx *c) {
@@ -269,7 +269,7 @@ static void free_ctx(Ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->graph_exec)
     if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
-  void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->st32, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
+  void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->st32, c->stw, c->wnuzn, c->wnu1, c->wmask, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
                   c->tl_rh, c->tl_bufA, c->tl_bufB, c->tl_bufC, c->bufA, c->bufB, c->bufC,
                   c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred, c->io};
   for (void *p : ptrs)
@@ -337,11 +337,12 @@ static int build_graph(Ctx *c, int kind) {
 
 static int run_steps_in(Ctx *c, int kind, int nsteps);
 
-// the FP32 working state (3D Navier-Stokes/Euler FP32-bulk) is converted once
-// around all of a call's steps, outside the captured graphs
+// the working copy of the 3D Navier-Stokes/Euler state (layout LayW; FP32 in
+// FP32-bulk) is converted once around all of a call's steps, outside the
+// captured graphs
 int run_steps(Ctx *c, int kind, int nsteps) {
   if (nsteps <= 0) return 0;
-  const bool work = kind == 0 && c->st32;
+  const bool work = kind == 0 && (c->st32 || c->stw);
   if (work)
     if (int r0 = ns3d_work_begin(c)) return r0;
   const int r = run_steps_in(c, kind, nsteps);
@@ -611,7 +612,7 @@ int psg_set_tables(psg_ctx *c, const double *nuzn, const double *nu1, const int 
     PSG_HIP(hipMemcpyAsync(c->mask, l8.data(), n, hipMemcpyHostToDevice, c->stream));
     PSG_HIP(hipStreamSynchronize(c->stream));
   }
-  return 0;
+  return ns3d_tables_w(c);
 }
 
 int psg_sync(psg_ctx *c) {

@@ -39,7 +39,7 @@ k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__r
     w[0][mm] = ld_fill(s0, q);
     w[1][mm] = ld_fill(s0 + fs, q);
     w[2][mm] = ld_fill(s0 + 2 * fs, q);
-    lm[mm] = (umask && q.idx >= 0) ? (double)mask[q.idx] : 1.0;
+    lm[mm] = (umask && q.idx >= 0) ? (double)mask[q.tidx] : 1.0;
   }
 #pragma unroll 1
   for (int f = 0; f < nout; f++) {

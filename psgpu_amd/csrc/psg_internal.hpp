@@ -55,8 +55,15 @@ struct Ctx {
   cpx<double> *rh = nullptr;        // Heun: RH store; RK4: stage arrays
   cpx<double> *stage = nullptr;     // RK4: Z0 stage input / generic scratch
   cpx<double> *tmp = nullptr;       // scratch state-sized buffer
-  cpx<float> *st32 = nullptr;       // FP32-bulk 3D Navier-Stokes/Euler: working state of a run
-  bool st32_live = false;           // st32 holds the current state (inside run_steps)
+  // 3D Navier-Stokes/Euler: working copy of the state during a run, in the
+  // z-run layout LayW (FP64, or FP32 in FP32-bulk), and the CN tables and
+  // mask in the same order (nktw modes per field, IKTX rounded up to 8)
+  cpx<double> *stw = nullptr;
+  cpx<float> *st32 = nullptr;
+  bool work_live = false;           // the working copy holds the current state (inside run_steps)
+  long nktw = 0;
+  double *wnuzn = nullptr, *wnu1 = nullptr;
+  int8_t *wmask = nullptr;
   double *nuzn = nullptr, *nu1 = nullptr;  // CN tables (host-built, reference formula)
   double *nu = nullptr;                    // NU = 10^(log10 v2 + 2 log10|k|), for dz/dt
   int8_t *mask = nullptr;           // active mask L
@@ -129,6 +136,7 @@ int ns3d_init(Ctx *c);
 int ns3d_run(Ctx *c, int nsteps, bool first);
 int ns3d_work_begin(Ctx *c);
 int ns3d_work_end(Ctx *c);
+int ns3d_tables_w(Ctx *c);
 int ns3d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
 int ns3d_physical(Ctx *c, int member, double *out_dev);
 int ns2d_init(Ctx *c);

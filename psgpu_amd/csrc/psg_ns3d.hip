@@ -19,7 +19,9 @@ namespace psg {
 // corrector subtracts (the pass precision: in FP32-bulk it only carries
 // FP32-accurate products).  S: storage type of the state (double; float for
 // the working state of an FP32-bulk run).  The update is evaluated in FP64.
-template <int PHASE, bool EULER, typename R = double, typename S = double> struct UpdHeun {
+// LAY: layout of st, rh and the tables (LayNKT: ik is the mode's index;
+// LayW: the working copy, indexed from the mode's wavenumbers through g)
+template <int PHASE, bool EULER, typename R = double, typename S = double, class LAY = LayNKT> struct UpdHeun {
   cpx<S> *st;
   cpx<R> *rh;
   long ms, fs;  // member stride, field stride (same for st and rh)
@@ -27,12 +29,19 @@ template <int PHASE, bool EULER, typename R = double, typename S = double> struc
   const int8_t *L;
   Force F;
   double dt;
-  __device__ __forceinline__ void operator()(int m, long ik, int, double kx, double ky, double kz,
+  Grid g;
+  __device__ __forceinline__ void operator()(int m, long ik_nkt, int ikx, double kx, double ky, double kz,
                                              cpx<double> (&N)[3]) const {
     PSG_CURL
+    long ik = ik_nkt, it = ik_nkt;
+    if constexpr (std::is_same<LAY, LayW>::value) {
+      const int iky = (int)ky + g.kty, ikz = (int)kz + g.ktz;
+      ik = LayW::at(g, ikx, iky, ikz);
+      it = LayW::tab(g, ikx, iky, ikz);
+    }
     cpx<S> *s = st + (long)m * ms + ik;
     cpx<R> *r = rh + (long)m * ms + ik;
-    const double ll = (double)L[ik];
+    const double ll = (double)L[it];
     if constexpr (PHASE == 0) {
       if constexpr (EULER) {
         r[0] = to<R>(cpx<double>{ll * xiX, ll * xiY});
@@ -43,8 +52,8 @@ template <int PHASE, bool EULER, typename R = double, typename S = double> struc
         s[fs] = to<S>(cpx<double>{b.x + dt * etX, b.y + dt * etY});
         s[2 * fs] = to<S>(cpx<double>{c.x + dt * ztX, c.y + dt * ztY});
       } else {
-        const double NUZN = nuzn[ik];
-        const double NU1 = ll * nu1[ik];
+        const double NUZN = nuzn[it];
+        const double NU1 = ll * nu1[it];
         add_forcing(F, kx, ky, kz, xiX, etY, ztX, ztY);
         r[0] = to<R>(cpx<double>{xiX, xiY});
         r[fs] = to<R>(cpx<double>{etX, etY});
@@ -58,7 +67,7 @@ template <int PHASE, bool EULER, typename R = double, typename S = double> struc
       double NU2;
       if constexpr (EULER) NU2 = 0.5 * dt;
       else {
-        NU2 = 0.5 * nu1[ik];
+        NU2 = 0.5 * nu1[it];
         add_forcing(F, kx, ky, kz, xiX, etY, ztX, ztY);
       }
       const cpx<double> a = up(s[0]), b = up(s[fs]), c = up(s[2 * fs]);
@@ -161,14 +170,14 @@ template <typename P> static int xpass_ns(Ctx *c, const cpx<P> *B, cpx<P> *Cb, i
 }
 
 // One right-hand side evaluation for all members, ending in functor upd.
-template <typename P, class UPD, typename S = double>
+template <typename P, class UPD, typename S = double, class LAY = LayNKT>
 static int rhs3(Ctx *c, const cpx<S> *st, long st_ms, int umask, UPD upd, long state_bytes, int nm) {
   const Grid &g = c->g;
   cpx<P> *A = (cpx<P> *)c->bufA, *B = (cpx<P> *)c->bufB, *Cb = (cpx<P> *)c->bufC;
   long pb[5];
   pass_bytes(c, 6, 3, sizeof(cpx<P>), pb, sizeof(cpx<S>));
   prof_begin(c, "zinv", pb[0] * nm);
-  PSG_TRY(launch_zinv<P>(c, st, st_ms, A, 6 * g.zsz, true, umask, nm));
+  PSG_TRY((launch_zinv<P, S, LAY>(c, st, st_ms, A, 6 * g.zsz, true, umask, nm)));
   prof_end(c);
   // passes 2-4 run slab by slab of z planes when the context asks for it, so
   // that the slab's Y and C blocks are read back while they are still held
@@ -195,71 +204,117 @@ static int rhs3(Ctx *c, const cpx<S> *st, long st_ms, int umask, UPD upd, long s
 }
 
 // Heun steps of the state st (storage type S) with the passes in precision P
+// Heun steps of the working copy st (layout LayW, storage type S; RH in the
+// same layout) with the passes in precision P
 template <typename P, typename S> static int run_t(Ctx *c, cpx<S> *st, int nsteps, bool first) {
   const Grid &g = c->g;
   const bool euler = c->cfg.flavour == PSG_EULER3D;
   Force F{c->cfg.force, c->cfg.kf, c->cfg.ampfor};
-  const long ms = 3 * g.nkt, fs = g.nkt;
+  const long ms = 3 * c->nktw, fs = 8;
   // state + RH (pass precision) traffic per mode of each half-step
   const long sb = (6 * (long)sizeof(cpx<S>) + 3 * (long)sizeof(cpx<P>)) * g.nkt;
   cpx<P> *rh = (cpx<P> *)c->rh;
-  const double *nuzn = c->nuzn, *nu1 = c->nu1;
+  const double *nuzn = c->wnuzn, *nu1 = c->wnu1;
+  const int8_t *L = c->wmask;
   for (int nt = 0; nt < nsteps; nt++) {
     const int umask = (first && nt == 0) ? 1 : 0;
     if (euler) {
-      UpdHeun<0, true, P, S> up{st, rh, ms, fs, nuzn, nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, st, ms, umask, up, sb, c->batch));
-      UpdHeun<1, true, P, S> uc{st, rh, ms, fs, nuzn, nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, st, ms, 0, uc, sb, c->batch));
+      UpdHeun<0, true, P, S, LayW> up{st, rh, ms, fs, nuzn, nu1, L, F, c->cfg.dt, g};
+      PSG_TRY((rhs3<P, decltype(up), S, LayW>(c, st, ms, umask, up, sb, c->batch)));
+      UpdHeun<1, true, P, S, LayW> uc{st, rh, ms, fs, nuzn, nu1, L, F, c->cfg.dt, g};
+      PSG_TRY((rhs3<P, decltype(uc), S, LayW>(c, st, ms, 0, uc, sb, c->batch)));
     } else {
-      UpdHeun<0, false, P, S> up{st, rh, ms, fs, nuzn, nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, st, ms, umask, up, sb, c->batch));
-      UpdHeun<1, false, P, S> uc{st, rh, ms, fs, nuzn, nu1, c->mask, F, c->cfg.dt};
-      PSG_TRY(rhs3<P>(c, st, ms, 0, uc, sb, c->batch));
+      UpdHeun<0, false, P, S, LayW> up{st, rh, ms, fs, nuzn, nu1, L, F, c->cfg.dt, g};
+      PSG_TRY((rhs3<P, decltype(up), S, LayW>(c, st, ms, umask, up, sb, c->batch)));
+      UpdHeun<1, false, P, S, LayW> uc{st, rh, ms, fs, nuzn, nu1, L, F, c->cfg.dt, g};
+      PSG_TRY((rhs3<P, decltype(uc), S, LayW>(c, st, ms, 0, uc, sb, c->batch)));
     }
   }
   return 0;
 }
 
-// element-wise precision change of n values (complex or real)
-template <typename A, typename B> __global__ void k_convert(const A *__restrict__ a, B *__restrict__ b, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    if constexpr (std::is_arithmetic<A>::value) b[i] = (B)a[i];
-    else b[i] = B{(decltype(b[i].x))a[i].x, (decltype(b[i].x))a[i].y};
+// State <-> working copy: every entry of the LayW copy of `batch` members
+// (padding columns past IKTX are zero), with the precision change of an
+// FP32-bulk run
+template <typename A, typename B, bool TO_W>
+__global__ void k_work(const cpx<A> *__restrict__ a, cpx<B> *__restrict__ b, Grid g, long nktw, long total) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long m = i / (3 * nktw), w = i - m * 3 * nktw;
+    const int c = (int)(w & 7), f = (int)((w >> 3) % 3);
+    const long run = w / 24;
+    const int ikz = (int)(run % g.iktz);
+    const long t = run / g.iktz;
+    const int iky = (int)(t % g.ikty), ikx = (int)(t / g.ikty) * 8 + c;
+    const long n = m * 3 * g.nkt + f * g.nkt + LayNKT::at(g, ikx < g.iktx ? ikx : 0, iky, ikz);
+    if constexpr (TO_W) {
+      const cpx<A> v = ikx < g.iktx ? a[n] : cpx<A>{0, 0};
+      b[i] = cpx<B>{(B)v.x, (B)v.y};
+    } else if (ikx < g.iktx) {
+      b[n] = cpx<B>{(B)a[i].x, (B)a[i].y};
+    }
   }
 }
 
-template <typename A, typename B> static int convert(Ctx *c, const A *a, B *b, long n) {
-  hipLaunchKernelGGL((k_convert<A, B>), dim3((unsigned)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0,
-                     c->stream, a, b, n);
+template <typename A, typename B, bool TO_W> static int work_copy(Ctx *c, const cpx<A> *a, cpx<B> *b) {
+  const long total = 3 * c->nktw * c->batch;
+  hipLaunchKernelGGL((k_work<A, B, TO_W>), dim3((unsigned)std::min<long>(8192, (total + 255) / 256)), dim3(256), 0,
+                     c->stream, a, b, c->g, c->nktw, total);
   PSG_HIP(hipGetLastError());
   return 0;
 }
 
-// FP32-bulk steps a float copy of the state (st32): FP64 is kept for the
-// state between runs, where Newton's residual, the diagnostics and the API
-// read it; a run starts from the state rounded to FP32.  Between
-// ns3d_work_begin and ns3d_work_end the copy is live and ns3d_run steps it
-// without converting (run_steps brackets a call's eager steps and graph
-// replays this way).
+// CN tables and mask in the working layout (zero at padding columns)
+__global__ void k_tables_w(const double *__restrict__ nuzn, const double *__restrict__ nu1,
+                           const int8_t *__restrict__ L, double *__restrict__ wnuzn, double *__restrict__ wnu1,
+                           int8_t *__restrict__ wL, Grid g, long nktw) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nktw; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i & 7);
+    const long run = i >> 3;
+    const int ikz = (int)(run % g.iktz);
+    const long t = run / g.iktz;
+    const int iky = (int)(t % g.ikty), ikx = (int)(t / g.ikty) * 8 + c;
+    const bool in = ikx < g.iktx;
+    const long n = LayNKT::at(g, in ? ikx : 0, iky, ikz);
+    wnuzn[i] = in ? nuzn[n] : 0.0;
+    wnu1[i] = in ? nu1[n] : 0.0;
+    wL[i] = in ? L[n] : (int8_t)0;
+  }
+}
+
+int ns3d_tables_w(Ctx *c) {
+  if (!c->wnuzn) return 0;
+  hipLaunchKernelGGL(k_tables_w, dim3((unsigned)std::min<long>(4096, (c->nktw + 255) / 256)), dim3(256), 0, c->stream,
+                     c->nuzn, c->nu1, c->mask, c->wnuzn, c->wnu1, c->wmask, c->g, c->nktw);
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+// A run steps a working copy of the state in the LayW layout (FP32 in
+// FP32-bulk, where a run starts from the state rounded to FP32): the FP64
+// state in the reference order is kept between runs, where Newton's
+// residual, the diagnostics and the API read it.  Between ns3d_work_begin
+// and ns3d_work_end the copy is live and ns3d_run steps it without
+// converting (run_steps brackets a call's eager steps and graph replays
+// this way).
 int ns3d_work_begin(Ctx *c) {
-  if (!c->st32 || c->st32_live) return 0;
-  PSG_TRY((convert<cpx<double>, cpx<float>>(c, c->state, c->st32, 3 * c->g.nkt * c->batch)));
-  c->st32_live = true;
+  if (c->work_live) return 0;
+  if (c->st32) PSG_TRY((work_copy<double, float, true>(c, c->state, c->st32)));
+  else PSG_TRY((work_copy<double, double, true>(c, c->state, c->stw)));
+  c->work_live = true;
   return 0;
 }
 
 int ns3d_work_end(Ctx *c) {
-  if (!c->st32_live) return 0;
-  c->st32_live = false;
-  return convert<cpx<float>, cpx<double>>(c, c->st32, c->state, 3 * c->g.nkt * c->batch);
+  if (!c->work_live) return 0;
+  c->work_live = false;
+  if (c->st32) return work_copy<float, double, false>(c, c->st32, c->state);
+  return work_copy<double, double, false>(c, c->stw, c->state);
 }
 
 int ns3d_run(Ctx *c, int nsteps, bool first) {
-  if (c->cfg.precision != PSG_FP32BULK) return run_t<double, double>(c, c->state, nsteps, first);
-  const bool own = !c->st32_live;
+  const bool own = !c->work_live;
   if (own) PSG_TRY(ns3d_work_begin(c));
-  const int r = run_t<float, float>(c, c->st32, nsteps, first);
+  const int r = c->st32 ? run_t<float, float>(c, c->st32, nsteps, first) : run_t<double, double>(c, c->stw, nsteps, first);
   if (!own) return r;
   const int r2 = ns3d_work_end(c);
   return r ? r : r2;
@@ -275,10 +330,10 @@ template <typename P> static int step_rec_t(Ctx *c, int umask, cpx<double> *wn, 
   const size_t bytes = sizeof(cpx<double>) * 3 * g.nkt;
   if (wn) PSG_HIP(hipMemcpyAsync(wn, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
   const long sb = (6 * 16 + 3 * (long)sizeof(cpx<P>)) * g.nkt;
-  UpdHeun<0, false, P> up{c->state, (cpx<P> *)c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+  UpdHeun<0, false, P> up{c->state, (cpx<P> *)c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt, c->g};
   PSG_TRY(rhs3<P>(c, c->state, ms, umask, up, sb, 1));
   if (wstar) PSG_HIP(hipMemcpyAsync(wstar, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
-  UpdHeun<1, false, P> uc{c->state, (cpx<P> *)c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt};
+  UpdHeun<1, false, P> uc{c->state, (cpx<P> *)c->rh, ms, fs, c->nuzn, c->nu1, c->mask, F, c->cfg.dt, c->g};
   PSG_TRY(rhs3<P>(c, c->state, ms, 0, uc, sb, 1));
   return 0;
 }
@@ -370,10 +425,18 @@ int ns3d_init(Ctx *c) {
   PSG_HIP(hipMalloc(&c->bufA, c->szA));
   PSG_HIP(hipMalloc(&c->bufB, c->szB));
   PSG_HIP(hipMalloc(&c->bufC, c->szC));
-  PSG_HIP(hipMalloc(&c->rh, sizeof(cpx<double>) * 3 * g.nkt * c->batch));
-  PSG_HIP(hipMemsetAsync(c->rh, 0, sizeof(cpx<double>) * 3 * g.nkt * c->batch, c->stream));
-  if (c->cfg.precision == PSG_FP32BULK) PSG_HIP(hipMalloc(&c->st32, sizeof(cpx<float>) * 3 * g.nkt * c->batch));
-  return 0;
+  // RH serves the working-layout stepper and the reference-order steps of
+  // the adjoint's recorded trajectory: the larger of the two sizes
+  c->nktw = (long)((g.iktx + 7) / 8) * 8 * g.ikty * g.iktz;
+  const long nrh = 3 * std::max<long>(c->nktw, g.nkt) * c->batch;
+  PSG_HIP(hipMalloc(&c->rh, sizeof(cpx<double>) * nrh));
+  PSG_HIP(hipMemsetAsync(c->rh, 0, sizeof(cpx<double>) * nrh, c->stream));
+  if (c->cfg.precision == PSG_FP32BULK) PSG_HIP(hipMalloc(&c->st32, sizeof(cpx<float>) * 3 * c->nktw * c->batch));
+  else PSG_HIP(hipMalloc(&c->stw, sizeof(cpx<double>) * 3 * c->nktw * c->batch));
+  PSG_HIP(hipMalloc(&c->wnuzn, sizeof(double) * c->nktw));
+  PSG_HIP(hipMalloc(&c->wnu1, sizeof(double) * c->nktw));
+  PSG_HIP(hipMalloc(&c->wmask, c->nktw));
+  return ns3d_tables_w(c);
 }
 
 }  // namespace psg

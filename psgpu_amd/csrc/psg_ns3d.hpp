@@ -2,6 +2,7 @@
 // and the launch helpers of the five pencil passes.  Included by the stepper
 // and by the tangent-linear/adjoint translation unit.
 #pragma once
+#include <type_traits>
 #include "psg_launch.hpp"
 
 namespace psg {
@@ -40,24 +41,29 @@ __device__ __forceinline__ void add_forcing(const Force &F, double kx, double ky
 template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
 template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
 
-template <typename P, typename S = double>
+// LAY: layout of the state st (LayNKT: the context's state and caller
+// buffers; LayW: the stepper's working copy, with the mask in LayW order)
+template <typename P, typename S = double, class LAY = LayNKT>
 static int launch_zinv(Ctx *c, const cpx<S> *st, long st_ms, cpx<P> *out, long out_ms, bool bs, int umask,
                        int nm) {
   const Grid &g = c->g;
+  constexpr bool W = std::is_same<LAY, LayW>::value;
+  const long st_fs = W ? 8 : g.nkt;
+  const int8_t *mask = W ? c->wmask : c->mask;
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     if (bs) {
-      auto k = k_zinv<P, N, T, true, S>;
+      auto k = k_zinv<P, N, T, true, S, LAY>;
       PSG_HIP(lds_attr(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
-                         twp<P>(c, 2), c->mask, umask);
+      hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, st_fs, out, out_ms, g,
+                         twp<P>(c, 2), mask, umask);
     } else {
-      auto k = k_zinv<P, N, T, false, S>;
+      auto k = k_zinv<P, N, T, false, S, LAY>;
       PSG_HIP(lds_attr(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, g.nkt, out, out_ms, g,
-                         twp<P>(c, 2), c->mask, umask);
+      hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, st_fs, out, out_ms, g,
+                         twp<P>(c, 2), mask, umask);
     }
   });
   PSG_HIP(hipGetLastError());

@@ -188,17 +188,48 @@ __device__ __forceinline__ void biot_savart(double kx, double ky, double kz, cpx
 // vorticity component, two for a velocity component); the block's state tile
 // stays cache-resident, so HBM sees each state value once while the kernel
 // keeps only one pencil's worth of registers live.
+// Storage layouts of a 3D state (per member; field f of mode (ikx, iky, ikz)
+// at at() + f fs(), and per-mode tables at tab()):
+//  LayNKT: the reference order, field-major, ik = (ikz IKTY + iky) IKTX + ikx
+//  LayW:   the stepper's working copy: runs along z of 8-column tiles with the
+//          three fields of a mode adjacent, [tile][iky][ikz][field][8].  A
+//          z-pass block (one tile, one iky) then reads and writes one
+//          contiguous run of IKTZ x 3 x 8 points instead of three 8-point
+//          pieces per kz spread over the three fields.
+struct LayNKT {
+  static __device__ __forceinline__ long at(const Grid &g, int ikx, int iky, int ikz) {
+    return ((long)ikz * g.ikty + iky) * g.iktx + ikx;
+  }
+  static __device__ __forceinline__ long tab(const Grid &g, int ikx, int iky, int ikz) { return at(g, ikx, iky, ikz); }
+  static __device__ __forceinline__ long fs(const Grid &g) { return g.nkt; }
+};
+struct LayW {
+  static __device__ __forceinline__ long run(const Grid &g, int ikx, int iky, int ikz) {
+    return ((long)(ikx >> 3) * g.ikty + iky) * g.iktz + ikz;
+  }
+  static __device__ __forceinline__ long at(const Grid &g, int ikx, int iky, int ikz) {
+    return run(g, ikx, iky, ikz) * 24 + (ikx & 7);
+  }
+  static __device__ __forceinline__ long tab(const Grid &g, int ikx, int iky, int ikz) {
+    return run(g, ikx, iky, ikz) * 8 + (ikx & 7);
+  }
+  static __device__ __forceinline__ long fs(const Grid &) { return 8; }
+};
+
 struct FillIdx {
-  long idx;     // stored index to read (mirror for the filled half-plane), -1 = zero
+  long idx;     // field-0 index to read (mirror for the filled half-plane), -1 = zero
+  long tidx;    // the same mode's index in the per-mode tables
   bool conj;    // conjugate on read
 };
+template <class LAY = LayNKT>
 __device__ __forceinline__ FillIdx fill_index(const Grid &g, int ikx, int iky, int ky, int kz) {
-  FillIdx r{-1, false};
+  FillIdx r{-1, -1, false};
   if (kz == PSG_NOK) return r;
   if (ikx == 0 && ky == 0 && kz == 0) return r;
   const bool mirror = (ikx == 0) && (ky < 0 || (ky == 0 && kz < 0));
-  r.idx = mirror ? ((long)(-kz + g.ktz) * g.ikty + (-ky + g.kty)) * g.iktx
-                 : ((long)(kz + g.ktz) * g.ikty + iky) * g.iktx + ikx;
+  const int sx = mirror ? 0 : ikx, sy = mirror ? -ky + g.kty : iky, sz = mirror ? -kz + g.ktz : kz + g.ktz;
+  r.idx = LAY::at(g, sx, sy, sz);
+  r.tidx = LAY::tab(g, sx, sy, sz);
   r.conj = mirror;
   return r;
 }
@@ -212,7 +243,8 @@ __device__ __forceinline__ cpx<S> ld_fill(const cpx<S> *__restrict__ f, const Fi
 
 // S: the state's storage type (double; float for the FP32 working state of
 // an FP32-bulk run)
-template <typename P, int N, int T, bool BS, typename S = double>
+// LAY: the state's layout (st_fs its field stride); mask is in LAY's table order
+template <typename P, int N, int T, bool BS, typename S = double, class LAY = LayNKT>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv(const cpx<S> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
        const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask) {
@@ -233,11 +265,11 @@ k_zinv(const cpx<S> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict
 #pragma unroll
   for (int mm = 0; mm < EPT; mm++) {
     const int kz = valid ? wrapk(j + TP * mm, g.nz, g.ktz) : PSG_NOK;
-    const FillIdx q = fill_index(g, ikx, iky, ky, kz);
+    const FillIdx q = fill_index<LAY>(g, ikx, iky, ky, kz);
     w[0][mm] = to<P>(ld_fill(s0, q));
     w[1][mm] = to<P>(ld_fill(s0 + st_fs, q));
     w[2][mm] = to<P>(ld_fill(s0 + 2 * st_fs, q));
-    lm[mm] = (umask && q.idx >= 0) ? (P)mask[q.idx] : (P)1;
+    lm[mm] = (umask && q.idx >= 0) ? (P)mask[q.tidx] : (P)1;
   }
   constexpr int NFO = BS ? 6 : 3;
 #pragma unroll
