@@ -51,7 +51,7 @@ def log(*a):
 
 # rows of the traffic summary for the FP64 pass names of the step profile
 TRAFFIC_ROW = {"zinv": "k_zinv<double, 256", "yinv": "k_yinv<double, 256", "xpass": "k_xpass<double, 256",
-               "yfwd": "k_yfwd<double, 256"}
+               "yfwd": "k_yfwd<double, 256", "zfwd_update": "k_zfwd<double, 256"}
 
 
 def measured_traffic(kernel_prefix):
@@ -106,7 +106,7 @@ def time_form(P, st0, n, precision, steps, warmup, tune=True, prof_steps=20):
         s.sync()
         prof = s.profile_report()
         s.profile(False)
-    variant = {"xpass": s.get_variant(0), "zfwd": s.get_variant(1)}
+    variant = {"xpass": s.get_variant(0), "zfwd": s.get_variant(1), "slab": s.get_variant(2)}
     s.close()
     return dt, prof, e, variant
 
@@ -128,13 +128,14 @@ def energy_drift(P, st0, n, nsteps=1000, every=100):
             "rel_err_max": max(rel)}
 
 
-def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-4):
+def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-4, eps2=1e-6):
     """Gradient of J = ||Phi_T(u0) - u0||^2 at 128^3, 200 steps, checkpoints every 20,
     against fourth-order central differences along seeded solenoidal directions (FD
     with the FP64 stepper); FP64 and FP32-bulk adjoints.  With J ~ 10 and h = 1e-4 the
-    FD's round-off is ~1e-10 and its truncation error O(h^4): a second-order
-    difference at h = 1e-6 would carry ~5e-9 of round-off, 5e-6 relative on a
-    direction whose derivative is 1e-3."""
+    FD's round-off is ~1e-10 and its truncation error O(h^4).  The configuration's
+    literal check, second-order central differences at eps2 = 1e-6, is reported
+    beside it ("cd2_*"): its round-off, ~5e-9, is 5e-6 relative on a direction whose
+    derivative is 1e-3, so its error is also stated over ||grad||."""
     g = P.hgrid("ns3d", n, n, n)
     L = P.tables(g)[3].astype(float)
     z0 = P.ic_solenoidal(g, 20240603, 0.5)
@@ -154,7 +155,7 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-4):
         t0 = time.perf_counter()
         J(z0)
         out["forward_seconds"] = time.perf_counter() - t0
-        errs64, errs32, fds, nerr64 = [], [], [], []
+        errs64, errs32, fds, nerr64, cd2, cd2n = [], [], [], [], [], []
         gnorm = float(np.sqrt(np.sum(np.abs(g64) ** 2 * L[None, :])))
         with P.Stepper(flavour="ns3d", n=n, v2=0.01, dt=0.002, force="kolmogorov", precision="fp32") as s32:
             s32.residual_grad(z0, nsteps, ckpt)
@@ -167,6 +168,9 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-4):
             fd = (8 * (J(z0 + eps * v) - J(z0 - eps * v)) - (J(z0 + 2 * eps * v) - J(z0 - 2 * eps * v))) / (12 * eps)
             ip = lambda a: float(np.sum((np.conj(a) * v).real * L[None, :]))
             fds.append(fd)
+            c2 = (J(z0 + eps2 * v) - J(z0 - eps2 * v)) / (2 * eps2)
+            cd2.append(abs(ip(g64) - c2) / abs(c2))
+            cd2n.append(abs(ip(g64) - c2) / gnorm)
             errs64.append(abs(ip(g64) - fd) / abs(fd))
             errs32.append(abs(ip(g32) - fd) / abs(fd))
             nerr64.append(abs(ip(g64) - fd) / gnorm)
@@ -174,7 +178,9 @@ def adjoint_case(P, n=128, nsteps=200, ckpt=20, ndir=16, eps=1e-4):
     # inflate when a direction happens to be nearly orthogonal to the gradient
     out.update({"J": J0, "grad_norm": gnorm, "fp64_max_rel_err_vs_fd": max(errs64),
                 "fp64_median_rel_err_vs_fd": float(np.median(errs64)), "fp32_max_rel_err_vs_fd": max(errs32),
-                "fp64_max_err_over_grad_norm": max(nerr64), "fd_values": fds})
+                "fp64_max_err_over_grad_norm": max(nerr64), "fd_values": fds,
+                "cd2_eps": eps2, "cd2_fp64_max_rel_err": max(cd2), "cd2_fp64_median_rel_err": float(np.median(cd2)),
+                "cd2_fp64_max_err_over_grad_norm": max(cd2n)})
     return out
 
 

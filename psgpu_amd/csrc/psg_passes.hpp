@@ -636,8 +636,8 @@ k_xpass_split(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow
       const P h = (P)0.5;
       cpx<P> A{h * (zk.x + zc.x), h * (zk.y - zc.y)};
       cpx<P> B{h * (zk.y + zc.y), h * (zc.x - zk.x)};
-      st_pass<NT<P, PASS_XPASS>::st>(&oa[k], A);
-      if (2 * q + 1 < FO) st_pass<NT<P, PASS_XPASS>::st>(&ob[k], B);
+      st_pass<false>(&oa[k], A);   // 2D grids: the y-forward reads C back from the caches
+      if (2 * q + 1 < FO) st_pass<false>(&ob[k], B);
     }
   }
 }
@@ -736,7 +736,7 @@ k2_yinv(const cpx<double> *__restrict__ st, long st_ms, cpx<P> *__restrict__ out
       if (valid) {
         cpx<P> *o = out + (long)m * out_ms + (long)(3 * src + f) * g.ysz;
 #pragma unroll
-        for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_YINV>::st>(&o[yoff<P>(g, ikx, j + TP * mm, 0)], v[mm]);
+        for (int mm = 0; mm < EPT; mm++) st_pass<false>(&o[yoff<P>(g, ikx, j + TP * mm, 0)], v[mm]);   // 2D: the next pass reads it from cache
       }
     }
   }
