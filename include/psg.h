@@ -17,6 +17,13 @@
  *
  * Errors: every function returns 0 on success or a negative PSG_E* code;
  * psg_last_error() returns a message for the calling thread's last failure.
+ *
+ * Threads: a context owns its HIP stream, buffers, graphs and profiling
+ * events, and must be driven by one host thread at a time (calls on one
+ * context from several threads at once are not serialised and race).
+ * Different contexts may be driven from different threads; the state shared
+ * between contexts (the LDS attribute cache, the per-device scratch of the
+ * allocation-free diagnostics) is guarded by mutexes.
  */
 #ifndef PSG_H
 #define PSG_H
