@@ -141,6 +141,18 @@ void orb_run(orb_ctx *c, orc *xii, orc *eta, orc *zet, orc *rho, int nsteps);
 void orb_rhs(orb_ctx *c, const orc *xii, const orc *eta, const orc *zet, const orc *rho,
              orc *dxii, orc *deta, orc *dzet, orc *drho);
 
+/* ---------------- recurrence search (or_rec.c) ----------------
+ * rule 0: 3D Navier-Stokes/Euler, 1: Boussinesq, 2: 2D.  L: active mask per
+ * stored mode (NKT bytes).  state0/state: nf*NKT complex.  See or_rec.c. */
+typedef struct or_rec or_rec;
+or_rec *or_rec_new(int rule, int nf, int iktx, int ikty, int iktz, int kty, int ktz, double alpha, const int8_t *L,
+                   int store_size, int norm_width, int nsx, int nsy, int nsz, int nout, double dt0, double tstart,
+                   double thr, double ri, double v2, const double *state0);
+void or_rec_free(or_rec *r);
+int or_rec_update(or_rec *r, long NT, double delt, const double *state, double *guess, double *guess_state,
+                  double *chk);
+int or_rec_last(const or_rec *r, double *res, int *slots, int cap);
+
 /* ---------------- threading ---------------- */
 void or_set_threads(int n);
 int  or_get_threads(void);

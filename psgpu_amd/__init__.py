@@ -69,6 +69,25 @@ class FloquetResult(C.Structure):
     _fields_ = [("nconv", C.c_int), ("matvecs", C.c_int), ("cycles", C.c_int), ("seconds", C.c_double)]
 
 
+class RecOpts(C.Structure):
+    """psg_rec_opts (include/psg_rec.h)."""
+    _fields_ = [("rule", C.c_int), ("member", C.c_int), ("store_size", C.c_int), ("norm_width", C.c_int),
+                ("nshift_x", C.c_int), ("nshift_y", C.c_int), ("nshift_z", C.c_int), ("nout", C.c_int),
+                ("tstart", C.c_double), ("threshold", C.c_double), ("ri", C.c_double), ("keep_states", C.c_int)]
+
+
+class RecGuess(C.Structure):
+    _fields_ = [("index", C.c_int), ("store", C.c_int), ("time", C.c_double), ("v2", C.c_double),
+                ("period", C.c_double), ("sx", C.c_double), ("sy", C.c_double), ("sz", C.c_double),
+                ("residual", C.c_double), ("time_under", C.c_double)]
+
+
+class RecCheck(C.Structure):
+    _fields_ = [("checked", C.c_int), ("compared", C.c_int), ("store", C.c_int), ("count", C.c_int),
+                ("residual", C.c_double), ("sx", C.c_double), ("sy", C.c_double), ("sz", C.c_double),
+                ("time", C.c_double)]
+
+
 NEWTON_STATUS = {0: "CONVERGED", 1: "GMRES FAILED", 2: "H-STEP FAILED", 3: "T-STEP FAILED", 4: "NEWTON STALLED",
                  5: "PERIOD TOO SMALL", 6: "NEWTON FAILED"}
 
@@ -157,6 +176,12 @@ def lib():
     L.psg_newton_solve.argtypes = [_P, _P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(NewtonOpts),
                                    C.POINTER(NewtonResult)]
     L.psg_flow_map.argtypes = [_P, _P, _D, _D, _I]
+    L.psg_rec_defaults.argtypes = [_P, C.POINTER(RecOpts)]
+    L.psg_rec_create.argtypes = [_P, C.POINTER(RecOpts), C.POINTER(_P)]
+    L.psg_rec_update.argtypes = [_P, _L, _D, C.POINTER(RecGuess), _P, C.POINTER(RecCheck)]
+    L.psg_rec_last.argtypes = [_P, _P, _P, C.POINTER(_I), _I]
+    L.psg_rec_state.argtypes = [_P, _I, _P]
+    L.psg_rec_destroy.argtypes = [_P]
     L.psg_tangent.argtypes = [_P, _P, _P, _D, _D, _I, _P, _P]
     _lib = L
     return L
@@ -466,3 +491,57 @@ class Stepper:
             rows.append({"kernel": nm.value.decode(), "avg_ms": ms.value, "bytes": b.value, "count": cnt.value})
             i += 1
         return rows
+
+
+def _struct_dict(s):
+    return {name: getattr(s, name) for name, _ in s._fields_}
+
+
+class Recurrence:
+    """Recurrence search over the DNS history of one context member
+    (include/psg_rec.h; the reference's recurrence_check).  Call update(nt, dt)
+    at every NOUT block of the time loop, after the steps up to it; it returns
+    (guess dict or None, guess state or None, check dict)."""
+
+    def __init__(self, stepper, **opts):
+        self.st = stepper
+        o = RecOpts()
+        _check(lib().psg_rec_defaults(stepper.h, C.byref(o)), "psg_rec_defaults")
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise TypeError("unknown recurrence option %r" % k)
+            setattr(o, k, v)
+        self.opts = o
+        self.ns = o.nshift_x * o.nshift_y * o.nshift_z
+        h = _P()
+        _check(lib().psg_rec_create(stepper.h, C.byref(o), C.byref(h)), "psg_rec_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psg_rec_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def update(self, nt, dt, want_state=False):
+        g, c = RecGuess(), RecCheck()
+        gs = np.zeros((self.st.nf, self.st.nkt), np.complex128) if want_state else None
+        r = lib().psg_rec_update(self.h, int(nt), float(dt), C.byref(g), _ptr(gs) if want_state else None,
+                                 C.byref(c))
+        _check(min(r, 0), "psg_rec_update")
+        return (_struct_dict(g) if r == 1 else None), (gs if r == 1 else None), _struct_dict(c)
+
+    def last(self):
+        """Residual table of the last check, (compared stores x shifts), and the stores' slots."""
+        cap = self.opts.store_size
+        res = np.zeros(cap * self.ns)
+        slots = np.zeros(cap, np.int32)
+        nj = C.c_int()
+        _check(lib().psg_rec_last(self.h, _ptr(res), _ptr(slots), C.byref(nj), cap), "psg_rec_last")
+        return res[:nj.value * self.ns].reshape(nj.value, self.ns), slots[:nj.value]
+
+    def state(self, store):
+        s = np.zeros((self.st.nf, self.st.nkt), np.complex128)
+        _check(lib().psg_rec_state(self.h, int(store), _ptr(s)), "psg_rec_state")
+        return s

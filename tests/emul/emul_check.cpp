@@ -1,6 +1,6 @@
 // Runs the product C-ABI, built for the CPU against tests/emul/hip/hip_runtime.h
 // with AddressSanitizer, over every flavour, precision, x-pass shape variant,
-// graph path, tangent-linear/adjoint and Newton entry point at small sizes,
+// graph path, tangent-linear/adjoint, Newton and recurrence-search entry point at small sizes,
 // and checks the results against the FP64 oracle.  Exit status 0 = all passed.
 // TEST INFRASTRUCTURE ONLY.
 #include <math.h>
@@ -14,6 +14,7 @@
 #include "psg.h"
 #include "psg_adjoint.h"
 #include "psg_newton.h"
+#include "psg_rec.h"
 
 typedef std::vector<double> V;
 static int g_fail = 0, g_pass = 0;
@@ -233,6 +234,100 @@ static void floquet_case(int ncv, int jv) {
   check(rc == 0 && mr[0] > 1.0 && mi[0] == 0.0, "floquet 2d laminar Kolmogorov (TL Jv, ncv 200)", er[0]);
 }
 
+// Recurrence search through the C-ABI against the oracle on a random history
+// (the states are set with psg_set_state; no stepping): residual tables of every
+// check, check summaries and guess records.
+static void rec_case(const char *name, int flav, int n, int rule, int W, int nsx, int nsy, int nsz, double ri,
+                     int store, int entries) {
+  const bool two = flav == PSG_NS2D;
+  psg_config c = cfg3(flav, n, n, two ? 1 : n, 1.0, PSG_FP64, 1, 0.02, 0.01, PSG_FORCE_KOLMO);
+  c.ri = ri;
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, name, 1); return; }
+  or_grid g;
+  if (two) or_grid2(&g, n, n, 1.0);
+  else or_grid3(&g, n, n, n, 1.0);
+  std::vector<double> kx(g.nkt), ky(g.nkt), kz(g.nkt);
+  std::vector<int> L(g.nkt);
+  or_tables(&g, kx.data(), ky.data(), kz.data(), L.data());
+  std::vector<int8_t> L8(L.begin(), L.end());
+  const int nf = two ? 1 : (flav == PSG_BOUS3D ? 4 : 3);
+  const long nk = (long)nf * g.nkt;
+  std::vector<V> hist(entries, V(2 * nk));
+  uint64_t x = 88172645463325252ull;
+  for (auto &h : hist)
+    for (long i = 0; i < nk; i++) {
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      const double a = (double)(x >> 11) / 9007199254740992.0 - 0.5;
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      const double b = (double)(x >> 11) / 9007199254740992.0 - 0.5;
+      h[2 * i] = a * L[i % g.nkt];
+      h[2 * i + 1] = b * L[i % g.nkt];
+    }
+  // plant near-recurrences: entry e (e % 4 == 1) is entry e-lag shifted by the
+  // x-shift 5 of the grid plus 1% noise, so the check that follows it has a
+  // small best residual and the next one a large one
+  // (the Boussinesq rule skips periods below 0.5, i.e. lags under 5 entries: lag 6 there)
+  const double sx5 = 2.0 * M_PI * 5 / nsx;
+  const int lag = flav == PSG_BOUS3D ? 6 : 3;
+  for (int e = lag + 2; e < entries; e++) {
+    if (e % 4 != 1) continue;
+    for (long i = 0; i < nk; i++) {
+      const double ph = kx[i % g.nkt] * sx5, cr = cos(ph), ci = sin(ph);
+      const double ar = hist[e - lag][2 * i], ai = hist[e - lag][2 * i + 1];
+      hist[e][2 * i] = cr * ar - ci * ai + 0.01 * hist[e][2 * i];
+      hist[e][2 * i + 1] = cr * ai + ci * ar + 0.01 * hist[e][2 * i + 1];
+    }
+  }
+  psg_set_state(ctx, 0, hist[0].data());
+  psg_rec_opts o;
+  psg_rec_defaults(ctx, &o);
+  o.rule = rule; o.store_size = store; o.norm_width = W; o.nshift_x = nsx; o.nshift_y = nsy; o.nshift_z = nsz;
+  o.nout = 10; o.threshold = 0.5; o.ri = ri;
+  psg_rec *r;
+  if (psg_rec_create(ctx, &o, &r)) { printf("rec: %s\n", psg_last_error()); check(false, name, 1); psg_destroy(ctx); return; }
+  or_rec *orr = or_rec_new(rule, nf, g.iktx, g.ikty, two ? 1 : g.iktz, g.kty, two ? 0 : g.ktz, 1.0, L8.data(), store,
+                           W, nsx, nsy, nsz, 10, 0.01, 0.0, 0.5, ri, 0.02, hist[0].data());
+  const int ns = nsx * nsy * nsz;
+  double worst = 0.0;
+  bool same = true;
+  int guesses = 0;
+  for (int e = 1; e < entries; e++) {
+    psg_set_state(ctx, 0, hist[e].data());
+    psg_rec_guess gg;
+    psg_rec_check cc;
+    V gs(2 * nk), ogs(2 * nk);
+    const int a = psg_rec_update(r, 10L * e, 0.01, &gg, gs.data(), &cc);
+    double og[10], oc[9];
+    const int b = or_rec_update(orr, 10L * e, 0.01, hist[e].data(), og, ogs.data(), oc);
+    if (a != b || cc.compared != (int)oc[1] || cc.store != (int)oc[2] || cc.count != (int)oc[3]) same = false;
+    if (a == 1) {
+      guesses++;
+      if (gg.index != (int)og[0] || gg.store != (int)og[1] || fabs(gg.period - og[4]) > 1e-12 || gs != ogs)
+        same = false;
+    }
+    std::vector<double> res((size_t)store * ns), ores((size_t)store * ns);
+    std::vector<int> sl(store), osl(store);
+    int nj = 0;
+    psg_rec_last(r, res.data(), sl.data(), &nj, store);
+    const int onj = or_rec_last(orr, ores.data(), osl.data(), store);
+    if (nj != onj) same = false;
+    for (int j = 0; j < nj && j < onj; j++) {
+      if (sl[j] != osl[j]) same = false;
+      for (int s2 = 0; s2 < ns; s2++) {
+        // compare res^2 = normFN / norm (an O(1) quantity whose rounding is absolute)
+        const double a2 = res[(size_t)j * ns + s2], b2 = ores[(size_t)j * ns + s2];
+        worst = fmax(worst, fabs(a2 * a2 - b2 * b2));
+      }
+    }
+  }
+  psg_rec_destroy(r);
+  or_rec_free(orr);
+  psg_destroy(ctx);
+  printf("%s: %d guesses, decisions %s\n", name, guesses, same ? "equal" : "DIFFER");
+  check(same && worst < 1e-13, name, worst);
+}
+
 int main(int argc, char **argv) {
   omp_set_dynamic(0);
   omp_set_max_active_levels(1);
@@ -246,6 +341,14 @@ int main(int argc, char **argv) {
   }
   if (argc > 1 && strcmp(argv[1], "floquet") == 0) {
     floquet_case(200, 1);
+    printf("%d passed, %d failed\n", g_pass, g_fail);
+    return g_fail ? 1 : 0;
+  }
+  if (argc > 1 && strcmp(argv[1], "rec") == 0) {
+    rec_case("rec ns3d 32^3 W 8 shifts 30x2x2 ring 6", PSG_NS3D, 32, PSG_REC_NS, 8, 30, 2, 2, 0.0, 6, 14);
+    rec_case("rec bous 32^3 W 8 shifts 64 ring 40", PSG_BOUS3D, 32, PSG_REC_BOUS, 8, 64, 1, 1, 0.7, 40, 12);
+    rec_case("rec bous 32^3 W 8 shifts 64 ring 8 (wraps)", PSG_BOUS3D, 32, PSG_REC_BOUS, 8, 64, 1, 1, 0.7, 8, 18);
+    rec_case("rec ns2d 128^2 W 42 shifts 30x4 ring 8", PSG_NS2D, 128, PSG_REC_NS2D, 42, 30, 4, 1, 0.0, 8, 12);
     printf("%d passed, %d failed\n", g_pass, g_fail);
     return g_fail ? 1 : 0;
   }
@@ -293,6 +396,9 @@ int main(int argc, char **argv) {
   if (!quick) tladj_case(16, PSG_FP32BULK);
   newton2d_case(0);
   newton2d_case(1);
+  rec_case("rec ns3d 32^3 W 8 shifts 30x2x2 ring 6", PSG_NS3D, 32, PSG_REC_NS, 8, 30, 2, 2, 0.0, 6, 14);
+  rec_case("rec bous 32^3 W 8 shifts 64 ring 8 (wraps)", PSG_BOUS3D, 32, PSG_REC_BOUS, 8, 64, 1, 1, 0.7, 8, 18);
+  rec_case("rec ns2d 128^2 W 42 shifts 30x4 ring 8", PSG_NS2D, 128, PSG_REC_NS2D, 42, 30, 4, 1, 0.0, 8, 12);
   printf("%d passed, %d failed\n", g_pass, g_fail);
   return g_fail ? 1 : 0;
 }

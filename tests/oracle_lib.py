@@ -79,6 +79,13 @@ def _declare(L):
     L.or2_diag.argtypes = [P, P, P, P, P]
     L.or_set_threads.argtypes = [I]
     L.or_get_threads.restype = I
+    L.or_rec_new.argtypes = [I, I, I, I, I, I, I, D, P, I, I, I, I, I, I, D, D, D, D, D, P]
+    L.or_rec_new.restype = P
+    L.or_rec_free.argtypes = [P]
+    L.or_rec_update.argtypes = [P, C.c_long, D, P, P, P, P]
+    L.or_rec_update.restype = I
+    L.or_rec_last.argtypes = [P, P, P, I]
+    L.or_rec_last.restype = I
     if hasattr(L, "orb_create"):
         L.orb_create.argtypes = [I, I, I, D, D, D, D, D, D, D, D]
         L.orb_create.restype = P
@@ -301,3 +308,49 @@ class BOUS:
         out = np.zeros_like(s)
         lib().orb_rhs(self.h, *[ptr(s[i]) for i in range(4)], *[ptr(out[i]) for i in range(4)])
         return out
+
+
+class Rec:
+    """Oracle recurrence search (oracle/or_rec.c): rule 0 3D Navier-Stokes,
+    1 Boussinesq, 2 2D.  state arrays are (nf, nkt) complex."""
+
+    def __init__(self, rule, g, L, state0, store_size=500, norm_width=8, nshift=(30, 2, 2), nout=10, dt=0.01,
+                 tstart=0.0, threshold=1.0, ri=0.0, v2=0.01):
+        s0 = np.ascontiguousarray(state0, dtype=np.complex128)
+        self.nf = s0.shape[0] if s0.ndim == 2 else 1
+        self.nkt = g.nkt
+        self.ns = int(np.prod(nshift))
+        self.store_size = store_size
+        Lb = np.ascontiguousarray(np.asarray(L).astype(np.int8))
+        two = g.dim == 2
+        self.h = lib().or_rec_new(rule, self.nf, g.iktx, g.ikty, 1 if two else g.iktz, g.kty, 0 if two else g.ktz,
+                                  g.alpha, ptr(Lb), store_size, norm_width, nshift[0], nshift[1], nshift[2], nout,
+                                  dt, tstart, threshold, ri, v2, ptr(s0))
+        assert self.h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_rec_free(self.h)
+            self.h = None
+
+    def update(self, nt, dt, state, want_state=False):
+        """Returns (guess or None, guess_state or None, check): guess is a dict of the
+        record fields, check of the best match of this check."""
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        g = np.zeros(10)
+        c = np.zeros(9)
+        gs = np.zeros((self.nf, self.nkt), np.complex128) if want_state else None
+        r = lib().or_rec_update(self.h, nt, dt, ptr(s), ptr(g), ptr(gs) if want_state else None, ptr(c))
+        chk = dict(checked=int(c[0]), compared=int(c[1]), store=int(c[2]), count=int(c[3]), residual=c[4],
+                   sx=c[5], sy=c[6], sz=c[7], time=c[8])
+        if r != 1:
+            return None, None, chk
+        guess = dict(index=int(g[0]), store=int(g[1]), time=g[2], v2=g[3], period=g[4], sx=g[5], sy=g[6], sz=g[7],
+                     residual=g[8], time_under=g[9])
+        return guess, gs, chk
+
+    def last(self):
+        res = np.zeros(self.store_size * self.ns)
+        slots = np.zeros(self.store_size, np.int32)
+        nj = lib().or_rec_last(self.h, ptr(res), ptr(slots), self.store_size)
+        return res[:nj * self.ns].reshape(nj, self.ns), slots[:nj]
