@@ -11,10 +11,16 @@
  *    dt = max(C/|Umax|, dt0) with C = 2 pi/NX and Umax = max |u_x|
  *    (init.h:31-34, out_low.h:10-15);
  *  - full-field dumps JOB<rank>/{U,V,W,rho}NNN.raw (NR doubles, x fastest) at
- *    Vout crossings when statsFLAG != 0 and nStop > 1 (out_low.h:79-110).
+ *    Vout crossings when statsFLAG != 0 and nStop > 1 (out_low.h:79-110);
+ *  - with RecFLAG = 1, the recurrence search after every step that started at
+ *    a Tout crossing (out_high.h:8-33 with the tcheck of out_low.h:5-6), on the
+ *    GPU (psg_rec, Boussinesq rule), guesses to JOB<rank>/UPO_ZK.out and
+ *    UPOinfo_ts.dat.  The reference's check also overwrites the loop's time
+ *    (functions.h:501 sets the global tme back by NOUT steps, so the same Tout
+ *    crossing recurs and the loop does not end); here the loop keeps its time.
  * Not built (reported once on stderr when requested): the forcing throttle
  * (Dtarget != 0), the RiG, buoyancy-flux, statistics, means, pdfs and
- * spectra outputs of out_low.h/out_high.h/finalise.h, and recurrence checks. */
+ * spectra outputs of out_low.h/out_high.h/finalise.h. */
 #include "legacy_common.h"
 #include <sys/stat.h>
 
@@ -32,17 +38,17 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
                             double *Theta, double *alpha, double *Dtarget, int *ikF, int *ikN, int *LC, int *IKTX,
                             int *IKTY, int *IKTZ, int *KTZ, int *NKT, int *NX, int *NY, int *NZ, int *NOUT,
                             int *NOUTV, int *statsFLAG, int *RecFLAG, int *adaptFLAG, int *RANK) {
-  (void)kx; (void)ky; (void)kz; (void)ResThresh; (void)ikF; (void)ikN; (void)IKTX; (void)IKTY; (void)IKTZ;
+  (void)kx; (void)ky; (void)kz; (void)ikF; (void)ikN; (void)IKTX; (void)IKTY; (void)IKTZ;
   (void)KTZ;
   const double nStop = *Tstep / *DELT;
   double dt = (nStop * *DELT != *Tstep) ? *Tstep / nStop : *DELT;
   const double dt0 = dt, courant = 2.0 * M_PI / *NX;
   const double tout = *NOUT * dt, vout = *NOUTV * dt;
-  if (*Dtarget != 0.0 || *RecFLAG == 1 || *statsFLAG != 0) {
+  if (*Dtarget != 0.0 || *statsFLAG != 0) {
     static int warned = 0;
     if (!warned)
-      fprintf(stderr, "psgpu legacy Boussinesq: forcing throttle, recurrence checks and statistics files are not "
-                      "produced; stepping, adaptive dt and field dumps are\n");
+      fprintf(stderr, "psgpu legacy Boussinesq: the forcing throttle and the statistics files are not "
+                      "produced; stepping, adaptive dt, field dumps and recurrence checks are\n");
     warned = 1;
   }
   psg_config c;
@@ -59,9 +65,23 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   psg_double2 *f[4] = {xii, eta, zet, rho};
   lg_pack(buf, nkt, 4, f);
   if (psg_set_tables(ctx, NULL, NULL, LC) || psg_set_state(ctx, 0, buf)) { lg_fail("upload"); free(buf); return; }
+  lg_upo upo;
+  memset(&upo, 0, sizeof upo);
+  if (*RecFLAG == 1) {
+    char dir[32];
+    snprintf(dir, sizeof dir, "JOB%1d", *RANK);
+    mkdir(dir, 0755);
+    if (lg_upo_open(&upo, ctx, *RANK, 4, nkt, *NOUT > 0 ? *NOUT : 1, *TSTART, *ResThresh)) {
+      lg_fail("recurrence search");
+      lg_upo_close(&upo);
+    }
+  }
   double tme = *TSTART;
+  long timestep = 0;
   int count = 0;
   while (tme <= *Tstep) {
+    /* the recurrence block follows a step whose start crossed a Tout multiple */
+    const int hi = upo.rec && lg_crossing(tme, dt, tout);
     const int at_tout = *statsFLAG != 0 && *adaptFLAG == 1 && lg_crossing(tme, dt, tout);
     int have_phys = 0;
     if (at_tout) {
@@ -98,18 +118,23 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
         if (r) { fwrite(phys + off[i], sizeof(double), (size_t)nr, r); fclose(r); }
       }
     }
-    /* steps up to the next step that has work before it */
+    /* steps up to the next step that has work before or after it */
     int n = 1;
     double t = tme + dt;
-    while (t <= *Tstep && n < 1 << 20 &&
+    while (!hi && t <= *Tstep && n < 1 << 20 &&
            !(*statsFLAG != 0 && ((*adaptFLAG == 1 && lg_crossing(t, dt, tout)) ||
-                                 (nStop > 1 && lg_crossing(t, dt, vout))))) {
+                                 (nStop > 1 && lg_crossing(t, dt, vout)))) &&
+           !(upo.rec && lg_crossing(t, dt, tout))) {
       t += dt;
       n++;
     }
     if (psg_run(ctx, n)) { lg_fail("run"); break; }
     for (int i = 0; i < n; i++) { *Time += dt; tme += dt; }
+    timestep += n;
+    double tcheck_time;
+    if (hi && lg_upo_update(&upo, timestep, dt, &tcheck_time)) { lg_fail("recurrence search"); break; }
   }
+  lg_upo_close(&upo);
   if (psg_get_state(ctx, 0, buf) == 0) lg_unpack(buf, nkt, 4, f);
   else lg_fail("download");
   free(phys); free(buf);

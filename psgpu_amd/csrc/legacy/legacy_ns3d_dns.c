@@ -4,8 +4,12 @@
  * (from step 0, when NSTOP > 1) the physical vorticity is copied to xir, etr,
  * ztr and written to JOB<RANK>/{xii,eta,zet}NNN.raw; with statsFLAG != 0,
  * JOB<RANK>/stats.dat and engy.dat get one line every NOUT steps
- * (computeStats, 3D_NavierStokes/src/functions.h:128-209).  The recurrence
- * search (RecFLAG) and the time-averaged means/pdfs files are not produced. */
+ * (computeStats, 3D_NavierStokes/src/functions.h:128-209).  With RecFLAG = 1
+ * every NOUT block runs the recurrence search on the GPU (psg_rec: check of the
+ * newest history entry, then the push of the current state, cuda.cu:114-139)
+ * and writes its guesses to JOB<RANK>/UPO_ZK.out and UPOinfo_ts.dat; as in the
+ * reference, a check resets tme (the time column of stats.dat) to
+ * TSTART + (NT - NOUT) DELT.  The time-averaged means/pdfs files are not produced. */
 #include "legacy_common.h"
 
 static void stats_line(const double *st, long nkt, const double *kx, const double *ky, const double *kz,
@@ -54,7 +58,7 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
                             double *KFY, double *V2, double *alpha, int *ikF, int *ikN, int *LC, int *IKTX,
                             int *IKTY, int *IKTZ, int *KTZ, int *NKT, int *NX, int *NY, int *NZ, int *NSTOP,
                             int *NOUT, int *NOUTV, int *statsFLAG, int *RecFLAG, int *RANK) {
-  (void)ResThresh; (void)ikF; (void)ikN; (void)IKTX; (void)IKTY; (void)IKTZ; (void)KTZ; (void)RecFLAG;
+  (void)ikF; (void)ikN; (void)IKTX; (void)IKTY; (void)IKTZ; (void)KTZ;
   psg_config c;
   memset(&c, 0, sizeof c);
   c.flavour = PSG_NS3D; c.precision = lg_precision(); c.batch = 1;
@@ -88,9 +92,16 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
     }
   }
   const int nout = *NOUT > 0 ? *NOUT : 1, nvort = *NOUTV > 0 ? *NOUTV : 1, nstop = *NSTOP;
+  lg_upo upo;
+  memset(&upo, 0, sizeof upo);
+  if (*RecFLAG == 1 && lg_upo_open(&upo, ctx, *RANK, 3, nkt, nout, *TSTART, *ResThresh)) {
+    lg_fail("recurrence search");
+    lg_upo_close(&upo);
+  }
+  const int hf = fs || upo.rec;   /* high-frequency blocks every NOUT steps */
   double *phys = NULL;
   int count = 0;
-  double tme = 0.0;
+  double tme = *TSTART;
   int nt = 0;
   while (nt < nstop) {
     if (nt % nvort == 0 && nstop > 1) {
@@ -112,13 +123,16 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
     /* run up to the next step with an output event: a dump at the top of
      * iteration e, or statistics after iteration e-1 */
     int e = nt + 1;
-    while (e < nstop && !((e % nvort == 0 && nstop > 1) || (fs && (e - 1) % nout == 0 && e - 1 != 0))) e++;
+    while (e < nstop && !((e % nvort == 0 && nstop > 1) || (hf && (e - 1) % nout == 0 && e - 1 != 0))) e++;
     if (psg_run(ctx, e - nt)) { lg_fail("run"); break; }
     tme += (e - nt) * *DELT;
     nt = e;
-    if (fs && (nt - 1) % nout == 0 && nt - 1 != 0) {
-      if (psg_get_state(ctx, 0, buf)) { lg_fail("download"); break; }
-      stats_line(buf, nkt, kx, ky, kz, LC, tavg, &tcount, *V2, *KFY, *AMPFOR, tme, fs, fe);
+    if (hf && (nt - 1) % nout == 0 && nt - 1 != 0) {
+      if (upo.rec && lg_upo_update(&upo, nt - 1, *DELT, &tme)) { lg_fail("recurrence search"); break; }
+      if (fs) {
+        if (psg_get_state(ctx, 0, buf)) { lg_fail("download"); break; }
+        stats_line(buf, nkt, kx, ky, kz, LC, tavg, &tcount, *V2, *KFY, *AMPFOR, tme, fs, fe);
+      }
     }
   }
   if (psg_get_state(ctx, 0, buf) == 0) lg_unpack(buf, nkt, 3, f);
@@ -126,6 +140,7 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   *Time = *TSTART + nstop * *DELT;
   if (fs) fclose(fs);
   if (fe) fclose(fe);
+  lg_upo_close(&upo);
   free(tavg); free(phys); free(buf);
 }
 

@@ -137,11 +137,11 @@ def test_legacy_euler_matches_oracle(gpu, tmp_path, monkeypatch):
 
 
 def bous_args(g, n, st, T, T0, Tstep, amp, dt, kf, v2, ri, sc, theta, L, kx, ky, kz, ikf, ikn,
-              nout=1, noutv=1, stats=0, rec=0, adapt=0, rank=0):
+              nout=1, noutv=1, stats=0, rec=0, adapt=0, rank=0, thr=1.0):
     """The 37-argument Boussinesq list (3D_Boussinesq/src/cuda.cu:21-58), by address."""
     d = lambda v: C.c_double(v)
     i = lambda v: C.c_int(v)
-    keep = [d(Tstep), d(amp), d(dt), d(1.0), d(kf), d(v2), d(ri), d(sc), d(theta), d(1.0), d(0.0),
+    keep = [d(Tstep), d(amp), d(dt), d(thr), d(kf), d(v2), d(ri), d(sc), d(theta), d(1.0), d(0.0),
             i(g.iktx), i(g.ikty), i(g.iktz), i(g.ktz), i(g.nkt), i(n), i(n), i(n), i(nout), i(noutv), i(stats),
             i(rec), i(adapt), i(rank)]
     a = [arr(st[0]), arr(st[1]), arr(st[2]), arr(st[3]), arr(kx), arr(ky), arr(kz), r(T), r(T0)]
