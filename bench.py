@@ -279,6 +279,50 @@ def bous_ensemble(P, n=512, members=4, steps=6, warmup=2):
     return out
 
 
+def recurrence_case(P, n=256, stores=500, cpu_sample=12, reps=5):
+    """Recurrence search at 256^3 with the reference's sizes (500 history entries,
+    the 2048-mode subset, 30x2x2 shifts): GPU time of a NOUT block (a check against
+    the full ring, 499 stores, and the push) averaged over `reps` blocks, and as its
+    CPU baseline the oracle's sequential check on one host core over a bounded
+    sample of stores, scaled to 499."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    st = P.Stepper(flavour="ns3d", n=n, dt=0.002, v2=0.01)
+    g = st.grid
+    rng = np.random.default_rng(5)
+    z = (rng.standard_normal((3, st.nkt)) + 1j * rng.standard_normal((3, st.nkt))) * st.mask[None]
+    st.set_state(z)
+    rec = P.Recurrence(st, nout=10, store_size=stores, keep_states=0, threshold=0.0)
+    ph = np.exp(1j * rng.standard_normal(st.nkt) * 0.01)
+    for e in range(1, stores + 1):
+        z = z * ph[None]
+        st.set_state(z)
+        rec.update(10 * e, 0.002)
+    t0 = time.perf_counter()
+    for e in range(stores + 1, stores + 1 + reps):
+        _, _, c = rec.update(10 * e, 0.002)
+    gpu_ms = 1e3 * (time.perf_counter() - t0) / reps
+    rec.close()
+    st.close()
+    # oracle: one check over cpu_sample + 1 entries (a ring of that size), one thread
+    og = O.grid3(n, n, n)
+    kx, ky, kz, L = O.tables(og)
+    O.lib().or_set_threads(1)
+    small = (rng.standard_normal((3, og.nkt)) + 1j * rng.standard_normal((3, og.nkt))) * L
+    orr = O.Rec(0, og, L, small, store_size=cpu_sample + 1, nout=10, dt=0.002, threshold=0.0)
+    for e in range(1, cpu_sample + 1):
+        orr.update(10 * e, 0.002, small)
+    t0 = time.perf_counter()
+    orr.update(10 * (cpu_sample + 1), 0.002, small)
+    cpu_s = time.perf_counter() - t0
+    del orr
+    O.lib().or_set_threads(os.cpu_count() or 1)
+    per_store = cpu_s / (cpu_sample - 1)
+    return {"grid": [n, n, n], "stores": c["compared"], "shifts": 120, "subset_modes": 2048,
+            "gpu_ms_per_check": gpu_ms, "cpu_oracle_ms_per_check_1core": 1e3 * per_store * (stores - 1),
+            "cpu_sample_stores": cpu_sample - 1}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -326,6 +370,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel shapes (no autotune)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^3 adjoint-gradient case")
     ap.add_argument("--no-bous", action="store_true", help="skip the 512^3 Boussinesq ensemble case")
+    ap.add_argument("--no-rec", action="store_true", help="skip the 256^3 recurrence-search timing")
     ap.add_argument("--no-energy", action="store_true",
                     help="skip the 1000-step FP32-bulk energy series (profiled runs)")
     ap.add_argument("--newton", type=int, default=1, metavar="NGUESS",
@@ -385,6 +430,12 @@ def main():
     if not a.no_bous:
         log("bench: boussinesq 512^3 ensemble")
         bous = bous_ensemble(P)
+    recur = None
+    if not a.no_rec and n == 256:
+        log("bench: recurrence search 256^3, 500-entry history")
+        recur = recurrence_case(P, n)
+        log("bench: recurrence check %.2f ms on the GPU (%d stores); oracle %.0f ms on one core" %
+            (recur["gpu_ms_per_check"], recur["stores"], recur["cpu_oracle_ms_per_check_1core"]))
     newton = None
     if a.newton:
         log("bench: newton 2D %d^2, %d guesses" % (a.newton_n, a.newton))
@@ -400,7 +451,8 @@ def main():
                       "forcing": "sin(4y) x", "ic": "solenoidal E(k)~k^4exp(-2(k/4)^2), E=0.5, seed 20240601",
                       "members": 1, "pass_variants": var64},
            "roofline": roof, "model": model, "fp32_bulk": fp32, "kernels_fp64": prof64,
-           "adjoint_128": adj, "bous_ensemble_512": bous, "newton_2d": newton, "cpu_baseline": cpu}
+           "adjoint_128": adj, "bous_ensemble_512": bous, "recurrence_256": recur, "newton_2d": newton,
+           "cpu_baseline": cpu}
     print(json.dumps(out))
 
 

@@ -1,20 +1,24 @@
-"""Near-recurrence search for Newton guesses in 2D Kolmogorov flow.
+"""Near-recurrence helpers for Newton guesses.
 
-The reference looks back over a stored history of DNS states for
-min over x-shifts s of ||S_s z(t+T) - z(t)|| / ||z(t)|| below a threshold, and
-writes such (z, T, s) triples as Newton guesses
-(2D_NavierStokes/src_RK4/functions.h:105-309, recurrence_check / computeZdash;
-3D_NavierStokes/src/functions.h:213-389).  This module does the same over
-snapshots taken from the GPU stepper: the shift that best aligns two states is
-found for all shifts at once, because the shifted inner product is a Fourier
-series in s over the kx index,
+The reference's own search (the shift sweep over a 500-entry DNS history,
+its turning-point and output-sequence rules, UPO_ZK.out records;
+3D_NavierStokes/src/functions.h:213-389, 3D_Boussinesq/src/functions.h:415-619,
+2D_NavierStokes/src_RK4/functions.h:105-309) runs on the GPU as
+psgpu_amd.Recurrence (libpsg's psg_rec_*).
+
+This module adds a continuous-shift variant for 2D Kolmogorov flow, used by
+the bench to pick its Newton guess from snapshots of the GPU stepper: the
+shift that best aligns two states is found for all shifts at once, because
+the shifted inner product is a Fourier series in s over the kx index,
 
     sum_L Re(conj(a) e^{-i kx s} b) = Re sum_kx e^{-i kx s} c(kx),
     c(kx) = sum_ky conj(a(kx,ky)) b(kx,ky) L(kx,ky),
 
-evaluated on M equally spaced shifts by one FFT of length M per pair.
-The shift convention S_s z = exp(-i kx s) z is the Newton solver's
-(include/psg_newton.h), so a hit (t, T, s) is a guess for S_s Phi_T(z) = z.
+evaluated on M equally spaced shifts by one FFT of length M per pair (the
+reference tries NSHIFTX = 30 shifts per pair).  The shift convention
+S_s z = exp(-i kx s) z is the Newton solver's (include/psg_newton.h), so a hit
+(t, T, s) is a guess for S_s Phi_T(z) = z.  It also reads and writes
+UPO_ZK.out stream records.
 """
 import numpy as np
 
