@@ -153,6 +153,18 @@ int or_rec_update(or_rec *r, long NT, double delt, const double *state, double *
                   double *chk);
 int or_rec_last(const or_rec *r, double *res, int *slots, int cap);
 
+/* ---------------- run statistics (or_stats.c) ---------------- */
+void or_stats_ns3d(long nkt, const double *kx, const double *ky, const double *kz, const int *LL, const orc *X,
+                   const orc *E, const orc *Z, orc *Xt, orc *Et, orc *Zt, double cnt, double ampfor, double kfy,
+                   double *out);
+void or_stats_bous(long nkt, const double *kx, const double *ky, const double *kz, const int *LL, const orc *X,
+                   const orc *E, const orc *Z, const orc *R, orc *Xt, orc *Et, orc *Zt, orc *Rt, double cnt,
+                   double ampfor, double kfy, double ri, double theta, double *out);
+void or_rig(int nx, int ny, int nz, double theta, const double *U, const double *V, const double *W,
+            const double *RO, double *navg, double *savg);
+int or_spectra(long nkt, int nx, const double *kx, const double *ky, const double *kz, const orc *Xt, const orc *Et,
+               const orc *Zt, double theta, double *spch, double *spcv);
+
 /* ---------------- threading ---------------- */
 void or_set_threads(int n);
 int  or_get_threads(void);

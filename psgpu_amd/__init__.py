@@ -176,6 +176,14 @@ def lib():
     L.psg_newton_solve.argtypes = [_P, _P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(NewtonOpts),
                                    C.POINTER(NewtonResult)]
     L.psg_flow_map.argtypes = [_P, _P, _D, _D, _I]
+    L.psg_stats_create.argtypes = [_P, _I, C.POINTER(_P)]
+    L.psg_stats_update.argtypes = [_P, _L, _P]
+    L.psg_stats_physical.argtypes = [_P, _I, _L, C.POINTER(_D), _P, _P, C.POINTER(_I)]
+    L.psg_stats_spectra.argtypes = [_P, _P, _P, _I, C.POINTER(_I)]
+    L.psg_stats_mean_fields.argtypes = [_P, _P]
+    L.psg_stats_tavg.argtypes = [_P, _P]
+    L.psg_stats_destroy.argtypes = [_P]
+    L.psg_set_ampfor.argtypes = [_P, _D]
     L.psg_rec_defaults.argtypes = [_P, C.POINTER(RecOpts)]
     L.psg_rec_create.argtypes = [_P, C.POINTER(RecOpts), C.POINTER(_P)]
     L.psg_rec_update.argtypes = [_P, _L, _D, C.POINTER(RecGuess), _P, C.POINTER(RecCheck)]
@@ -545,3 +553,58 @@ class Recurrence:
         s = np.zeros((self.st.nf, self.st.nkt), np.complex128)
         _check(lib().psg_rec_state(self.h, int(store), _ptr(s)), "psg_rec_state")
         return s
+
+
+class Stats:
+    """Run statistics of one context member on the GPU (include/psg_stats.h):
+    computeStats sums with device time averages, and for the Boussinesq flavour
+    the physical-space variances, max|u|, RiG plane sums, spectra and mean fields."""
+
+    def __init__(self, stepper, member=0):
+        self.st = stepper
+        h = _P()
+        _check(lib().psg_stats_create(stepper.h, member, C.byref(h)), "psg_stats_create")
+        self.h = h
+        self.nv = 12 if stepper.cfg.flavour == BOUS3D else 11
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psg_stats_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def update(self, count):
+        v = np.zeros(self.nv)
+        _check(lib().psg_stats_update(self.h, int(count), _ptr(v)), "psg_stats_update")
+        return v
+
+    def physical(self, what, count):
+        c = self.st.cfg
+        npl_max = max(c.ny, c.nz)
+        na, sa = np.zeros(npl_max), np.zeros(npl_max)
+        um, npl = C.c_double(0.0), C.c_int(0)
+        _check(lib().psg_stats_physical(self.h, int(what), int(count), C.byref(um), _ptr(na), _ptr(sa),
+                                        C.byref(npl)), "psg_stats_physical")
+        return um.value, na[:npl.value], sa[:npl.value]
+
+    def spectra(self):
+        cap = self.st.cfg.nx + 1
+        h, v = np.zeros(cap), np.zeros(cap)
+        size = C.c_int(0)
+        _check(lib().psg_stats_spectra(self.h, _ptr(h), _ptr(v), cap, C.byref(size)), "psg_stats_spectra")
+        return h, v, size.value
+
+    def mean_fields(self):
+        c = self.st.cfg
+        out = np.zeros((7, c.nz, c.ny, c.nx))
+        _check(lib().psg_stats_mean_fields(self.h, _ptr(out)), "psg_stats_mean_fields")
+        return out
+
+    def tavg(self):
+        t = np.zeros((self.st.nf, self.st.nkt), np.complex128)
+        _check(lib().psg_stats_tavg(self.h, _ptr(t)), "psg_stats_tavg")
+        return t
+
+
+STP_RMS, STP_UMAX, STP_RIG = 1, 2, 4

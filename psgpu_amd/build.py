@@ -32,7 +32,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
 CORE_HIP = ["psg_api.hip", "psg_ns3d.hip", "psg_ns2d.hip", "psg_bous.hip", "psg_newton.hip", "psg_tladj.hip",
-            "psg_rec.hip"]
+            "psg_rec.hip", "psg_stats.hip"]
 CORE_C = ["psg_host.c"]
 # legacy shims: (library suffix, source)
 LEGACY = [("ns3d_dns", "legacy/legacy_ns3d_dns.c"), ("ns3d_gmres", "legacy/legacy_ns3d_gmres.c"),

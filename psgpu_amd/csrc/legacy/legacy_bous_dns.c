@@ -5,24 +5,37 @@
  *  - time step: nStop = Tstep/DELT, dt = Tstep/nStop unless nStop*DELT == Tstep
  *    (init.h:24-29); steps run while tme <= Tstep with tme from TSTART
  *    (cuda.cu:68-99), *Time advanced by dt per step;
- *  - output times: the step that first reaches a multiple of Tout = NOUT*dt
- *    (tcheck < 0, out_low.h:5-8) and of Vout = NOUTV*dt (out_low.h:79-80);
- *  - adaptive dt (adaptFLAG = 1, at Tout crossings with statsFLAG != 0):
- *    dt = max(C/|Umax|, dt0) with C = 2 pi/NX and Umax = max |u_x|
- *    (init.h:31-34, out_low.h:10-15);
+ *  - Tout = NOUT dt and Vout = NOUTV dt crossings tested at the top of a step
+ *    (tcheck < 0, out_low.h:5-8, 79-80);
+ *  - with statsFLAG != 0, at each Tout crossing (out_low.h:8-76): the running
+ *    mean of u^2, v^2, w^2 per grid point (avgVelocity), the adaptive step
+ *    (adaptFLAG = 1: dt = max(C/max|u|, dt0), C = 2 pi/NX), and the gradient
+ *    Richardson plane averages to JOB<rank>/RiG.dat;
  *  - full-field dumps JOB<rank>/{U,V,W,rho}NNN.raw (NR doubles, x fastest) at
  *    Vout crossings when statsFLAG != 0 and nStop > 1 (out_low.h:79-110);
- *  - with RecFLAG = 1, the recurrence search after every step that started at
- *    a Tout crossing (out_high.h:8-33 with the tcheck of out_low.h:5-6), on the
- *    GPU (psg_rec, Boussinesq rule), guesses to JOB<rank>/UPO_ZK.out and
- *    UPOinfo_ts.dat.  The reference's check also overwrites the loop's time
- *    (functions.h:501 sets the global tme back by NOUT steps, so the same Tout
- *    crossing recurs and the loop does not end); here the loop keeps its time.
- * Not built (reported once on stderr when requested): the forcing throttle
- * (Dtarget != 0), the RiG, buoyancy-flux, statistics, means, pdfs and
- * spectra outputs of out_low.h/out_high.h/finalise.h. */
+ *  - after a step that started at a Tout crossing (out_high.h:8-33): the
+ *    recurrence search (RecFLAG = 1, psg_rec, guesses to JOB<rank>/UPO_ZK.out
+ *    and UPOinfo_ts.dat) and computeStats (statsFLAG = 1, or 2 once tme > 50):
+ *    JOB<rank>/stats.dat and engy.dat lines, the Dtarget forcing throttle
+ *    ampfor <- ampfor (Dtarget - bflux)/input (functions.h:371-375) applied
+ *    from the next step (cuda.cu:104-106);
+ *  - at the end (statsFLAG = 1, nStop > 1, finalise.h:17-230): a last
+ *    computeStats, spec.dat, avgs.dat (seven time-stamped fields), means.dat,
+ *    the averages on stdout, a sweep.dat line and pdfs.dat.
+ * The sums, time averages, variances, max|u|, plane averages and spectra are
+ * device reductions (psg_stats); the scalar bookkeeping and the file layouts
+ * are the reference's.
+ * Deviations: the reference's recurrence check also overwrites the loop's
+ * time (functions.h:501 moves the global tme back by NOUT steps, so the same
+ * Tout crossing recurs and the loop does not end); here the loop keeps its
+ * time.  The per-dump bflux files are not written: computeBflux
+ * (functions.h:209-273, marked work in progress there) divides by counts it
+ * never initialises, so their contents are not defined.  The pdf bins keep
+ * the reference's index int((x - min)/dx) + 1, whose top value lands one past
+ * the 100 printed bins. */
 #include "legacy_common.h"
 #include <sys/stat.h>
+#include "../../../include/psg_stats.h"
 
 static const double lg_small = 1e-10; /* 3D_Boussinesq/src/kernels.h:31 */
 
@@ -30,6 +43,208 @@ static int lg_crossing(double tme, double dt, double period) {
   if (!(period > 0.0)) return 0;
   const double crit = period * (double)(int)((tme + lg_small) / period);
   return (tme - dt - crit + lg_small) * (tme + lg_small - crit) < 0.0;
+}
+
+/* the reference's host-side statistics state (kernels.h globals, init.h:48-65) */
+typedef struct {
+  psg_stats *st;
+  FILE *stats, *engy, *avgs, *means, *pdfs, *spec, *rig, *sweep;
+  double dissMax, dissMin, inputMax, inputMin, enerMax, enerMin;
+  double enerAvg, dissAvg, peAvg, pedAvg, mixAvg, inputAvg, eturbAvg, dturbAvg, bfluxAvg, RiGMax;
+  double ampfor, dtarget, umax;
+  long tavgCount;
+  int countStats, nStats;
+  double *Diss, *Energy, *Input;
+} bstats;
+
+static int bs_open(bstats *b, psg_ctx *ctx, int rank, double Tstep, double Tout, double ampfor, double dtarget) {
+  memset(b, 0, sizeof *b);
+  b->dissMin = b->inputMin = b->enerMin = 10000.0;
+  b->ampfor = ampfor;
+  b->dtarget = dtarget;
+  b->nStats = (int)(100 * Tstep / Tout + 1);
+  if (b->nStats < 1) b->nStats = 1;
+  b->Diss = malloc(sizeof(double) * b->nStats);
+  b->Energy = malloc(sizeof(double) * b->nStats);
+  b->Input = malloc(sizeof(double) * b->nStats);
+  if (!b->Diss || !b->Energy || !b->Input) return -1;
+  if (psg_stats_create(ctx, 0, &b->st)) return -1;
+  char dir[32], p[64];
+  snprintf(dir, sizeof dir, "JOB%1d", rank);
+  mkdir(dir, 0755);
+  b->sweep = fopen("sweep.dat", "a");
+  const char *nm[7] = {"stats.dat", "engy.dat", "avgs.dat", "means.dat", "pdfs.dat", "spec.dat", "RiG.dat"};
+  FILE **fp[7] = {&b->stats, &b->engy, &b->avgs, &b->means, &b->pdfs, &b->spec, &b->rig};
+  for (int i = 0; i < 7; i++) {
+    snprintf(p, sizeof p, "JOB%1d/%s", rank, nm[i]);
+    *fp[i] = fopen(p, "w");
+    if (!*fp[i]) { fprintf(stderr, "psgpu legacy Boussinesq: cannot open %s\n", p); return -1; }
+  }
+  return b->sweep ? 0 : -1;
+}
+
+static void bs_close(bstats *b) {
+  if (b->st) psg_stats_destroy(b->st);
+  FILE *f[8] = {b->stats, b->engy, b->avgs, b->means, b->pdfs, b->spec, b->rig, b->sweep};
+  for (int i = 0; i < 8; i++)
+    if (f[i]) fclose(f[i]);
+  free(b->Diss); free(b->Energy); free(b->Input);
+  memset(b, 0, sizeof *b);
+}
+
+/* computeStats (3D_Boussinesq/src/functions.h:275-411) on the device sums */
+static int bs_compute(bstats *b, double tme, double v2, double kfy, double ri) {
+  double s[PSG_STB_N];
+  if (psg_stats_update(b->st, b->tavgCount, s)) return -1;
+  double energy = s[PSG_STB_ENERGY], diss = s[PSG_STB_DISS], input = s[PSG_STB_INPUT], bflux = s[PSG_STB_BFLUX];
+  const double pe = s[PSG_STB_PE], ped = s[PSG_STB_PED], ped1 = s[PSG_STB_PED1], ped2 = s[PSG_STB_PED2];
+  double eturb = s[PSG_STB_ETURB], dturb = s[PSG_STB_DTURB], peturb = s[PSG_STB_PETURB], pedturb = s[PSG_STB_PEDTURB];
+  if (b->dtarget != 0.0) b->ampfor = b->ampfor * (b->dtarget - bflux) / input;
+  const double tc = (double)b->tavgCount;
+  b->dissAvg = (b->dissAvg * tc + diss) / (tc + 1.0);
+  b->enerAvg = (b->enerAvg * tc + energy) / (tc + 1.0);
+  b->peAvg = (b->peAvg * tc + pe) / (tc + 1.0);
+  b->pedAvg = (b->pedAvg * tc + ped) / (tc + 1.0);
+  b->inputAvg = (b->inputAvg * tc + input) / (tc + 1.0);
+  b->bfluxAvg = (b->bfluxAvg * tc + bflux) / (tc + 1.0);
+  b->mixAvg = ((ped / (diss / ri + ped)) * tc + b->mixAvg) / (tc + 1.0);
+  input *= v2 * kfy * kfy * 2.0;
+  diss *= v2 * v2 * 4.0 * kfy * kfy;
+  dturb = b->dissAvg - dturb;
+  dturb *= v2 * v2 * 4.0 * kfy * kfy;
+  eturb = b->enerAvg - eturb;
+  peturb = b->peAvg - peturb;
+  pedturb = b->pedAvg - pedturb;
+  b->dturbAvg = (b->dturbAvg * tc + dturb) / (tc + 1.0);
+  b->eturbAvg = (b->eturbAvg * tc + eturb) / (tc + 1.0);
+  b->dissMax = fmax(diss, b->dissMax);
+  b->dissMin = fmin(diss, b->dissMin);
+  b->enerMax = fmax(energy, b->enerMax);
+  b->enerMin = fmin(energy, b->enerMin);
+  b->inputMax = fmax(input, b->inputMax);
+  b->inputMin = fmin(input, b->inputMin);
+  if (b->countStats > b->nStats) {
+    /* out_high.h:38-62 doubles the history before the next store; the final
+     * computeStats of finalise.h can also reach one past it */
+    while (b->countStats > b->nStats) b->nStats *= 2;
+    double *d = realloc(b->Diss, sizeof(double) * b->nStats);
+    if (d) b->Diss = d;
+    double *e = realloc(b->Energy, sizeof(double) * b->nStats);
+    if (e) b->Energy = e;
+    double *i = realloc(b->Input, sizeof(double) * b->nStats);
+    if (i) b->Input = i;
+    if (!d || !e || !i) return -1;
+  }
+  b->Diss[b->countStats - 1] = diss;
+  b->Energy[b->countStats - 1] = energy;
+  b->Input[b->countStats - 1] = input;
+  b->tavgCount++;
+  fprintf(b->stats, "%e %e %e %e %e %e %e \n", tme, energy, eturb, diss, dturb, input, b->umax);
+  fflush(b->stats);
+  fprintf(b->engy, "%e %e %e %e %e %e %e %e\n", tme, bflux, pe, peturb, ped, pedturb, ped1, ped2);
+  fflush(b->engy);
+  return 0;
+}
+
+/* out_low.h:8-76 at a Tout crossing: the velocity variances, max|u| (with
+ * adaptFLAG) and the RiG plane averages, from one physical-space pass */
+static int bs_tout(bstats *b, int adapt, double tme, int nx, int ny, int nz) {
+  int np = 0;
+  const int cap = ny > nz ? ny : nz;
+  double *na = malloc(sizeof(double) * cap), *sa = malloc(sizeof(double) * cap);
+  if (!na || !sa) { free(na); free(sa); return -1; }
+  const int what = PSG_STP_RMS | PSG_STP_RIG | (adapt ? PSG_STP_UMAX : 0);
+  const int r = psg_stats_physical(b->st, what, b->tavgCount, &b->umax, na, sa, &np);
+  if (r == 0) {
+    b->RiGMax = 0.0;
+    for (int i = 0; i < np; i++) {
+      fprintf(b->rig, "%e %d %e %e\n", tme, i, na[i] / (double)(nx * nz), sa[i] / (double)(nx * nz));
+      b->RiGMax = fmax(b->RiGMax, na[i] / (double)nx / sa[i] / (double)nx);
+    }
+    fprintf(b->rig, "   \n");
+    fflush(b->rig);
+  }
+  free(na); free(sa);
+  return r;
+}
+
+/* finalise.h:17-230 */
+static int bs_finalise(bstats *b, double tme, double Time, double v2, double kfy, double ri, double alpha, int nx,
+                       int ny, int nz) {
+  b->countStats++;
+  if (bs_compute(b, tme, v2, kfy, ri)) return -1;
+  double *spch = calloc(nx + 1, sizeof(double)), *spcv = calloc(nx + 1, sizeof(double));
+  int size = 0;
+  if (!spch || !spcv || psg_stats_spectra(b->st, spch, spcv, nx + 1, &size)) { free(spch); free(spcv); return -1; }
+  for (int j = 0; j < size; j++) fprintf(b->spec, "%d %e %e\n", j, spch[j], spcv[j]);
+  fflush(b->spec);
+  free(spch); free(spcv);
+  const long nr = (long)nx * ny * nz;
+  double *f = malloc(sizeof(double) * 7 * nr);
+  if (!f || psg_stats_mean_fields(b->st, f)) { free(f); return -1; }
+  for (int k = 0; k < 7; k++) {
+    fwrite(&Time, sizeof(double), 1, b->avgs);
+    fwrite(f + k * nr, sizeof(double), (size_t)nr, b->avgs);
+  }
+  fflush(b->avgs);
+  const double twopi = 4.0 * asin(1.0);
+  for (int iz = 0; iz < nz; iz++) {
+    for (int iy = 0; iy < ny; iy++) {
+      double m[4] = {0, 0, 0, 0};   /* x-means of u, v, w and of <u^2> - <u>^2 */
+      const int src[4] = {1, 2, 3, 6};
+      for (int q = 0; q < 4; q++) {
+        const double *a = f + src[q] * nr + ((long)iz * ny + iy) * nx;
+        for (int ix = 0; ix < nx; ix++) m[q] += a[ix];
+        m[q] /= (double)nx;
+      }
+      /* the reference passes eight values to six conversions: the first six print */
+      fprintf(b->means, "%.15e %.15e %.15e %.15e %.15e %.15e \n", iy * twopi / (double)ny, iz * twopi / (double)nz,
+              m[0], m[1], m[2], m[3]);
+    }
+    fprintf(b->means, " \n");
+  }
+  fflush(b->means);
+  free(f);
+  printf("AVERAGE STATS *****\n");
+  printf("energy = %e,  eturb = %e \n", b->enerAvg, b->eturbAvg);
+  printf("dissipation = %e,  dturb = %e \n", b->dissAvg, b->dturbAvg);
+  printf("input E = %e \n", b->inputAvg);
+  printf("bouyancy flux = %e \n", b->bfluxAvg);
+  printf("max/min dissipation %e, %e \n ", b->dissMax, b->dissMin);
+  printf("max/min energy %e, %e \n", b->enerMax, b->enerMin);
+  printf("max/min input %e, %e \n", b->inputMax, b->inputMin);
+  fprintf(b->sweep, "%.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e \n", 1.0 / v2, ri, alpha, b->dissAvg,
+          b->inputAvg, b->mixAvg, b->RiGMax, b->pedAvg);
+  fflush(b->sweep);
+  if (b->dissMax - b->dissMin > 0.001 && b->inputMax - b->inputMin > 0.001 && b->enerMax - b->enerMin > 0.001) {
+    enum { nBin = 100 };
+    const double correct = 1.0000001;
+    int I_E[nBin + 2], I_D[nBin + 2], I_I[nBin + 2];
+    const double dD = (b->dissMax * correct - b->dissMin) / (double)nBin;
+    const double dI = (b->inputMax * correct - b->inputMin) / (double)nBin;
+    const double dE = (b->enerMax * correct - b->enerMin) / (double)nBin;
+    memset(I_E, 0, sizeof I_E); memset(I_D, 0, sizeof I_D); memset(I_I, 0, sizeof I_I);
+    printf("dD = %e \n", dD);
+    printf("dI = %e \n", dI);
+    printf("dE = %e \n", dE);
+    for (int NT = 0; NT < b->countStats; NT++) {
+      int j = (int)((b->Energy[NT] - b->enerMin) / dE) + 1;
+      if (j >= 0 && j <= nBin + 1) I_E[j]++;
+      j = (int)((b->Diss[NT] - b->dissMin) / dD) + 1;
+      if (j > nBin) printf("error in j!");
+      if (j >= 0 && j <= nBin + 1) I_D[j]++;
+      j = (int)((b->Input[NT] - b->inputMin) / dI) + 1;
+      if (j > nBin) printf("error in j!");
+      if (j >= 0 && j <= nBin + 1) I_I[j]++;
+    }
+    fflush(stdout);
+    for (int ib = 0; ib < nBin; ib++)
+      fprintf(b->pdfs, "%.15e %.15e %.15e %.15e %.15e %.15e \n", b->enerMin + (ib - 0.5) * dE,
+              I_E[ib] / ((b->nStats - 1) * dE), b->dissMin + (ib - 0.5) * dD, I_D[ib] / ((b->nStats - 1) * dD),
+              b->inputMin + (ib - 0.5) * dI, I_I[ib] / ((b->nStats - 1) * dI));
+    fflush(b->pdfs);
+  }
+  return 0;
 }
 
 void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet, psg_double2 *rho, double *kx,
@@ -44,13 +259,6 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   double dt = (nStop * *DELT != *Tstep) ? *Tstep / nStop : *DELT;
   const double dt0 = dt, courant = 2.0 * M_PI / *NX;
   const double tout = *NOUT * dt, vout = *NOUTV * dt;
-  if (*Dtarget != 0.0 || *statsFLAG != 0) {
-    static int warned = 0;
-    if (!warned)
-      fprintf(stderr, "psgpu legacy Boussinesq: the forcing throttle and the statistics files are not "
-                      "produced; stepping, adaptive dt, field dumps and recurrence checks are\n");
-    warned = 1;
-  }
   psg_config c;
   memset(&c, 0, sizeof c);
   c.flavour = PSG_BOUS3D; c.precision = lg_precision(); c.batch = 1;
@@ -65,6 +273,15 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   psg_double2 *f[4] = {xii, eta, zet, rho};
   lg_pack(buf, nkt, 4, f);
   if (psg_set_tables(ctx, NULL, NULL, LC) || psg_set_state(ctx, 0, buf)) { lg_fail("upload"); free(buf); return; }
+  bstats bs;
+  memset(&bs, 0, sizeof bs);
+  const int stats = *statsFLAG != 0;
+  if (stats && bs_open(&bs, ctx, *RANK, *Tstep, tout, *AMPFOR, *Dtarget)) {
+    lg_fail("statistics set-up");
+    bs_close(&bs);
+    free(buf);
+    return;
+  }
   lg_upo upo;
   memset(&upo, 0, sizeof upo);
   if (*RecFLAG == 1) {
@@ -76,35 +293,30 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
       lg_upo_close(&upo);
     }
   }
+  const int hf = upo.rec || stats;   /* out_high.h blocks */
   double tme = *TSTART;
   long timestep = 0;
   int count = 0;
   while (tme <= *Tstep) {
-    /* the recurrence block follows a step whose start crossed a Tout multiple */
-    const int hi = upo.rec && lg_crossing(tme, dt, tout);
-    const int at_tout = *statsFLAG != 0 && *adaptFLAG == 1 && lg_crossing(tme, dt, tout);
-    int have_phys = 0;
-    if (at_tout) {
-      if (!phys) phys = malloc(sizeof(double) * 7 * nr);
-      if (!phys) { lg_fail("host allocation"); break; }
-      if (psg_physical(ctx, 0, phys)) { lg_fail("physical"); break; }
-      have_phys = 1;
-      double umax = 0.0;
-      for (long i = 0; i < nr; i++) umax = fmax(umax, fabs(phys[3 * nr + i]));
-      const double nd = fmax(courant / umax, dt0);
-      if (nd != dt) {
-        if (lg_set_dt(ctx, nd)) { lg_fail("set_dt"); break; }
-        dt = nd;
+    /* out_low.h: tcheck of this step's start */
+    const int tcheck = lg_crossing(tme, dt, tout);
+    if (tcheck && stats) {
+      if (bs_tout(&bs, *adaptFLAG == 1, tme, *NX, *NY, *NZ)) { lg_fail("statistics"); break; }
+      if (*adaptFLAG == 1) {
+        const double nd = fmax(courant / fabs(bs.umax), dt0);
+        if (nd != dt) {
+          if (lg_set_dt(ctx, nd)) { lg_fail("set_dt"); break; }
+          dt = nd;
+        }
       }
     }
     /* the dump test uses the updated dt, as out_low.h:79-80 follows :10-15 */
-    const int at_vout = *statsFLAG != 0 && nStop > 1 && lg_crossing(tme, dt, vout);
-    if (at_vout && !have_phys) {
+    const int at_vout = stats && nStop > 1 && lg_crossing(tme, dt, vout);
+    if (at_vout) {
       if (!phys) phys = malloc(sizeof(double) * 7 * nr);
       if (!phys) { lg_fail("host allocation"); break; }
       if (psg_physical(ctx, 0, phys)) { lg_fail("physical"); break; }
-    }
-    if (at_vout) {
+      printf("DT = %e \n", dt);
       count++;
       char dir[32];
       snprintf(dir, sizeof dir, "JOB%1d", *RANK);
@@ -121,22 +333,35 @@ void timestep_cuda_bous_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
     /* steps up to the next step that has work before or after it */
     int n = 1;
     double t = tme + dt;
-    while (!hi && t <= *Tstep && n < 1 << 20 &&
-           !(*statsFLAG != 0 && ((*adaptFLAG == 1 && lg_crossing(t, dt, tout)) ||
-                                 (nStop > 1 && lg_crossing(t, dt, vout)))) &&
-           !(upo.rec && lg_crossing(t, dt, tout))) {
+    while (!(tcheck && hf) && t <= *Tstep && n < 1 << 20 && !(hf && lg_crossing(t, dt, tout)) &&
+           !(stats && nStop > 1 && lg_crossing(t, dt, vout))) {
       t += dt;
       n++;
     }
     if (psg_run(ctx, n)) { lg_fail("run"); break; }
     for (int i = 0; i < n; i++) { *Time += dt; tme += dt; }
     timestep += n;
-    double tcheck_time;
-    if (hi && lg_upo_update(&upo, timestep, dt, &tcheck_time)) { lg_fail("recurrence search"); break; }
+    if (tcheck && hf) {
+      /* out_high.h */
+      bs.countStats++;
+      double tcheck_time;
+      if (upo.rec && lg_upo_update(&upo, timestep, dt, &tcheck_time)) { lg_fail("recurrence search"); break; }
+      if ((*statsFLAG == 1 || (*statsFLAG == 2 && tme > 50.0)) && bs_compute(&bs, tme, *V2, *KFY, *Ri)) {
+        lg_fail("statistics");
+        break;
+      }
+      if (stats && *Dtarget != 0.0 && psg_set_ampfor(ctx, bs.ampfor)) { lg_fail("forcing throttle"); break; }
+    }
   }
-  lg_upo_close(&upo);
+  if (stats && *statsFLAG == 1 && nStop > 1 &&
+      bs_finalise(&bs, tme, *Time, *V2, *KFY, *Ri, *alpha, *NX, *NY, *NZ))
+    lg_fail("final statistics");
   if (psg_get_state(ctx, 0, buf) == 0) lg_unpack(buf, nkt, 4, f);
   else lg_fail("download");
+  /* the cached context keeps the caller's forcing amplitude */
+  if (stats && *Dtarget != 0.0) psg_set_ampfor(ctx, *AMPFOR);
+  bs_close(&bs);
+  lg_upo_close(&upo);
   free(phys); free(buf);
 }
 

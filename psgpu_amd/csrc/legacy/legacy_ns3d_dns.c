@@ -4,52 +4,30 @@
  * (from step 0, when NSTOP > 1) the physical vorticity is copied to xir, etr,
  * ztr and written to JOB<RANK>/{xii,eta,zet}NNN.raw; with statsFLAG != 0,
  * JOB<RANK>/stats.dat and engy.dat get one line every NOUT steps
- * (computeStats, 3D_NavierStokes/src/functions.h:128-209).  With RecFLAG = 1
+ * (computeStats, 3D_NavierStokes/src/functions.h:128-209; statsFLAG = 2 from
+ * tme > 500), from device reductions and device time averages (psg_stats).  With RecFLAG = 1
  * every NOUT block runs the recurrence search on the GPU (psg_rec: check of the
  * newest history entry, then the push of the current state, cuda.cu:114-139)
  * and writes its guesses to JOB<RANK>/UPO_ZK.out and UPOinfo_ts.dat; as in the
  * reference, a check resets tme (the time column of stats.dat) to
  * TSTART + (NT - NOUT) DELT.  The time-averaged means/pdfs files are not produced. */
 #include "legacy_common.h"
+#include "../../../include/psg_stats.h"
 
-static void stats_line(const double *st, long nkt, const double *kx, const double *ky, const double *kz,
-                       const int *LL, double *tavg, int *tcount, double v2, double kfy, double ampfor, double tme,
-                       FILE *fs, FILE *fe) {
-  const double *X = st, *E = st + 2 * nkt, *Z = st + 4 * nkt;
-  double energy = 0, diss = 0, input = 0, eturb = 0, dturb = 0, XIa = 0, ETa = 0, ZTa = 0, Ua = 0, Va = 0, Wa = 0;
-  double *XT = tavg, *ET = tavg + 2 * nkt, *ZT = tavg + 4 * nkt;
-  const double tc = (double)*tcount;
-  for (long i = 0; i < nkt; i++) {
-    if (LL[i] != 1) continue;
-    double wk = kx[i] * kx[i] + ky[i] * ky[i] + kz[i] * kz[i];
-    wk = wk > 0.001 ? wk : 0.001;
-    const double xr = X[2 * i], xi = X[2 * i + 1], er = E[2 * i], ei = E[2 * i + 1], zr = Z[2 * i], zi = Z[2 * i + 1];
-    const double vzx = xr * xr + xi * xi, vzy = er * er + ei * ei, vzz = zr * zr + zi * zi;
-    const double vz = vzx + vzy + vzz;
-    diss += vz;
-    energy += vz / wk;
-    XIa += vzx; ETa += vzy; ZTa += vzz;
-    Ua += pow((ky[i] * zi - kz[i] * ei) / wk, 2) + pow((kz[i] * er - ky[i] * zr) / wk, 2);
-    Va += pow((kz[i] * xi - kx[i] * zi) / wk, 2) + pow((kx[i] * zr - kz[i] * xr) / wk, 2);
-    Wa += pow((kx[i] * ei - ky[i] * xi) / wk, 2) + pow((ky[i] * xr - kx[i] * er) / wk, 2);
-    XT[2 * i] = (XT[2 * i] * tc + xr) / (tc + 1.0); XT[2 * i + 1] = (XT[2 * i + 1] * tc + xi) / (tc + 1.0);
-    ET[2 * i] = (ET[2 * i] * tc + er) / (tc + 1.0); ET[2 * i + 1] = (ET[2 * i + 1] * tc + ei) / (tc + 1.0);
-    ZT[2 * i] = (ZT[2 * i] * tc + zr) / (tc + 1.0); ZT[2 * i + 1] = (ZT[2 * i + 1] * tc + zi) / (tc + 1.0);
-    const double v = XT[2 * i] * XT[2 * i] + XT[2 * i + 1] * XT[2 * i + 1] + ET[2 * i] * ET[2 * i] +
-                     ET[2 * i + 1] * ET[2 * i + 1] + ZT[2 * i] * ZT[2 * i] + ZT[2 * i + 1] * ZT[2 * i + 1];
-    eturb += v / wk;
-    dturb += v;
-    if (kx[i] == 0.0 && ky[i] == kfy && kz[i] == kfy) input += ampfor * xr;
-    if (kx[i] == 0.0 && ky[i] == kfy && kz[i] == -kfy) input -= ampfor * xr;
-  }
-  input *= v2 * kfy * kfy;
-  diss *= v2 * v2 * 8.0 * kfy * kfy;
-  dturb *= v2 * 16.0 * kfy * kfy;
-  (*tcount)++;
-  fprintf(fs, "%e %e %e %e %e %e \n", tme, energy, eturb, diss, dturb, input);
+/* computeStats (3D_NavierStokes/src/functions.h:128-209) from the device sums of
+ * psg_stats: the normalisations and the two output lines */
+static int stats_line(psg_stats *st, long tcount, double v2, double kfy, double tme, FILE *fs, FILE *fe) {
+  double v[PSG_STNS_N];
+  if (psg_stats_update(st, tcount, v)) return -1;
+  const double input = v[PSG_STNS_INPUT] * v2 * kfy * kfy;
+  const double diss = v[PSG_STNS_DISS] * v2 * v2 * 8.0 * kfy * kfy;
+  const double dturb = v[PSG_STNS_DTURB] * v2 * 16.0 * kfy * kfy;
+  fprintf(fs, "%e %e %e %e %e %e \n", tme, v[PSG_STNS_ENERGY], v[PSG_STNS_ETURB], diss, dturb, input);
   fflush(fs);
-  fprintf(fe, "%e %e %e %e %e %e %e \n", tme, Ua, Va, Wa, XIa, ETa, ZTa);
+  fprintf(fe, "%e %e %e %e %e %e %e \n", tme, v[PSG_STNS_UA], v[PSG_STNS_VA], v[PSG_STNS_WA], v[PSG_STNS_XIA],
+          v[PSG_STNS_ETA], v[PSG_STNS_ZTA]);
   fflush(fe);
+  return 0;
 }
 
 void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet, double *xir, double *etr,
@@ -58,7 +36,7 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
                             double *KFY, double *V2, double *alpha, int *ikF, int *ikN, int *LC, int *IKTX,
                             int *IKTY, int *IKTZ, int *KTZ, int *NKT, int *NX, int *NY, int *NZ, int *NSTOP,
                             int *NOUT, int *NOUTV, int *statsFLAG, int *RecFLAG, int *RANK) {
-  (void)ikF; (void)ikN; (void)IKTX; (void)IKTY; (void)IKTZ; (void)KTZ;
+  (void)kx; (void)ky; (void)kz; (void)ikF; (void)ikN; (void)IKTX; (void)IKTY; (void)IKTZ; (void)KTZ;
   psg_config c;
   memset(&c, 0, sizeof c);
   c.flavour = PSG_NS3D; c.precision = lg_precision(); c.batch = 1;
@@ -73,22 +51,20 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   lg_pack(buf, nkt, 3, f);
   if (psg_set_tables(ctx, nuzn, nu1, LC) || psg_set_state(ctx, 0, buf)) { lg_fail("upload"); free(buf); return; }
   FILE *fs = NULL, *fe = NULL;
-  double *tavg = NULL;
-  int tcount = 0;
+  psg_stats *st = NULL;
+  long tcount = 0;
   if (*statsFLAG != 0) {
     char p[64];
     snprintf(p, sizeof p, "JOB%1d/stats.dat", *RANK);
     fs = fopen(p, "w");
     snprintf(p, sizeof p, "JOB%1d/engy.dat", *RANK);
     fe = fopen(p, "w");
-    tavg = calloc(6 * nkt, sizeof(double));
-    if (!fs || !fe) { fprintf(stderr, "psgpu legacy: cannot open JOB%d/stats.dat\n", *RANK); }
-    if (!tavg) {
-      lg_fail("host allocation");
+    if (!fs || !fe || psg_stats_create(ctx, 0, &st)) {
+      if (!fs || !fe) fprintf(stderr, "psgpu legacy: cannot open JOB%d/stats.dat\n", *RANK);
+      else lg_fail("statistics");
       if (fs) fclose(fs);
       if (fe) fclose(fe);
-      free(buf);
-      return;
+      fs = fe = NULL;
     }
   }
   const int nout = *NOUT > 0 ? *NOUT : 1, nvort = *NOUTV > 0 ? *NOUTV : 1, nstop = *NSTOP;
@@ -129,9 +105,9 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
     nt = e;
     if (hf && (nt - 1) % nout == 0 && nt - 1 != 0) {
       if (upo.rec && lg_upo_update(&upo, nt - 1, *DELT, &tme)) { lg_fail("recurrence search"); break; }
-      if (fs) {
-        if (psg_get_state(ctx, 0, buf)) { lg_fail("download"); break; }
-        stats_line(buf, nkt, kx, ky, kz, LC, tavg, &tcount, *V2, *KFY, *AMPFOR, tme, fs, fe);
+      if (fs && (*statsFLAG == 1 || (*statsFLAG == 2 && tme > 500.0))) {
+        if (stats_line(st, tcount, *V2, *KFY, tme, fs, fe)) { lg_fail("statistics"); break; }
+        tcount++;
       }
     }
   }
@@ -140,8 +116,9 @@ void timestep_cuda_ns3d_dns(psg_double2 *xii, psg_double2 *eta, psg_double2 *zet
   *Time = *TSTART + nstop * *DELT;
   if (fs) fclose(fs);
   if (fe) fclose(fe);
+  if (st) psg_stats_destroy(st);
   lg_upo_close(&upo);
-  free(tavg); free(phys); free(buf);
+  free(phys); free(buf);
 }
 
 void timestep_cuda_(psg_double2 *, psg_double2 *, psg_double2 *, double *, double *, double *, double *, double *,

@@ -336,15 +336,19 @@ int bous_timederiv(Ctx *c, const cpx<double> *in_dev, cpx<double> *out_dev) {
 }
 
 // physical fields of one member: w(3), u(3), rho, NR doubles each, x fastest
-template <typename P> static int physb_t(Ctx *c, int member, double *out) {
+template <typename P> static int physb_t(Ctx *c, const cpx<double> *st, double *out) {
   const Grid &g = c->g;
-  PSG_TRYB(launch_zinv_b<P>(c, c->state + (long)member * 4 * g.nkt, 4 * g.nkt, 1, 1, 7));
+  PSG_TRYB(launch_zinv_b<P>(c, st, 4 * g.nkt, 1, 1, 7));
   PSG_TRYB(launch_yinv_g<P>(c, (const cpx<P> *)c->bufA, (cpx<P> *)c->bufB, 7));
   return launch_xphys<P>(c, (const cpx<P> *)c->bufB, out, 7);
 }
 
+int bous_physical_state(Ctx *c, const cpx<double> *st, double *out_dev) {
+  return c->cfg.precision == PSG_FP32BULK ? physb_t<float>(c, st, out_dev) : physb_t<double>(c, st, out_dev);
+}
+
 int bous_physical(Ctx *c, int member, double *out_dev) {
-  return c->cfg.precision == PSG_FP32BULK ? physb_t<float>(c, member, out_dev) : physb_t<double>(c, member, out_dev);
+  return bous_physical_state(c, c->state + (long)member * 4 * c->g.nkt, out_dev);
 }
 
 int bous_init(Ctx *c) {

@@ -148,6 +148,8 @@ int bous_init(Ctx *c);
 int bous_run(Ctx *c, int nsteps, bool first);
 int bous_timederiv(Ctx *c, const cpx<double> *in_dev, cpx<double> *out_dev);
 int bous_physical(Ctx *c, int member, double *out_dev);
+// physical fields (xi, eta, zeta, u, v, w, rho) of a Boussinesq state held anywhere in device memory
+int bous_physical_state(Ctx *c, const cpx<double> *st, double *out_dev);
 int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out);
 int ns2d_tl_run(Ctx *c, int nsteps, bool first);
 int ns3d_tl_run(Ctx *c, int nsteps, bool first);

@@ -15,6 +15,7 @@
 #include "psg_adjoint.h"
 #include "psg_newton.h"
 #include "psg_rec.h"
+#include "psg_stats.h"
 
 typedef std::vector<double> V;
 static int g_fail = 0, g_pass = 0;
@@ -328,6 +329,66 @@ static void rec_case(const char *name, int flav, int n, int rule, int W, int nsx
   check(same && worst < 1e-13, name, worst);
 }
 
+// Run statistics through the C-ABI: sums and averages against the oracle, the
+// physical-space pass (both gravity orientations), spectra against the oracle,
+// mean fields finite; a bounds and barrier check of every psg_stats kernel.
+static void stats_case(const char *name, int flav, int n, double theta) {
+  psg_config c = cfg3(flav, n, n, n, 1.0, PSG_FP64, 1, 0.02, 0.004, PSG_FORCE_KOLMO);
+  c.theta = theta;
+  c.ri = 0.7;
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, name, 1); return; }
+  const int nf = flav == PSG_BOUS3D ? 4 : 3;
+  or_grid g;
+  or_grid3(&g, n, n, n, 1.0);
+  std::vector<double> kx(g.nkt), ky(g.nkt), kz(g.nkt);
+  std::vector<int> L(g.nkt);
+  or_tables(&g, kx.data(), ky.data(), kz.data(), L.data());
+  psg_stats *s;
+  if (psg_stats_create(ctx, 0, &s)) { check(false, name, 1); psg_destroy(ctx); return; }
+  std::vector<orc> t((size_t)nf * g.nkt, orc{0.0, 0.0});
+  double worst = 0.0;
+  bool fin = true;
+  for (int k = 0; k < 3; k++) {
+    V z = ic3(n, n, n, 1.0, 300 + k, nf);
+    psg_set_state(ctx, 0, z.data());
+    double v[16], w[16];
+    psg_stats_update(s, k, v);
+    const orc *Z = (const orc *)z.data();
+    if (nf == 4)
+      or_stats_bous(g.nkt, kx.data(), ky.data(), kz.data(), L.data(), Z, Z + g.nkt, Z + 2 * g.nkt, Z + 3 * g.nkt,
+                    t.data(), t.data() + g.nkt, t.data() + 2 * g.nkt, t.data() + 3 * g.nkt, k, c.ampfor, c.kf, 0.7,
+                    theta, w);
+    else
+      or_stats_ns3d(g.nkt, kx.data(), ky.data(), kz.data(), L.data(), Z, Z + g.nkt, Z + 2 * g.nkt, t.data(),
+                    t.data() + g.nkt, t.data() + 2 * g.nkt, k, c.ampfor, c.kf, w);
+    for (int q = 0; q < (nf == 4 ? PSG_STB_N : PSG_STNS_N); q++)
+      worst = fmax(worst, fabs(v[q] - w[q]) / fmax(fabs(w[q]), 1e-300));
+    if (nf == 4) {
+      double um = 0, na[64], sa[64];
+      int np = 0;
+      if (psg_stats_physical(s, PSG_STP_RMS | PSG_STP_UMAX | PSG_STP_RIG, k, &um, na, sa, &np) || np != n) fin = false;
+      for (int i = 0; i < np; i++) fin = fin && std::isfinite(na[i]) && std::isfinite(sa[i]);
+    }
+  }
+  if (nf == 4) {
+    std::vector<double> h(n + 1), vv(n + 1), oh(n + 1), ov(n + 1);
+    int size = 0;
+    psg_stats_spectra(s, h.data(), vv.data(), n + 1, &size);
+    const int os = or_spectra(g.nkt, n, kx.data(), ky.data(), kz.data(), t.data(), t.data() + g.nkt,
+                              t.data() + 2 * g.nkt, theta, oh.data(), ov.data());
+    if (os != size) fin = false;
+    for (int i = 0; i <= n; i++) worst = fmax(worst, fabs(h[i] - oh[i]) / fmax(fabs(oh[i]), 1e-300) * (oh[i] != 0) +
+                                                      fabs(vv[i] - ov[i]) / fmax(fabs(ov[i]), 1e-300) * (ov[i] != 0));
+    std::vector<double> m((size_t)7 * n * n * n);
+    if (psg_stats_mean_fields(s, m.data())) fin = false;
+    for (double x : m) fin = fin && std::isfinite(x);
+  }
+  psg_stats_destroy(s);
+  psg_destroy(ctx);
+  check(fin && worst < 1e-12, name, worst);
+}
+
 int main(int argc, char **argv) {
   omp_set_dynamic(0);
   omp_set_max_active_levels(1);
@@ -341,6 +402,13 @@ int main(int argc, char **argv) {
   }
   if (argc > 1 && strcmp(argv[1], "floquet") == 0) {
     floquet_case(200, 1);
+    printf("%d passed, %d failed\n", g_pass, g_fail);
+    return g_fail ? 1 : 0;
+  }
+  if (argc > 1 && strcmp(argv[1], "stats") == 0) {
+    stats_case("stats ns3d 16^3", PSG_NS3D, 16, M_PI / 2);
+    stats_case("stats bous 16^3 theta pi/2", PSG_BOUS3D, 16, M_PI / 2);
+    stats_case("stats bous 16^3 theta pi/8", PSG_BOUS3D, 16, M_PI / 8);
     printf("%d passed, %d failed\n", g_pass, g_fail);
     return g_fail ? 1 : 0;
   }
@@ -399,6 +467,9 @@ int main(int argc, char **argv) {
   rec_case("rec ns3d 32^3 W 8 shifts 30x2x2 ring 6", PSG_NS3D, 32, PSG_REC_NS, 8, 30, 2, 2, 0.0, 6, 14);
   rec_case("rec bous 32^3 W 8 shifts 64 ring 8 (wraps)", PSG_BOUS3D, 32, PSG_REC_BOUS, 8, 64, 1, 1, 0.7, 8, 18);
   rec_case("rec ns2d 128^2 W 42 shifts 30x4 ring 8", PSG_NS2D, 128, PSG_REC_NS2D, 42, 30, 4, 1, 0.0, 8, 12);
+  stats_case("stats ns3d 16^3", PSG_NS3D, 16, M_PI / 2);
+  stats_case("stats bous 16^3 theta pi/2", PSG_BOUS3D, 16, M_PI / 2);
+  stats_case("stats bous 16^3 theta pi/8", PSG_BOUS3D, 16, M_PI / 8);
   printf("%d passed, %d failed\n", g_pass, g_fail);
   return g_fail ? 1 : 0;
 }

@@ -79,6 +79,11 @@ def _declare(L):
     L.or2_diag.argtypes = [P, P, P, P, P]
     L.or_set_threads.argtypes = [I]
     L.or_get_threads.restype = I
+    L.or_stats_ns3d.argtypes = [C.c_long] + [P] * 10 + [D, D, D, P]
+    L.or_stats_bous.argtypes = [C.c_long] + [P] * 12 + [D, D, D, D, D, P]
+    L.or_rig.argtypes = [I, I, I, D, P, P, P, P, P, P]
+    L.or_spectra.argtypes = [C.c_long, I, P, P, P, P, P, P, D, P, P]
+    L.or_spectra.restype = I
     L.or_rec_new.argtypes = [I, I, I, I, I, I, I, D, P, I, I, I, I, I, I, D, D, D, D, D, P]
     L.or_rec_new.restype = P
     L.or_rec_free.argtypes = [P]
@@ -354,3 +359,40 @@ class Rec:
         slots = np.zeros(self.store_size, np.int32)
         nj = lib().or_rec_last(self.h, ptr(res), ptr(slots), self.store_size)
         return res[:nj * self.ns].reshape(nj, self.ns), slots[:nj]
+
+
+def stats_ns3d(g, state, tavg, count, ampfor, kf):
+    """raw computeStats sums (3D Navier-Stokes); tavg (3, nkt) updated in place"""
+    kx, ky, kz, L = tables(g)
+    st = np.ascontiguousarray(state, dtype=np.complex128)
+    out = np.zeros(11)
+    lib().or_stats_ns3d(g.nkt, ptr(kx), ptr(ky), ptr(kz), ptr(L), ptr(st[0]), ptr(st[1]), ptr(st[2]),
+                        ptr(tavg[0]), ptr(tavg[1]), ptr(tavg[2]), float(count), ampfor, kf, ptr(out))
+    return out
+
+
+def stats_bous(g, state, tavg, count, ampfor, kf, ri, theta):
+    kx, ky, kz, L = tables(g)
+    st = np.ascontiguousarray(state, dtype=np.complex128)
+    out = np.zeros(12)
+    lib().or_stats_bous(g.nkt, ptr(kx), ptr(ky), ptr(kz), ptr(L), ptr(st[0]), ptr(st[1]), ptr(st[2]), ptr(st[3]),
+                        ptr(tavg[0]), ptr(tavg[1]), ptr(tavg[2]), ptr(tavg[3]), float(count), ampfor, kf, ri, theta,
+                        ptr(out))
+    return out
+
+
+def rig(nx, ny, nz, theta, U, V, W, RO):
+    npl = ny if theta < 2 * np.pi / 8 else nz
+    na, sa = np.zeros(npl), np.zeros(npl)
+    f = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (U, V, W, RO)]
+    lib().or_rig(nx, ny, nz, theta, ptr(f[0]), ptr(f[1]), ptr(f[2]), ptr(f[3]), ptr(na), ptr(sa))
+    return na, sa
+
+
+def spectra(g, tavg, theta):
+    kx, ky, kz, L = tables(g)
+    t = np.ascontiguousarray(tavg, dtype=np.complex128)
+    h, v = np.zeros(g.nx + 1), np.zeros(g.nx + 1)
+    size = lib().or_spectra(g.nkt, g.nx, ptr(kx), ptr(ky), ptr(kz), ptr(t[0]), ptr(t[1]), ptr(t[2]), theta, ptr(h),
+                            ptr(v))
+    return h, v, size
