@@ -249,10 +249,11 @@ def newton_case(P, n=512, t_spin=100.0, t_window=300.0, dt_sample=0.25, nguess=4
     return out
 
 
-def bous_ensemble(P, n=512, members=4, steps=6, warmup=2):
+def bous_ensemble(P, n=512, members=4, steps=20, warmup=2, tune=True):
     """3D Boussinesq n^3, N=Ri=1, Pr=1, nu=kappa=1/400, Kolmogorov forcing, RK4 with
     per-stage CN, dt=0.002, theta=90 (radians, as the reference passes it): `members`
-    seeded ensemble members stepped together, captured step graphs, FP64 and FP32-bulk."""
+    seeded ensemble members stepped together, captured step graphs, FP64 and FP32-bulk,
+    x-pass shape autotuned per precision; the energy comparison is after warmup + steps."""
     g = P.hgrid("bous", n, n, n)
     out = {"grid": [n, n, n], "members": members, "steps": steps, "dt": 0.002, "nu": 1 / 400.0, "ri": 1.0,
            "sc": 1.0, "theta": 90.0}
@@ -260,9 +261,16 @@ def bous_ensemble(P, n=512, members=4, steps=6, warmup=2):
     for prec in ("fp64", "fp32"):
         with P.Stepper(flavour="bous", n=n, v2=1 / 400.0, sc=1.0, ri=1.0, theta=90.0, dt=0.002, batch=members,
                        force="kolmogorov", precision=prec) as s:
-            s.set_graph(2)
             for m in range(members):
                 s.set_state(sts[m], m)
+            var = 0
+            if tune:
+                rep = s.autotune(2)
+                log("bench: bous %s autotune: %s" % (prec, rep.strip().replace("\n", "; ")))
+                var = s.get_variant(0)
+                for m in range(members):
+                    s.set_state(sts[m], m)
+            s.set_graph(2)
             s.run(warmup)
             s.sync()
             t0 = time.perf_counter()
@@ -271,8 +279,9 @@ def bous_ensemble(P, n=512, members=4, steps=6, warmup=2):
             dt = time.perf_counter() - t0
             e = [s.diag(m)[0] for m in range(members)]
         out[prec] = {"ensemble_steps_per_s": steps / dt, "member_steps_per_s": members * steps / dt,
-                     "ms_per_step": 1e3 * dt / steps, "energy": e}
-        log("bench: bous %d^3 x%d %s %.2f ensemble steps/s" % (n, members, prec, steps / dt))
+                     "ms_per_step": 1e3 * dt / steps, "energy": e, "xpass_variant": var}
+        log("bench: bous %d^3 x%d %s %.2f ensemble steps/s (x-pass variant %d)" % (n, members, prec, steps / dt,
+                                                                                   var))
     out["fp32_speedup"] = out["fp32"]["ensemble_steps_per_s"] / out["fp64"]["ensemble_steps_per_s"]
     out["fp32_energy_rel_err_vs_fp64"] = max(abs(a - b) / b for a, b in zip(out["fp32"]["energy"],
                                                                              out["fp64"]["energy"]))
@@ -370,6 +379,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel shapes (no autotune)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^3 adjoint-gradient case")
     ap.add_argument("--no-bous", action="store_true", help="skip the 512^3 Boussinesq ensemble case")
+    ap.add_argument("--bous-steps", type=int, default=20,
+                    help="timed Boussinesq ensemble steps (the configuration's run is 500)")
     ap.add_argument("--no-rec", action="store_true", help="skip the 256^3 recurrence-search timing")
     ap.add_argument("--no-energy", action="store_true",
                     help="skip the 1000-step FP32-bulk energy series (profiled runs)")
@@ -429,7 +440,7 @@ def main():
     bous = None
     if not a.no_bous:
         log("bench: boussinesq 512^3 ensemble")
-        bous = bous_ensemble(P)
+        bous = bous_ensemble(P, steps=a.bous_steps, tune=not a.no_tune)
     recur = None
     if not a.no_rec and n == 256:
         log("bench: recurrence search 256^3, 500-entry history")

@@ -149,6 +149,12 @@ int psg_device_count(void);
  *   9  as 8 with 4 rows per block
  *  10  as 1 with registers capped for 4 waves/SIMD (FP64) or 5 (FP32-bulk)
  *  11  as 1 with registers capped for 5 waves/SIMD (FP64) or 6 (FP32-bulk)
+ * x-pass of the Boussinesq right-hand side (9 inputs, 4 outputs), variants 0..11:
+ *   0-4 as above; 5, 6 as 2 with 2 and 4 rows per block; 7 as 4 with 1 row per
+ *   block; 8, 9 as above in FP32-bulk (in FP64: 4 points/thread, inputs per pair,
+ *   2 rows per block); 10 as 2 with registers capped for 4 waves/SIMD (FP64) or 5;
+ *   11 as 4 with registers capped for 5 waves/SIMD (FP64) or 6
+ *   (the Boussinesq flavour has no z-forward or slab variants)
  * z-forward + update pass of the 3D Navier-Stokes and Euler step (PSG_PASS_ZFWD),
  * variants 0..1:
  *   0  the three fields' inputs front-loaded

@@ -747,7 +747,7 @@ int psg_get_variant(psg_ctx *c, int pass) {
 int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
   if (!c) return PSG_EINVAL;
   std::string rep;
-  if (c->g.dim != 3 || c->cfg.flavour == PSG_BOUS3D) {
+  if (c->g.dim != 3) {
     rep = "no shape variants for this flavour";
   } else {
     if (steps < 1) steps = 1;
@@ -763,10 +763,13 @@ int psg_autotune(psg_ctx *c, int steps, char *report, int report_len) {
     PSG_HIP(hipEventCreate(&e1));
     int rc = 0;
     // passes are tuned one after the other, each with the others' choices fixed
+    // (the Boussinesq flavour has x-pass shapes only)
     struct { const char *name; int *var; int n; } passes[] = {{"xpass", &c->xvar, PSG_XPASS_VARIANTS},
                                                                 {"zfwd", &c->zvar, PSG_ZFWD_VARIANTS},
                                                                 {"slab", &c->slvar, PSG_SLAB_VARIANTS}};
-    for (auto &ps : passes) {
+    const int npass = c->cfg.flavour == PSG_BOUS3D ? 1 : 3;
+    for (int ip = 0; ip < npass; ip++) {
+      auto &ps = passes[ip];
       double best = 1e300;
       int bestv = *ps.var;
       for (int v = 0; v < ps.n && rc == 0; v++) {

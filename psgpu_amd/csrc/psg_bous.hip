@@ -11,7 +11,7 @@
 // One right-hand side: pass 1 from the stage state (vorticity, Biot-Savart
 // velocity and i k rho), y-inverse of 9 fields, x-pass product (-(u x w),
 // u.grad rho), y-forward of 4 fields, z-forward ending in the stage update.
-#include "psg_launch.hpp"
+#include "psg_ns3d.hpp"
 
 namespace psg {
 
@@ -222,34 +222,36 @@ static int launch_zinv_b(Ctx *c, const cpx<double> *st, long st_ms, int umask, i
   return 0;
 }
 
-// generic column/row passes shared with the Navier-Stokes path
-template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
-template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
 template <typename P> int launch_xphys(Ctx *c, const cpx<P> *in, double *out, int nf);
 
-template <typename P, int FI, int FO, class PROD>
-static int launch_xpass_b(Ctx *c, const cpx<P> *in, cpx<P> *out, long nrow_m, int nm, PROD prod, int zs0 = 0) {
-  const Grid &g = c->g;
-  PSG_DISPATCH_N(g.nx, {
-    if (g.iktx != xband<N>()) {
-      set_error("x-pass: IKTX %d is not NX/3+1", g.iktx);
-      return PSG_EINVAL;
-    }
-    constexpr int TE = Row<N>::TEAMS;
-    long nrows = nrow_m * nm;
-    if (nrows >= (1L << 31)) {
-      set_error("x-pass: %ld rows exceed the 32-bit row index", nrows);
-      return PSG_EINVAL;
-    }
-    dim3 grid((unsigned)((nrows + TE - 1) / TE));
-    size_t lds = (size_t)(row_len<N>() * TE + N) * sizeof(cpx<P>);
-    auto k = k_xpass<P, N, TE, FI, FO, PROD>;
-    PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Row<N>::THREADS), lds, c->stream, in, out, nrow_m, nrows, g, twp<P>(c, 0),
-                       prod, zs0);
-  });
-  PSG_HIP(hipGetLastError());
-  return 0;
+// x-pass shapes of the Boussinesq product (9 inputs, 4 outputs), chosen per
+// context by psg_autotune: the variants of the Navier-Stokes x-pass (points per
+// thread, input front-loading, per-pair product re-evaluation, rows per block,
+// register caps)
+template <typename P> static int xpass_bous(Ctx *c, const cpx<P> *B, cpx<P> *Cb, int nm) {
+  const long nr = (long)c->g.ny * c->g.nz;
+  const ProdBous<P> pb{};
+  switch (c->xvar) {
+    case 1: return launch_xpass<P, 9, 4, ProdBous<P>, 8, true, true>(c, B, Cb, nr, nm, pb);
+    case 2: return launch_xpass<P, 9, 4, ProdBous<P>, 8, false, true>(c, B, Cb, nr, nm, pb);
+    case 3: return launch_xpass<P, 9, 4, ProdBous<P>, 4, true, false>(c, B, Cb, nr, nm, pb);
+    case 4: return launch_xpass<P, 9, 4, ProdBous<P>, 4, false, true>(c, B, Cb, nr, nm, pb);
+    case 5: return launch_xpass<P, 9, 4, ProdBous<P>, 8, false, true, 2>(c, B, Cb, nr, nm, pb);
+    case 6: return launch_xpass<P, 9, 4, ProdBous<P>, 8, false, true, 4>(c, B, Cb, nr, nm, pb);
+    case 7: return launch_xpass<P, 9, 4, ProdBous<P>, 4, false, true, 1>(c, B, Cb, nr, nm, pb);
+    case 8:
+    case 9:
+      if constexpr (std::is_same<P, float>::value) {
+        if (c->xvar == 8) return launch_xpass<P, 9, 4, ProdBous<P>, 16, false, true>(c, B, Cb, nr, nm, pb);
+        return launch_xpass<P, 9, 4, ProdBous<P>, 16, false, true, 4>(c, B, Cb, nr, nm, pb);
+      }
+      return launch_xpass<P, 9, 4, ProdBous<P>, 4, false, true, 2>(c, B, Cb, nr, nm, pb);
+    case 10:
+      return launch_xpass<P, 9, 4, ProdBous<P>, 8, false, true, 0, sizeof(P) == 8 ? 4 : 5>(c, B, Cb, nr, nm, pb);
+    case 11:
+      return launch_xpass<P, 9, 4, ProdBous<P>, 4, false, true, 0, sizeof(P) == 8 ? 5 : 6>(c, B, Cb, nr, nm, pb);
+    default: return launch_xpass<P, 9, 4, ProdBous<P>>(c, B, Cb, nr, nm, pb);
+  }
 }
 
 template <typename P, class UPD> static int launch_zfwd_b(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
@@ -278,7 +280,7 @@ template <typename P, class UPD> static int rhs_b(Ctx *c, const cpx<double> *s0,
   PSG_TRYB(launch_yinv_g<P>(c, A, B, 9 * nm));
   prof_end(c);
   prof_begin(c, "xpass_bous", 13 * Y * b * nm);
-  PSG_TRYB((launch_xpass_b<P, 9, 4>(c, B, Cb, (long)g.ny * g.nz, nm, ProdBous<P>{})));
+  PSG_TRYB(xpass_bous<P>(c, B, Cb, nm));
   prof_end(c);
   prof_begin(c, "yfwd", 4 * (Y + Z) * b * nm);
   PSG_TRYB(launch_yfwd_g<P>(c, Cb, A, 4 * nm));

@@ -83,11 +83,13 @@ static void ns3d_case(const char *name, int flav, int nx, int ny, int nz, double
   check(worst < tol, name, worst);
 }
 
-static void bous_case(const char *name, int nx, int ny, int nz, double alpha, int prec, int batch, int graph) {
+static void bous_case(const char *name, int nx, int ny, int nz, double alpha, int prec, int batch, int graph,
+                      int variant = 0) {
   psg_config c = cfg3(PSG_BOUS3D, nx, ny, nz, alpha, prec, batch, 0.01, 0.004, PSG_FORCE_KOLMO);
   psg_ctx *ctx;
   if (psg_create(&c, &ctx)) { check(false, name, 1); return; }
   psg_set_graph(ctx, graph);
+  psg_set_variant(ctx, PSG_PASS_XPASS, variant);
   std::vector<V> st;
   for (int m = 0; m < batch; m++) { st.push_back(ic3(nx, ny, nz, alpha, 40 + m, 4)); psg_set_state(ctx, m, st[m].data()); }
   psg_run(ctx, 4);
@@ -362,7 +364,7 @@ static void stats_case(const char *name, int flav, int n, double theta) {
     else
       or_stats_ns3d(g.nkt, kx.data(), ky.data(), kz.data(), L.data(), Z, Z + g.nkt, Z + 2 * g.nkt, t.data(),
                     t.data() + g.nkt, t.data() + 2 * g.nkt, k, c.ampfor, c.kf, w);
-    for (int q = 0; q < (nf == 4 ? PSG_STB_N : PSG_STNS_N); q++)
+    for (int q = 0; q < (nf == 4 ? (int)PSG_STB_N : (int)PSG_STNS_N); q++)
       worst = fmax(worst, fabs(v[q] - w[q]) / fmax(fabs(w[q]), 1e-300));
     if (nf == 4) {
       double um = 0, na[64], sa[64];
@@ -402,6 +404,17 @@ int main(int argc, char **argv) {
   }
   if (argc > 1 && strcmp(argv[1], "floquet") == 0) {
     floquet_case(200, 1);
+    printf("%d passed, %d failed\n", g_pass, g_fail);
+    return g_fail ? 1 : 0;
+  }
+  if (argc > 1 && strcmp(argv[1], "bousvar") == 0) {
+    for (int v = 0; v < 12; v++) {
+      char nm[96];
+      snprintf(nm, sizeof nm, "bous 16^3 fp64 xpass variant %d", v);
+      bous_case(nm, 16, 16, 16, 1.0, PSG_FP64, 1, 0, v);
+      snprintf(nm, sizeof nm, "bous 32x16x16 a=0.5 fp32 xpass variant %d batch 2", v);
+      bous_case(nm, 32, 16, 16, 0.5, PSG_FP32BULK, 2, 0, v);
+    }
     printf("%d passed, %d failed\n", g_pass, g_fail);
     return g_fail ? 1 : 0;
   }
@@ -456,6 +469,14 @@ int main(int argc, char **argv) {
             PSG_FORCE_GMRES, 1e-11);
   ns3d_case("euler 16^3 fp64 graph", PSG_EULER3D, 16, 16, 16, 1.0, PSG_FP64, 0, 2, 5, PSG_FORCE_NONE, 1e-11);
   bous_case("bous 16^3 fp64 batch 2 graph", 16, 16, 16, 1.0, PSG_FP64, 2, 2);
+  for (int v = 1; v < 12; v++) {
+    char nm[96];
+    snprintf(nm, sizeof nm, "bous 16^3 fp64 xpass variant %d", v);
+    bous_case(nm, 16, 16, 16, 1.0, PSG_FP64, 1, 0, v);
+    if (quick) continue;
+    snprintf(nm, sizeof nm, "bous 32x16x16 a=0.5 fp32 xpass variant %d batch 2", v);
+    bous_case(nm, 32, 16, 16, 0.5, PSG_FP32BULK, 2, 0, v);
+  }
   if (!quick) bous_case("bous 32x16x16 a=0.5 fp32", 32, 16, 16, 0.5, PSG_FP32BULK, 1, 0);
   ns2d_case("ns2d 32^2 fp64 graph", 32, PSG_FP64, 1, 2, 5);
   ns2d_case("ns2d 64^2 fp64 batch 3", 64, PSG_FP64, 3, 0, 3);
