@@ -90,6 +90,19 @@ typedef struct {
 } psg_alc;
 int psg_newton_solve_alc(psg_ctx *ctx, double *z, double *T, double *sx, double *v2, const psg_alc *alc,
                          const psg_newton_opts *o, psg_newton_result *res);
+/* Many guesses on one batched 2D context.  The reference solves its guesses one
+ * after the other (2D_NavierStokes/src_RK4/src_gmres/main.F90:663-1845, the loop
+ * over IGUESS); here the context's `batch` members are lanes: guess g
+ * (0 <= g < nguess) runs psg_newton_solve's iteration on a free lane, and each
+ * time every active solver waits for a flow map (residual, finite-difference Jv,
+ * hookstep candidate) one batched run of the context serves them all, member m
+ * stepping with its own dt = T_m / INT(T_m/dt0) and standing still once done;
+ * a finished guess frees its lane for the next.  z: nguess * nf*nkt complex
+ * (host, in/out); T, sx (and sz, may be NULL): nguess each, in/out; res: nguess
+ * results.  Finite-difference Jv (jv = 0), uniform steps, no continuation.
+ * Returns 0, or the first error of any guess. */
+int psg_newton_batch(psg_ctx *ctx, int nguess, double *z, double *T, double *sx, double *sz,
+                     const psg_newton_opts *o, psg_newton_result *res);
 /* Phi_T with the solver's time policy: z (host, in/out) advanced by T */
 int psg_flow_map(psg_ctx *ctx, double *z, double T, double dt0, int time_policy);
 

@@ -176,6 +176,7 @@ def lib():
     L.psg_newton_solve.argtypes = [_P, _P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(NewtonOpts),
                                    C.POINTER(NewtonResult)]
     L.psg_flow_map.argtypes = [_P, _P, _D, _D, _I]
+    L.psg_newton_batch.argtypes = [_P, _I, _P, _P, _P, _P, C.POINTER(NewtonOpts), _P]
     L.psg_stats_create.argtypes = [_P, _I, C.POINTER(_P)]
     L.psg_stats_update.argtypes = [_P, _L, _P]
     L.psg_stats_physical.argtypes = [_P, _I, _L, C.POINTER(_D), _P, _P, C.POINTER(_I)]
@@ -455,6 +456,27 @@ class Stepper:
         res = {f: getattr(r, f) for f, _ in NewtonResult._fields_}
         res["status_name"] = NEWTON_STATUS.get(r.status, "?")
         return z, res
+
+    def newton_batch(self, guesses, T, sx=None, **opts):
+        """Newton-GMRES-hookstep on many guesses at once, the context's members serving as
+        lanes (psg_newton_batch); returns (z[nguess, nf, nkt], T, sx, list of result dicts)."""
+        o = NewtonOpts()
+        lib().psg_newton_defaults(self.cfg.flavour, C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        z = np.ascontiguousarray(guesses, dtype=np.complex128).reshape(-1, self.nf, self.nkt).copy()
+        ng = z.shape[0]
+        Tv = np.ascontiguousarray(np.broadcast_to(np.asarray(T, float), (ng,))).copy()
+        sv = np.zeros(ng) if sx is None else np.ascontiguousarray(np.broadcast_to(np.asarray(sx, float), (ng,))).copy()
+        rs = (NewtonResult * ng)()
+        _check(lib().psg_newton_batch(self.h, ng, _ptr(z), _ptr(Tv), _ptr(sv), None, C.byref(o),
+                                      C.cast(rs, C.c_void_p)), "psg_newton_batch")
+        out = []
+        for r in rs:
+            d = {f: getattr(r, f) for f, _ in NewtonResult._fields_}
+            d["status_name"] = NEWTON_STATUS.get(r.status, "?")
+            out.append(d)
+        return z, Tv, sv, out
 
     def floquet(self, orbit, T, sx=0.0, sz=0.0, vectors=False, **opts):
         """Leading Floquet multipliers of S_s Phi_T at the orbit: (mu, err, vecs or None, result dict)."""

@@ -34,6 +34,7 @@ C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-I" + os.path.join(ROOT, "inc
 CORE_HIP = ["psg_api.hip", "psg_ns3d.hip", "psg_ns2d.hip", "psg_bous.hip", "psg_newton.hip", "psg_tladj.hip",
             "psg_rec.hip", "psg_stats.hip"]
 CORE_C = ["psg_host.c"]
+CXX20_HIP = {"psg_newton.hip"}   # the Newton solver is a C++20 coroutine
 # legacy shims: (library suffix, source)
 LEGACY = [("ns3d_dns", "legacy/legacy_ns3d_dns.c"), ("ns3d_gmres", "legacy/legacy_ns3d_gmres.c"),
           ("euler", "legacy/legacy_euler.c"), ("ns2d_gmres", "legacy/legacy_ns2d_gmres.c"),
@@ -64,7 +65,10 @@ def _compile(src, verbose):
     base = os.path.splitext(os.path.basename(src))[0]
     obj = os.path.join(OBJ, base + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        flags = HIP_FLAGS
+        if src in CXX20_HIP:   # coroutines
+            flags = [f if f != "-std=c++17" else "-std=c++20" for f in HIP_FLAGS]
+        cmd = [HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
     else:
         cmd = ["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     _run(cmd, verbose)

@@ -271,7 +271,7 @@ static void free_ctx(Ctx *c) {
     if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
   void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->st32, c->stw, c->wnuzn, c->wnu1, c->wmask, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
                   c->tl_rh, c->tl_bufA, c->tl_bufB, c->tl_bufC, c->bufA, c->bufB, c->bufC,
-                  c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred, c->io};
+                  c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred, c->io, c->dtm};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto e : c->evs) hipEventDestroy(e);
@@ -437,6 +437,23 @@ int tl_flow_map(Ctx *c, const cpx<double> *z0, const cpx<double> *dz0, cpx<doubl
   return 0;
 }
 
+// per-member steps of a 2D context (host array of batch doubles, or NULL to go back
+// to cfg.dt); used by the batched Newton flows.  Synchronous: the array is read by
+// every later step, captured graphs included.
+int set_member_dt(Ctx *c, const double *host) {
+  if (c->g.dim != 2) { set_error("per-member steps: 2D contexts only"); return PSG_EINVAL; }
+  PSG_HIP(hipStreamSynchronize(c->stream));
+  if (!host) {
+    if (c->dtm_on) c->graph_gen++;
+    c->dtm_on = false;
+    return 0;
+  }
+  if (!c->dtm) PSG_HIP(hipMalloc(&c->dtm, sizeof(double) * c->batch));
+  PSG_HIP(hipMemcpy(c->dtm, host, sizeof(double) * c->batch, hipMemcpyHostToDevice));
+  if (!c->dtm_on) c->graph_gen++;
+  c->dtm_on = true;
+  return 0;
+}
 }  // namespace psg
 
 using namespace psg;

@@ -74,6 +74,9 @@ struct Ctx {
   cpx<double> *tw64[3] = {nullptr, nullptr, nullptr};
   cpx<float> *tw32[3] = {nullptr, nullptr, nullptr};
   double time = 0.0;
+  // 2D: per-member time steps (device, batch doubles) used instead of cfg.dt while dtm_on
+  double *dtm = nullptr;
+  bool dtm_on = false;
   // tangent-linear companion state (allocated on first use): perturbation,
   // its stage input and stage increments, and pass buffers sized for the
   // combined base + perturbation transforms
@@ -139,6 +142,7 @@ int ns3d_work_end(Ctx *c);
 int ns3d_tables_w(Ctx *c);
 int ns3d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
 int ns3d_physical(Ctx *c, int member, double *out_dev);
+int set_member_dt(Ctx *c, const double *host);
 int ns2d_init(Ctx *c);
 int ns2d_run(Ctx *c, int nsteps, bool first);
 int ns2d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);

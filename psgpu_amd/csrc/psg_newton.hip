@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <complex>
+#include <coroutine>
 #include <vector>
 #include "psg_launch.hpp"
 #include "../../include/psg_newton.h"
@@ -118,19 +119,19 @@ struct VecOps {
   explicit VecOps(Ctx *cc) : c(cc), n((long)cc->nf * cc->g.nkt), hpart(RED_BLOCKS) {
     if (hipMalloc(&dpart, sizeof(double) * RED_BLOCKS) != hipSuccess) dpart = nullptr;
   }
-  ~VecOps() { if (dpart) hipFree(dpart); }
+  ~VecOps() { if (dpart) (void)hipFree(dpart); }
   double dot(const cpx<double> *a, const cpx<double> *b) {
     hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(256), 0, c->stream, a, b, c->mask, c->g.nkt, n, dpart);
-    hipMemcpyAsync(hpart.data(), dpart, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, c->stream);
-    hipStreamSynchronize(c->stream);
+    (void)hipMemcpyAsync(hpart.data(), dpart, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
     double s = 0.0;
     for (int i = 0; i < RED_BLOCKS; i++) s += hpart[i];
     return s;
   }
   double dot_low(const cpx<double> *a, const cpx<double> *b, double kmax2) {
     hipLaunchKernelGGL(k_dot_low, dim3(RED_BLOCKS), dim3(256), 0, c->stream, a, b, c->g, kmax2, c->mask, n, dpart);
-    hipMemcpyAsync(hpart.data(), dpart, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, c->stream);
-    hipStreamSynchronize(c->stream);
+    (void)hipMemcpyAsync(hpart.data(), dpart, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
     double s = 0.0;
     for (int i = 0; i < RED_BLOCKS; i++) s += hpart[i];
     return s;
@@ -483,7 +484,7 @@ int psg_flow_map(psg_ctx *c, double *z, double T, double dt0, int policy) {
     PSG_HIP(hipMemcpyAsync(z, d, bytes, hipMemcpyDeviceToHost, c->stream));
     PSG_HIP(hipStreamSynchronize(c->stream));
   }
-  hipFree(d);
+  (void)hipFree(d);
   return r;
 }
 
@@ -502,8 +503,8 @@ int psg_tangent(psg_ctx *c, const double *z, const double *dz0, double T, double
     if ((z1 && hipMemcpyAsync(z1, o1, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
         hipMemcpyAsync(dz1, o2, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) { set_error("copy"); r = PSG_EHIP; }
   }
-  hipStreamSynchronize(c->stream);
-  hipFree(d);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
   return r;
 }
 
@@ -525,26 +526,63 @@ struct TlPrecision {
 // condition <dX/dr, X - X_r> = ds a further row (the reference's arclcFLAG2
 // branches, 2D_NavierStokes/src_RK4/src_gmres/main.F90:805-812, 1049-1062,
 // 1123-1155, 1350-1366, 1441-1530, 1678, 1720-1726).
-static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szio, double *v2io,
+}  // extern "C"
+
+// The solver is a coroutine that suspends at every flow map it needs (a residual,
+// a finite-difference Jv, a hookstep candidate); its driver serves the request,
+// either with one flow on the context (psg_newton_solve) or, for many guesses at
+// once, with one batched run of all waiting members (psg_newton_batch).
+struct FlowReq {
+  const cpx<double> *in = nullptr;
+  cpx<double> *out = nullptr;
+  double T = 0.0, dt0 = 0.0;
+  int policy = 0, rc = 0;
+};
+struct NTask {
+  struct promise_type {
+    FlowReq req;
+    int result = 0;
+    NTask get_return_object() { return NTask{std::coroutine_handle<promise_type>::from_promise(*this)}; }
+    std::suspend_always initial_suspend() noexcept { return {}; }
+    std::suspend_always final_suspend() noexcept { return {}; }
+    void return_value(int v) { result = v; }
+    void unhandled_exception() { result = PSG_EINVAL; }
+  };
+  std::coroutine_handle<promise_type> h;
+  explicit NTask(std::coroutine_handle<promise_type> hh) : h(hh) {}
+  NTask(NTask &&o) noexcept : h(o.h) { o.h = {}; }
+  NTask(const NTask &) = delete;
+  ~NTask() { if (h) h.destroy(); }
+};
+struct FlowAwait {
+  FlowReq r;
+  std::coroutine_handle<NTask::promise_type> hp;
+  bool await_ready() const noexcept { return false; }
+  void await_suspend(std::coroutine_handle<NTask::promise_type> h) noexcept {
+    hp = h;
+    h.promise().req = r;
+  }
+  int await_resume() const noexcept { return hp.promise().req.rc; }
+};
+
+NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szio, double *v2io,
                        const psg_alc *alc, const psg_newton_opts *o, psg_newton_result *res) {
-  if (!c || !zh || !Tio || !o || !res) { set_error("null argument"); return PSG_EINVAL; }
-  if (alc && (!v2io || !alc->dzdr || !alc->zr || !(*v2io > 0.0))) { set_error("newton: continuation needs v2, dzdr and zr"); return PSG_EINVAL; }
-  if (c->batch != 1) { set_error("psg_newton_solve needs a batch-1 context"); return PSG_EINVAL; }
-  if (o->jv < 0 || o->jv > 2) { set_error("newton: jv %d is not 0, 1 or 2", o->jv); return PSG_EINVAL; }
+  if (!c || !zh || !Tio || !o || !res) { set_error("null argument"); co_return PSG_EINVAL; }
+  if (alc && (!v2io || !alc->dzdr || !alc->zr || !(*v2io > 0.0))) { set_error("newton: continuation needs v2, dzdr and zr"); co_return PSG_EINVAL; }
+  if (o->jv < 0 || o->jv > 2) { set_error("newton: jv %d is not 0, 1 or 2", o->jv); co_return PSG_EINVAL; }
   if (o->jv == 2 && c->cfg.flavour != PSG_NS2D && c->cfg.flavour != PSG_NS3D) {
     set_error("newton: tangent-linear Jv is 2D/3D Navier-Stokes only");
-    return PSG_EINVAL;
+    co_return PSG_EINVAL;
   }
   TlPrecision tlp(c, o->jv == 2);
   const auto t_start = std::chrono::steady_clock::now();
   memset(res, 0, sizeof(*res));
   VecOps V(c);
-  if (!V.dpart) { set_error("newton: reduction buffer"); return PSG_ENOMEM; }
+  if (!V.dpart) { set_error("newton: reduction buffer"); co_return PSG_ENOMEM; }
   const long n = V.n;
   const size_t bytes = sizeof(cpx<double>) * n;
   const int K = o->max_gmres;
   const int ns = o->nshift;                    // shift unknowns (x[, z])
-  const int na = ns + (o->fixed_period ? 0 : 1);  // scalar unknowns
   const int sdir[2] = {0, 2};
   // device buffers
   std::vector<cpx<double> *> bufs;
@@ -566,10 +604,10 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
     else bufs.push_back(Q);
   }
   auto cleanup = [&]() {
-    hipStreamSynchronize(c->stream);
-    for (auto p : bufs) hipFree(p);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto p : bufs) (void)hipFree(p);
   };
-  if (!ok) { cleanup(); set_error("newton: device allocation"); return PSG_ENOMEM; }
+  if (!ok) { cleanup(); set_error("newton: device allocation"); co_return PSG_ENOMEM; }
   auto q = [&](int i) { return Q + (size_t)i * n; };
   constexpr int NQ = 4;   // scalar parts of a Krylov vector: [T, sx, sz, v2]
   std::vector<double> qN((size_t)(K + 1) * NQ, 0.0);
@@ -587,9 +625,10 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
   do {                       \
     if ((rc = (x))) {        \
       cleanup();             \
-      return rc;             \
+      co_return rc;          \
     }                        \
   } while (0)
+#define NFLOW(in, out, TT) NTRY((co_await FlowAwait{FlowReq{(in), (out), (TT), dt0, o->time_policy, 0}}))
   auto shift_of = [&](const cpx<double> *in, cpx<double> *out, double sx, double sz) {
     V.shift(in, out, sx, sy, ns > 1 ? sz : 0.0);
   };
@@ -614,7 +653,7 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
     V.copy(work, Zr);
   }
   NTRY(at_v2(v2N));
-  NTRY(flow(c, ZNEWT, ZN, T, dt0, o->time_policy));
+  NFLOW(ZNEWT, ZN, T);
   phi++;
   V.copy(ZN, ZNN);
   shift_of(ZN, ZNSH, sh[0], sh[1]);
@@ -645,9 +684,9 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
       // S dPhi/dv2 for the tangent-linear Jv, by a central difference in v2
       const double hv = 1e-4 * v2N;
       NTRY(at_v2(v2N + hv));
-      NTRY(flow(c, ZNEWT, dFdv, T, dt0, o->time_policy));
+      NFLOW(ZNEWT, dFdv, T);
       NTRY(at_v2(v2N - hv));
-      NTRY(flow(c, ZNEWT, work2, T, dt0, o->time_policy));
+      NFLOW(ZNEWT, work2, T);
       phi += 2;
       V.axpby(-1.0, work2, 1.0, dFdv);
       shift_of(dFdv, dFdv, sh[0], sh[1]);
@@ -666,19 +705,23 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
     int k = 0;
     std::vector<double> U, S, VV, P, X, Y;
     double GMR = 1.0;
-    // J applied to (dz, dT, ds): rows 1..N into out, scalar rows into vn
-    auto apply_J = [&](const cpx<double> *dz, const double *dsc, cpx<double> *out, double *vn) -> int {
+    // J applied to (dz, dT, ds): rows 1..N into out, scalar rows into vn.  In three
+    // parts around the flow a finite-difference Jv needs: jv_pre perturbs the state
+    // (into work), the driver flows it (APPLY_J), jv_post finishes
+    auto jv_pre = [&](const cpx<double> *dz, const double *dsc) -> int {
+      if (o->jv >= 1) return 0;
+      V.copy(ZNEWT, work);
+      V.axpby(o->eps, dz, 1.0, work);
+      // the finite difference perturbs v2 along with z (main.F90:1049-1062)
+      return alc ? at_v2(v2N + o->eps * dsc[3]) : 0;
+    };
+    auto jv_post = [&](const cpx<double> *dz, const double *dsc, cpx<double> *out, double *vn) -> int {
       int r;
       if (o->jv >= 1) {
         if ((r = tl_flow_map(c, ZNEWT, dz, work, out, T, dt0, o->time_policy, true))) return r;
         shift_of(out, out, sh[0], sh[1]);   // S dPhi dz
         if (alc && dsc[3] != 0.0) V.axpby(dsc[3], dFdv, 1.0, out);
       } else {
-        V.copy(ZNEWT, work);
-        V.axpby(o->eps, dz, 1.0, work);
-        // the finite difference perturbs v2 along with z (main.F90:1049-1062)
-        if (alc && (r = at_v2(v2N + o->eps * dsc[3]))) return r;
-        if ((r = flow(c, work, out, T, dt0, o->time_policy))) return r;
         if (alc && (r = at_v2(v2N))) return r;
         V.axpby(-1.0, ZNN, 1.0, out);
         shift_of(out, out, sh[0], sh[1]);
@@ -701,10 +744,16 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
       (void)a;
       return 0;
     };
+#define APPLY_J(dz, dsc, out, vn)                              \
+  do {                                                         \
+    NTRY(jv_pre((dz), (dsc)));                                 \
+    if (o->jv == 0) NFLOW(work, (out), T);                     \
+    NTRY(jv_post((dz), (dsc), (out), (vn)));                   \
+  } while (0)
     for (k = 1; k <= K; k++) {
       double vn[NQ] = {0, 0, 0, 0};
       double dsc[NQ] = {qn(k - 1, 0), qn(k - 1, 1), qn(k - 1, 2), qn(k - 1, 3)};
-      NTRY(apply_J(q(k - 1), dsc, v, vn));
+      APPLY_J(q(k - 1), dsc, v, vn);
       for (int i = 0; i < k; i++) {
         double hij = V.dot(q(i), v);
         if (!o->fixed_period) hij += qn(i, 0) * vn[0];
@@ -777,7 +826,7 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
       if (!alc) dsc[3] = 0.0;
       // predicted residual F + J DIFF
       double vn[NQ];
-      NTRY(apply_J(DIFF, dsc, v, vn));
+      APPLY_J(DIFF, dsc, v, vn);
       V.axpby(1.0, FN, 1.0, v);
       NORMFNpred = norm(v);
       // const = 0 in the reference (main.F90:135), so the bound is |b| itself (main.F90:1435-1442)
@@ -789,7 +838,7 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
       for (int s = 0; s < ns; s++) Splus[s] = sh[s] + dsc[1 + s];
       v2plus = v2N + dsc[3];
       NTRY(at_v2(v2plus));
-      NTRY(flow(c, Zplus, ZN, Tplus, dt0, o->time_policy));
+      NFLOW(Zplus, ZN, Tplus);
       NTRY(at_v2(v2N));
       phi++;
       shift_of(ZN, ZNSH, Splus[0], Splus[1]);
@@ -875,7 +924,7 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
     if (T < 0.1) { status = PSG_NEWTON_PERIOD_SMALL; break; }
   }
   NTRY(hipMemcpyAsync(zh, ZNEWT, bytes, hipMemcpyDeviceToHost, c->stream) == hipSuccess ? 0 : PSG_EHIP);
-  hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->stream);
   *Tio = T;
   if (sxio) *sxio = sh[0];
   if (szio) *szio = sh[1];
@@ -891,19 +940,124 @@ static int newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double
   res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   cleanup();
 #undef NTRY
-  return 0;
+#undef NFLOW
+#undef APPLY_J
+  co_return 0;
+}
+
+extern "C" {
+
+// serve each flow request of one solver with a flow on the context (member 0)
+static int run_direct(Ctx *c, NTask &t) {
+  auto h = t.h;
+  h.resume();
+  while (!h.done()) {
+    FlowReq &rq = h.promise().req;
+    rq.rc = flow(c, rq.in, rq.out, rq.T, rq.dt0, rq.policy);
+    h.resume();
+  }
+  return h.promise().result;
 }
 
 int psg_newton_solve(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szio, const psg_newton_opts *o,
                      psg_newton_result *res) {
-  return newton_core(c, zh, Tio, sxio, szio, nullptr, nullptr, o, res);
+  if (c && c->batch != 1) { set_error("psg_newton_solve needs a batch-1 context (psg_newton_batch: several)"); return PSG_EINVAL; }
+  NTask t = newton_core(c, zh, Tio, sxio, szio, nullptr, nullptr, o, res);
+  return run_direct(c, t);
 }
 
 int psg_newton_solve_alc(psg_ctx *c, double *zh, double *Tio, double *sxio, double *v2io, const psg_alc *alc,
                          const psg_newton_opts *o, psg_newton_result *res) {
   if (!alc) { set_error("null argument"); return PSG_EINVAL; }
   if (o && o->nshift > 1) { set_error("newton: continuation takes the x shift only"); return PSG_EINVAL; }
-  return newton_core(c, zh, Tio, sxio, nullptr, v2io, alc, o, res);
+  if (c && c->batch != 1) { set_error("psg_newton_solve_alc needs a batch-1 context"); return PSG_EINVAL; }
+  NTask t = newton_core(c, zh, Tio, sxio, nullptr, v2io, alc, o, res);
+  return run_direct(c, t);
+}
+
+int psg_newton_batch(psg_ctx *c, int nguess, double *z, double *T, double *sx, double *sz, const psg_newton_opts *o,
+                     psg_newton_result *res) {
+  if (!c || !z || !T || !o || !res || nguess < 1) { set_error("psg_newton_batch: bad argument"); return PSG_EINVAL; }
+  if (c->g.dim != 2) { set_error("psg_newton_batch: 2D Navier-Stokes contexts only (per-member time steps)"); return PSG_EINVAL; }
+  if (o->jv != 0) { set_error("psg_newton_batch: finite-difference Jv only (jv = 0)"); return PSG_EINVAL; }
+  if (o->time_policy != PSG_TIME_UNIFORM) { set_error("psg_newton_batch: uniform steps only"); return PSG_EINVAL; }
+  const int B = c->batch;
+  const long n = (long)c->nf * c->g.nkt;
+  const size_t bytes = sizeof(cpx<double>) * n;
+  using H = std::coroutine_handle<NTask::promise_type>;
+  std::vector<H> lane(B);
+  int next = 0, rc = 0;
+  // start guesses in lane m until one waits on a flow (or none are left)
+  auto start = [&](int m) {
+    while (next < nguess) {
+      const int gi = next++;
+      NTask t = newton_core(c, z + (size_t)gi * 2 * n, T + gi, sx ? sx + gi : nullptr, sz ? sz + gi : nullptr,
+                            nullptr, nullptr, o, res + gi);
+      H h = t.h;
+      t.h = {};
+      h.resume();
+      if (!h.done()) { lane[m] = h; return; }
+      if (h.promise().result && !rc) rc = h.promise().result;
+      h.destroy();
+    }
+    lane[m] = H{};
+  };
+  for (int m = 0; m < B; m++) start(m);
+  std::vector<double> dts(B, 0.0), dtm(B, 0.0);
+  std::vector<int> nst(B, 0);
+  int e = 0;
+  for (;;) {
+    // one batched flow for every lane that waits on one: member m steps with
+    // dt_m = T_m / N_m for N_m = INT(T_m / dt0) steps, then stands still (dt 0)
+    std::vector<int> bounds;
+    bool any = false;
+    for (int m = 0; m < B; m++) {
+      nst[m] = 0;
+      dtm[m] = 0.0;
+      if (!lane[m]) continue;
+      const FlowReq &rq = lane[m].promise().req;
+      int ns = (int)(rq.T / rq.dt0);
+      if (ns < 1) ns = 1;
+      nst[m] = ns;
+      dtm[m] = rq.T / (double)ns;
+      bounds.push_back(ns);
+      any = true;
+      if (hipMemcpyAsync(c->state + (size_t)m * n, rq.in, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+        set_error("psg_newton_batch: state copy");
+        e = PSG_EHIP;
+      }
+    }
+    if (!any || e) break;
+    std::sort(bounds.begin(), bounds.end());
+    bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
+    int cur = 0;
+    for (int b : bounds) {
+      for (int m = 0; m < B; m++) dts[m] = nst[m] > cur ? dtm[m] : 0.0;
+      if ((e = set_member_dt(c, dts.data())) || (e = psg_run(static_cast<psg_ctx *>(c), b - cur))) break;
+      cur = b;
+    }
+    if (e) break;
+    for (int m = 0; m < B; m++) {
+      if (!lane[m]) continue;
+      FlowReq &rq = lane[m].promise().req;
+      rq.rc = hipMemcpyAsync(rq.out, c->state + (size_t)m * n, bytes, hipMemcpyDeviceToDevice, c->stream) == hipSuccess
+                  ? 0 : PSG_EHIP;
+    }
+    for (int m = 0; m < B; m++) {
+      if (!lane[m]) continue;
+      lane[m].resume();
+      if (lane[m].done()) {
+        if (lane[m].promise().result && !rc) rc = lane[m].promise().result;
+        lane[m].destroy();
+        start(m);
+      }
+    }
+  }
+  for (int m = 0; m < B; m++)
+    if (lane[m]) lane[m].destroy();
+  const int e2 = set_member_dt(c, nullptr);
+  if (e) return e;
+  return rc ? rc : e2;
 }
 
 void psg_floquet_defaults(psg_floquet_opts *o) {
@@ -946,8 +1100,8 @@ int psg_floquet(psg_ctx *c, const double *zh, double T, double sx, double sz, co
   cpx<double> *Z0 = q(m + 1), *ZN = q(m + 2), *w = q(m + 3), *t1 = q(m + 4), *t2 = q(m + 5);
   int rc = 0;
   auto done = [&](int r) {
-    hipStreamSynchronize(c->stream);
-    hipFree(buf);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
     res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     return r;
   };

@@ -25,9 +25,11 @@ template <int STAGE> struct Upd2D {
   long nkt;
   const int8_t *L;
   double dt, v2, famp, kf;
+  const double *dtm = nullptr;   // per-member steps (batched Newton flows); dt 0 leaves a member unchanged
   __device__ __forceinline__ void operator()(int m, long ik, int, double kx, double ky,
                                              cpx<double> (&N)[2]) const {
     cpx<double> *Z = z + (long)m * nkt + ik;
+    const double dt = dtm ? dtm[m] : this->dt;
     if (L[ik] == 1) {
       // NZK[ik] = FFT(v w), NZK[nkt+ik] = FFT(u w)
       double NZKX = kx * N[1].y + ky * N[0].y;
@@ -183,13 +185,13 @@ template <typename P> static int run2_t(Ctx *c, int nsteps, bool first) {
   }
   const double famp = c->cfg.force == PSG_FORCE_NONE ? 0.0 : c->cfg.ampfor;
   for (int nt = 0; nt < nsteps; nt++) {
-    Upd2D<0> u0{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
+    Upd2D<0> u0{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf, c->dtm_on ? c->dtm : nullptr};
     PSG_TRY2(rhs2<P>(c, c->stage, u0, c->batch));
-    Upd2D<1> u1{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
+    Upd2D<1> u1{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf, c->dtm_on ? c->dtm : nullptr};
     PSG_TRY2(rhs2<P>(c, c->stage, u1, c->batch));
-    Upd2D<2> u2{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
+    Upd2D<2> u2{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf, c->dtm_on ? c->dtm : nullptr};
     PSG_TRY2(rhs2<P>(c, c->stage, u2, c->batch));
-    Upd2D<3> u3{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf};
+    Upd2D<3> u3{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf, c->dtm_on ? c->dtm : nullptr};
     PSG_TRY2(rhs2<P>(c, c->stage, u3, c->batch));
   }
   return 0;

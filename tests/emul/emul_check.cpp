@@ -210,6 +210,50 @@ static void newton2d_case(int jv) {
   check(r.status == PSG_NEWTON_CONVERGED, jv ? "newton 2d laminar (TL Jv)" : "newton 2d laminar (FD Jv)", r.ner);
 }
 
+// three guesses of different periods on two lanes against one-guess solves
+static void newton2d_batch_case() {
+  const int n = 32, ng = 3;
+  psg_config c = cfg3(PSG_NS2D, n, n, 1, 1.0, 0, 2, 1 / 5.0, 0.01, PSG_FORCE_KOLMO);
+  psg_config c1 = cfg3(PSG_NS2D, n, n, 1, 1.0, 0, 1, 1 / 5.0, 0.01, PSG_FORCE_KOLMO);
+  psg_ctx *ctx, *ctx1;
+  if (psg_create(&c, &ctx) || psg_create(&c1, &ctx1)) { check(false, "newton batch create", 1); return; }
+  or_grid g;
+  or_grid2(&g, n, n, 1.0);
+  std::vector<double> kx(g.nkt), ky(g.nkt);
+  std::vector<int> L(g.nkt);
+  or_tables(&g, kx.data(), ky.data(), nullptr, L.data());
+  V z((size_t)2 * g.nkt * ng, 0.0), z1;
+  for (int q = 0; q < ng; q++) {
+    V d(2 * g.nkt);
+    or_ic_ns2d_cfg(&g, 9 + q, 0.05, (orc *)d.data());
+    for (long i = 0; i < g.nkt; i++) {
+      double *zz = z.data() + (size_t)q * 2 * g.nkt;
+      if (kx[i] == 0 && ky[i] == 4) zz[2 * i] = -5.0 / 8;
+      zz[2 * i] += d[2 * i]; zz[2 * i + 1] += d[2 * i + 1];
+    }
+  }
+  z1 = z;
+  psg_newton_opts o;
+  psg_newton_defaults(PSG_NS2D, &o);
+  o.nshift = 0; o.fixed_period = 1; o.jv = 0; o.dt0 = 0.01; o.max_gmres = 12; o.max_newton = 8;
+  psg_newton_result r[ng], r1;
+  double T[ng] = {0.2, 0.33, 0.25}, sx[ng] = {0, 0, 0};
+  const int rc = psg_newton_batch(ctx, ng, z.data(), T, sx, nullptr, &o, r);
+  double worst = 0.0;
+  bool same = rc == 0;
+  for (int q = 0; q < ng; q++) {
+    double T1 = T[q], sx1 = 0, sz1 = 0;
+    psg_newton_solve(ctx1, z1.data() + (size_t)q * 2 * g.nkt, &T1, &sx1, &sz1, &o, &r1);
+    same = same && r[q].status == PSG_NEWTON_CONVERGED && r1.status == PSG_NEWTON_CONVERGED &&
+           r[q].newton_iters == r1.newton_iters && r[q].gmres_iters == r1.gmres_iters;
+    for (long i = 0; i < 2 * g.nkt; i++)
+      worst = fmax(worst, fabs(z[(size_t)q * 2 * g.nkt + i] - z1[(size_t)q * 2 * g.nkt + i]));
+  }
+  psg_destroy(ctx);
+  psg_destroy(ctx1);
+  check(same && worst < 1e-11, "newton 2d batch of 3 guesses on 2 lanes (FD Jv)", worst);
+}
+
 // Floquet analysis of the laminar Kolmogorov state (32^2, Re = 40, T = 0.2) with
 // the tangent-linear Jv and the Arnoldi size of the 2D reference (ncv = 200):
 // runs to completion, leading multiplier real and unstable
@@ -418,6 +462,13 @@ int main(int argc, char **argv) {
     printf("%d passed, %d failed\n", g_pass, g_fail);
     return g_fail ? 1 : 0;
   }
+  if (argc > 1 && strcmp(argv[1], "newton") == 0) {
+    newton2d_case(0);
+    newton2d_case(1);
+    newton2d_batch_case();
+    printf("%d passed, %d failed\n", g_pass, g_fail);
+    return g_fail ? 1 : 0;
+  }
   if (argc > 1 && strcmp(argv[1], "stats") == 0) {
     stats_case("stats ns3d 16^3", PSG_NS3D, 16, M_PI / 2);
     stats_case("stats bous 16^3 theta pi/2", PSG_BOUS3D, 16, M_PI / 2);
@@ -485,6 +536,7 @@ int main(int argc, char **argv) {
   if (!quick) tladj_case(16, PSG_FP32BULK);
   newton2d_case(0);
   newton2d_case(1);
+  newton2d_batch_case();
   rec_case("rec ns3d 32^3 W 8 shifts 30x2x2 ring 6", PSG_NS3D, 32, PSG_REC_NS, 8, 30, 2, 2, 0.0, 6, 14);
   rec_case("rec bous 32^3 W 8 shifts 64 ring 8 (wraps)", PSG_BOUS3D, 32, PSG_REC_BOUS, 8, 64, 1, 1, 0.7, 8, 18);
   rec_case("rec ns2d 128^2 W 42 shifts 30x4 ring 8", PSG_NS2D, 128, PSG_REC_NS2D, 42, 30, 4, 1, 0.0, 8, 12);

@@ -91,3 +91,37 @@ def test_newton_ner2_low_wavenumber_residual(gpu):
     print("NER %.6e (host %.6e)  NER2 %.6e (host %.6e)" % (res["ner"], ner, res["ner2"], ner2), flush=True)
     assert 0.0 < res["ner2"] <= res["ner"]
     assert abs(res["ner2"] - ner2) <= 1e-9 * ner2 and abs(res["ner"] - ner) <= 1e-9 * ner
+
+
+def test_newton_batch_matches_one_guess_at_a_time(gpu):
+    # five guesses of different periods on three lanes: every guess must reach the
+    # laminar fixed point exactly as the one-guess solver does from the same start
+    n, re_ = 32, 5.0
+    g = O.grid2(n, n)
+    kx, ky, _, L = O.tables(g)
+    lam = np.zeros(g.nkt, complex)
+    lam[np.where((kx == 0) & (ky == 4))[0][0]] = -re_ / 8
+    guesses = np.array([lam + masked_random(g, 20 + i, 0.05) for i in range(5)])[:, None, :]
+    Ts = np.array([1.5, 2.0, 2.5, 1.75, 3.0])
+    opts = dict(nshift=0, fixed_period=1, jv=0, dt0=0.01, max_gmres=40)
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / re_, dt=0.01, kf=4.0, ampfor=-2.0, batch=3) as sb:
+        zb, Tb, _, rb = sb.newton_batch(guesses, Ts, **opts)
+    with P.Stepper(flavour="ns2d", n=n, v2=1 / re_, dt=0.01, kf=4.0, ampfor=-2.0) as s1:
+        for i in range(5):
+            z1, r1 = s1.newton(guesses[i], T=Ts[i], **opts)
+            assert rb[i]["status_name"] == "CONVERGED" == r1["status_name"], (i, rb[i], r1)
+            assert rb[i]["newton_iters"] == r1["newton_iters"] and rb[i]["gmres_iters"] == r1["gmres_iters"]
+            assert abs(rb[i]["ner"] - r1["ner"]) <= 1e-3 * r1["ner"] + 1e-15
+            assert np.abs(zb[i, 0] - z1[0]).max() < 1e-11
+            assert np.abs(zb[i, 0] - lam).max() < 1e-10
+            assert Tb[i] == Ts[i]
+
+
+def test_newton_batch_rejects_what_it_does_not_batch(gpu):
+    with P.Stepper(flavour="ns2d", n=32, v2=0.2, dt=0.01, batch=2) as sb:
+        g = np.zeros((2, 1, sb.nkt), complex)
+        with pytest.raises(P.PsgError, match="finite-difference"):
+            sb.newton_batch(g, [1.0, 1.0], jv=1)
+    with P.Stepper(flavour="ns3d", n=16, batch=2) as s3:
+        with pytest.raises(P.PsgError, match="2D"):
+            s3.newton_batch(np.zeros((2, 3, s3.nkt), complex), [1.0, 1.0])
