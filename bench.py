@@ -249,6 +249,48 @@ def newton_case(P, n=512, t_spin=100.0, t_window=300.0, dt_sample=0.25, nguess=4
     return out
 
 
+def newton_batch_case(P, n=512, nguess=8, lanes=8, max_newton=20, t_spin=100.0, t_window=300.0, dt_sample=0.25,
+                      tmin=5.0, tmax=30.0, sequential=True):
+    """The Newton case with many guesses at once (psg_newton_batch): the nguess closest
+    recurrences of the same DNS as newton_case, solved with finite-difference Jv on one
+    context of `lanes` members, and (sequential) one after the other on a batch-1 context."""
+    from psgpu_amd import recurrence as R
+    g = P.hgrid("ns2d", n, n)
+    L = P.tables(g)[3]
+    z0 = P.ic_ns2d_cfg(g, 20240607, 1.0)
+    kw = dict(flavour="ns2d", n=n, v2=1 / 40.0, dt=0.005, kf=4.0, ampfor=-2.0)
+    with P.Stepper(**kw) as s:
+        times, snaps = R.record(s, z0, t_spin, t_window, dt_sample)
+    hits = R.best_recurrences(times, snaps, g.iktx, L, tmin, tmax, top=nguess)
+    zs = np.array([snaps[h["i"]][None] for h in hits])
+    Ts = np.array([h["T"] for h in hits])
+    ss = np.array([h["shift"] for h in hits])
+    opts = dict(jv=0, netol=1e-10, max_gmres=40, max_newton=max_newton, dt0=0.005, verbose=1)
+    out = {"grid": [n, n], "guesses": len(hits), "lanes": lanes, "T": list(Ts), "max_newton": max_newton,
+           "T_range": [tmin, tmax]}
+    log("bench: newton batch guesses T = " + ", ".join("%.3f" % t for t in Ts))
+    with P.Stepper(batch=lanes, **kw) as sb:
+        t0 = time.perf_counter()
+        zb, Tb, sxb, rb = sb.newton_batch(zs, Ts, sx=ss, **opts)
+        out["batched_seconds"] = time.perf_counter() - t0
+    out["batched"] = [{"status": r["status_name"], "ner": r["ner"], "T": r["T"], "iters": r["newton_iters"],
+                       "phi_evals": r["phi_evals"]} for r in rb]
+    out["batched_converged"] = sum(r["status_name"] == "CONVERGED" for r in rb)
+    log("bench: newton batch of %d guesses on %d lanes: %.1f s, %d converged" % (
+        len(hits), lanes, out["batched_seconds"], out["batched_converged"]))
+    if sequential:
+        with P.Stepper(**kw) as s:
+            t0 = time.perf_counter()
+            rs = [s.newton(zs[k], T=Ts[k], sx=ss[k], **opts)[1] for k in range(len(hits))]
+            out["sequential_seconds"] = time.perf_counter() - t0
+        out["sequential"] = [{"status": r["status_name"], "ner": r["ner"], "T": r["T"], "iters": r["newton_iters"]}
+                             for r in rs]
+        out["sequential_converged"] = sum(r["status_name"] == "CONVERGED" for r in rs)
+        log("bench: the same guesses one after the other: %.1f s, %d converged" % (
+            out["sequential_seconds"], out["sequential_converged"]))
+    return out
+
+
 def bous_ensemble(P, n=512, members=4, steps=20, warmup=2, tune=True):
     """3D Boussinesq n^3, N=Ri=1, Pr=1, nu=kappa=1/400, Kolmogorov forcing, RK4 with
     per-stage CN, dt=0.002, theta=90 (radians, as the reference passes it): `members`
@@ -387,6 +429,11 @@ def main():
     ap.add_argument("--newton", type=int, default=1, metavar="NGUESS",
                     help="2D 512^2 Newton case (FD vs TL Jv) on NGUESS recurrence guesses (0: skip)")
     ap.add_argument("--newton-n", type=int, default=512)
+    ap.add_argument("--newton-batch", type=int, default=0, metavar="LANES",
+                    help="also solve --newton-guesses guesses together on LANES members (0: skip)")
+    ap.add_argument("--newton-guesses", type=int, default=8)
+    ap.add_argument("--newton-batch-iters", type=int, default=10, help="Newton iterations per batched guess")
+    ap.add_argument("--newton-batch-tmax", type=float, default=8.0, help="longest guess period of the batch")
     ap.add_argument("--save-orbit", default=None, metavar="NPZ",
                     help="write the first orbit the finite-difference Newton converges (active modes, T, shift)")
     a = ap.parse_args()
@@ -451,6 +498,11 @@ def main():
     if a.newton:
         log("bench: newton 2D %d^2, %d guesses" % (a.newton_n, a.newton))
         newton = newton_case(P, n=a.newton_n, nguess=a.newton, save_orbit=a.save_orbit)
+    nbatch = None
+    if a.newton_batch:
+        log("bench: newton 2D %d^2, %d guesses on %d lanes" % (a.newton_n, a.newton_guesses, a.newton_batch))
+        nbatch = newton_batch_case(P, n=a.newton_n, nguess=a.newton_guesses, lanes=a.newton_batch,
+                                   max_newton=a.newton_batch_iters, tmax=a.newton_batch_tmax)
     cpu = None
     if not a.no_cpu:
         log("bench: cpu baseline (%d steps)" % a.cpu_steps)
@@ -463,6 +515,7 @@ def main():
                       "members": 1, "pass_variants": var64},
            "roofline": roof, "model": model, "fp32_bulk": fp32, "kernels_fp64": prof64,
            "adjoint_128": adj, "bous_ensemble_512": bous, "recurrence_256": recur, "newton_2d": newton,
+           "newton_2d_batch": nbatch,
            "cpu_baseline": cpu}
     print(json.dumps(out))
 

@@ -49,17 +49,25 @@ typedef struct {
   int nshift;         /* 1: sx unknown; 2: sx and sz unknown (Boussinesq) */
   int fixed_period;   /* FixedP: T not an unknown */
   int jv;             /* 0 finite difference, 1 tangent linear, 2 tangent linear with FP32-bulk
-                         passes (the residual flow keeps the context's precision) */
+                         passes (the residual flow keeps the context's precision).  With a
+                         tangent-linear Jv the hookstep's predicted residual F + J dx is formed
+                         from the Arnoldi relation J Q_k = Q_{k+1} H_k instead of one more Jv */
   int time_policy;    /* PSG_TIME_* */
   double dt0;         /* TRUEDELT */
   double sy;          /* fixed y shift SNEWTY */
   int stall_count;    /* NERcount limit (2D 3, 3D 5) */
   int verbose;
+  int tl_record;      /* tangent-linear Jv on a one-member 2D context: 1 (default) records the
+                         physical fields of every stage of the base flow Phi_T(z) once per Newton
+                         step, and each Jv then integrates the perturbation alone against the
+                         record; 0 integrates base and perturbation together at every Jv.  The
+                         record takes 4*steps*3*NX*NY values (8 bytes, 4 with jv = 2) and is
+                         skipped when the device lacks the room */
 } psg_newton_opts;
 
 typedef struct {
   double T, sx, sz;
-  double ner;         /* ||F|| / ||Phi_T(z)|| */
+  double ner;         /* ||F|| / ||Phi_T(z)|| (Boussinesq: ||F|| / ||(Phi_T(z), z)||) */
   double ner2;        /* low-wavenumber residual NER2 (|k|^2 <= KYmax2) */
   int newton_iters, gmres_iters, hook_iters, status;
   double seconds;
@@ -110,6 +118,16 @@ int psg_flow_map(psg_ctx *ctx, double *z, double T, double dt0, int time_policy)
  * z1 (may be NULL) receives Phi_T(z).  2D and 3D Navier-Stokes. */
 int psg_tangent(psg_ctx *ctx, const double *z, const double *dz0, double T, double dt0, int time_policy, double *z1,
                 double *dz1);
+/* dz1[i] = dPhi_T(z) dz0[i] for ndir directions about one base (host arrays,
+ * ndir * nf*nkt complex), z1 (may be NULL) = Phi_T(z).  On a one-member 2D
+ * context the base flow records the physical fields of its stages once
+ * (fp32 != 0: as FP32, and the directions then run FP32 passes) and every
+ * direction integrates the perturbation alone against the record; elsewhere,
+ * or when the device lacks room for the record, each direction runs the
+ * combined base + perturbation stepper of psg_tangent.  No counterpart in the
+ * reference (whose Jv are finite differences, main.F90:1084-1100). */
+int psg_tangent_many(psg_ctx *ctx, const double *z, int ndir, const double *dz0, double T, double dt0,
+                     int time_policy, int fp32, double *z1, double *dz1);
 
 /* ---------- Floquet (stability) analysis of a periodic orbit ----------
  * Leading eigenvalues mu of the monodromy operator A q = S_s dPhi_T(z) q
@@ -133,7 +151,9 @@ typedef struct {
   int max_restarts;   /* cycles after the first */
   double tol;         /* relative Ritz residual tolerance */
   int which;          /* PSG_FLOQUET_LR: largest real part; PSG_FLOQUET_LM: largest magnitude */
-  int jv;             /* 0 finite difference (eps), 1 tangent linear, 2 tangent linear in FP32-bulk passes */
+  int jv;             /* 0 finite difference (eps), 1 tangent linear, 2 tangent linear in FP32-bulk
+                         passes; on a one-member 2D context the tangent linear runs against a record
+                         of the orbit's flow taken once (as psg_tangent_many) when the device has room */
   double eps;         /* finite-difference epsilon, 1e-7 */
   int time_policy;    /* PSG_TIME_* */
   double dt0;

@@ -50,7 +50,7 @@ class NewtonOpts(C.Structure):
     _fields_ = [("max_newton", C.c_int), ("max_gmres", C.c_int), ("max_hook", C.c_int), ("netol", C.c_double),
                 ("gmtol", C.c_double), ("eps", C.c_double), ("delta0", C.c_double), ("nshift", C.c_int),
                 ("fixed_period", C.c_int), ("jv", C.c_int), ("time_policy", C.c_int), ("dt0", C.c_double),
-                ("sy", C.c_double), ("stall_count", C.c_int), ("verbose", C.c_int)]
+                ("sy", C.c_double), ("stall_count", C.c_int), ("verbose", C.c_int), ("tl_record", C.c_int)]
 
 
 class NewtonResult(C.Structure):
@@ -192,6 +192,7 @@ def lib():
     L.psg_rec_state.argtypes = [_P, _I, _P]
     L.psg_rec_destroy.argtypes = [_P]
     L.psg_tangent.argtypes = [_P, _P, _P, _D, _D, _I, _P, _P]
+    L.psg_tangent_many.argtypes = [_P, _P, _I, _P, _D, _D, _I, _I, _P, _P]
     _lib = L
     return L
 
@@ -416,6 +417,15 @@ class Stepper:
         d = np.ascontiguousarray(dstate, dtype=np.complex128).reshape(self.nf, self.nkt)
         z1, dz1 = np.zeros_like(s), np.zeros_like(d)
         _check(lib().psg_tangent(self.h, _ptr(s), _ptr(d), T, dt0, policy, _ptr(z1), _ptr(dz1)), "psg_tangent")
+        return z1, dz1
+
+    def tangent_many(self, state, dstates, T, dt0, policy=0, fp32=False):
+        """(Phi_T(state), [dPhi_T(state) d for d in dstates]) about one base (psg_tangent_many)."""
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(self.nf, self.nkt)
+        d = np.ascontiguousarray(dstates, dtype=np.complex128).reshape(-1, self.nf, self.nkt)
+        z1, dz1 = np.zeros_like(s), np.zeros_like(d)
+        _check(lib().psg_tangent_many(self.h, _ptr(s), d.shape[0], _ptr(d), T, dt0, policy, int(bool(fp32)), _ptr(z1),
+                                      _ptr(dz1)), "psg_tangent_many")
         return z1, dz1
 
     def tangent_steps(self, state, dstate, nsteps):

@@ -9,6 +9,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <math.h>
+#include <algorithm>
 #include <mutex>
 #include "psg_launch.hpp"
 #include "../../include/psg_host.h"
@@ -271,7 +272,8 @@ static void free_ctx(Ctx *c) {
     if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
   void *ptrs[] = {c->state, c->rh, c->stage, c->tmp, c->st32, c->stw, c->wnuzn, c->wnu1, c->wmask, c->nuzn, c->nu1, c->nu, c->mask, c->tl_state, c->tl_stage,
                   c->tl_rh, c->tl_bufA, c->tl_bufB, c->tl_bufC, c->bufA, c->bufB, c->bufC,
-                  c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred, c->io, c->dtm};
+                  c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1], c->tw32[2], c->dred, c->io, c->dtm,
+                  c->rec, c->rec_cnt};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto e : c->evs) hipEventDestroy(e);
@@ -302,6 +304,8 @@ int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out) {
 static int run_eager(Ctx *c, int kind, int n, bool first) {
   if (n <= 0) return 0;
   if (kind == 1) return c->g.dim == 2 ? ns2d_tl_run(c, n, first) : ns3d_tl_run(c, n, first);
+  if (kind == 2) return ns2d_rec_run(c, n, first);
+  if (kind == 3) return ns2d_tlrec_run(c, n, first);
   if (c->g.dim == 2) return ns2d_run(c, n, first);
   return c->cfg.flavour == PSG_BOUS3D ? bous_run(c, n, first) : ns3d_run(c, n, first);
 }
@@ -426,13 +430,106 @@ int tl_flow_map(Ctx *c, const cpx<double> *z0, const cpx<double> *dz0, cpx<doubl
       if (c->cfg.dt != dt0 && (r = psg_set_dt(static_cast<psg_ctx *>(c), dt0))) return r;
       if ((r = run(nstop))) return r;
     }
-    const double last = T - nstop * dt0;
+    const double last = last_step(T, dt0, nstop);
     if (last != 0.0) {
       if ((r = psg_set_dt(static_cast<psg_ctx *>(c), last))) return r;
       if ((r = run(1))) return r;
     }
   }
   if (z1) PSG_HIP(hipMemcpyAsync(z1, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
+  PSG_HIP(hipMemcpyAsync(dz1, c->tl_state, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+// ------------------------------------------------------------------ recorded base
+// steps of a flow over T under the Newton time policy
+static long flow_steps(double T, double dt0, int policy) {
+  long n = (long)(T / dt0);
+  if (policy == 0) return n < 1 ? 1 : n;
+  return n + (last_step(T, dt0, n) != 0.0 ? 1 : 0);
+}
+
+bool rec_usable(Ctx *c, double T, double dt0, int policy, int rp) {
+  const Grid &g = c->g;
+  if (g.dim != 2 || c->batch != 1 || g.ny >= 2048 || c->dtm_on || (rp != 4 && rp != 8)) return false;
+  if (c->cfg.precision == PSG_FP32BULK && rp != 4) return false;
+  if (!(T > 0.0) || !(dt0 > 0.0)) return false;
+  const long slots = 4 * flow_steps(T, dt0, policy);
+  if (c->rec && c->rec_p == rp && c->rec_slots >= slots) return true;
+  if (!c->rec_cnt && hipMalloc(&c->rec_cnt, sizeof(int)) != hipSuccess) {
+    c->rec_cnt = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  // room for a quarter more steps, so that small period updates keep the buffer
+  const long want = slots + slots / 4;
+  const size_t per = (size_t)3 * g.ny * g.nx * rp;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
+  if (c->rec) (void)hipFree(c->rec);
+  c->rec = nullptr;
+  c->rec_slots = 0;
+  c->graph_gen++;   // captured launches hold the record's address
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) { (void)hipGetLastError(); return false; }
+  // leave a tenth of the device (and at least 2 GiB) to everything else
+  const size_t reserve = std::max<size_t>(tot / 10, (size_t)2 << 30);
+  if (fr < reserve || (size_t)want * per > fr - reserve) return false;
+  if (hipMalloc(&c->rec, (size_t)want * per) != hipSuccess) {
+    c->rec = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  c->rec_slots = want;
+  c->rec_p = rp;
+  return true;
+}
+
+// c->state = in, the time-policy steps of `kind`, out = c->state (tl: on tl_state)
+static int policy_run(Ctx *c, int kind, double T, double dt0, int policy) {
+  int nstop = (int)(T / dt0), r = 0;
+  if (policy == 0) {
+    if (nstop < 1) nstop = 1;
+    const double dt = T / (double)nstop;
+    if (c->cfg.dt != dt && (r = psg_set_dt(static_cast<psg_ctx *>(c), dt))) return r;
+    return run_steps(c, kind, nstop);
+  }
+  if (nstop > 0) {
+    if (c->cfg.dt != dt0 && (r = psg_set_dt(static_cast<psg_ctx *>(c), dt0))) return r;
+    if ((r = run_steps(c, kind, nstop))) return r;
+  }
+  const double last = last_step(T, dt0, nstop);
+  if (last != 0.0) {
+    if ((r = psg_set_dt(static_cast<psg_ctx *>(c), last))) return r;
+    if ((r = run_steps(c, kind, 1))) return r;
+  }
+  return 0;
+}
+
+int ns2d_rec_reset(Ctx *c);
+
+int rec_flow(Ctx *c, const cpx<double> *in, cpx<double> *out, double T, double dt0, int policy) {
+  if (!c->rec || (long)4 * flow_steps(T, dt0, policy) > c->rec_slots) {
+    set_error("recorded base: the record holds fewer steps than the flow");
+    return PSG_EINVAL;
+  }
+  const size_t bytes = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  PSG_HIP(hipMemcpyAsync(c->state, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (int r = ns2d_rec_reset(c)) return r;
+  if (int r = policy_run(c, 2, T, dt0, policy)) return r;
+  PSG_HIP(hipMemcpyAsync(out, c->state, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+int tl_replay(Ctx *c, const cpx<double> *dz0, cpx<double> *dz1, double T, double dt0, int policy) {
+  if (int r = tl_alloc(c)) return r;
+  if (!c->rec || (long)4 * flow_steps(T, dt0, policy) > c->rec_slots) {
+    set_error("recorded tangent linear: the record holds fewer steps than the flow");
+    return PSG_EINVAL;
+  }
+  const size_t bytes = sizeof(cpx<double>) * c->nf * c->g.nkt;
+  PSG_HIP(hipMemcpyAsync(c->tl_state, dz0, bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (int r = ns2d_rec_reset(c)) return r;
+  if (int r = policy_run(c, 3, T, dt0, policy)) return r;
   PSG_HIP(hipMemcpyAsync(dz1, c->tl_state, bytes, hipMemcpyDeviceToDevice, c->stream));
   return 0;
 }

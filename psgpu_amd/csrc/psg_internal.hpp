@@ -83,12 +83,22 @@ struct Ctx {
   cpx<double> *tl_state = nullptr, *tl_stage = nullptr, *tl_rh = nullptr;
   bool tl_fp32 = false;             // tangent-linear passes in FP32-bulk in an FP64 context
   void *tl_bufA = nullptr, *tl_bufB = nullptr, *tl_bufC = nullptr;
-  // captured step graphs: [0] plain stepper, [1] tangent linear; rebuilt when
-  // graph_gen changes (dt, variants, graph length)
+  // 2D recorded base trajectory: the physical (w, u, v) of every RK stage of
+  // one base flow, [slot = 4 step + stage][field][y][x] in rec_p bytes per
+  // value (8, or 4 for FP32 tangent-linear passes); rec_cnt is the device
+  // step counter the passes index it by.  A tangent-linear flow that replays
+  // it transforms only the perturbation.
+  void *rec = nullptr;
+  long rec_slots = 0;
+  int rec_p = 0;
+  int *rec_cnt = nullptr;
+  // captured step graphs: [0] plain stepper, [1] tangent linear, [2] plain
+  // stepper recording the base, [3] tangent linear replaying it; rebuilt when
+  // graph_gen changes (dt, variants, graph length, record buffer)
   int graph_steps = 16;            // steps per captured graph, 0 = eager launches only
   unsigned graph_gen = 1;
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-  unsigned graph_built[2] = {0, 0};
+  hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};
+  unsigned graph_built[4] = {0, 0, 0, 0};
   bool graph_failed = false;       // a capture failed once: eager from then on
   // kernel shape variants (psg_autotune / psg_set_variant)
   int xvar = 0;    // Navier-Stokes x-pass: 0..PSG_XPASS_VARIANTS-1
@@ -111,6 +121,15 @@ struct Ctx {
   long prof_cur_bytes = 0;
   int prof_cur_e0 = -1;
 };
+
+// The shortened last step of the 3D time policy, T - INT(T/dt0) dt0
+// (3D_NavierStokes/src_gmres/main.F90:771-788).  When T/dt0 rounds up to an
+// integer it comes out a hair below zero; the reference then takes a step of
+// that sub-ulp negative length, here it is skipped.
+inline double last_step(double T, double dt0, long nstop) {
+  const double l = T - (double)nstop * dt0;
+  return l > 0.0 ? l : 0.0;
+}
 
 // z planes per slab of passes 2-4 (the whole grid when slabs are off)
 inline int slab_planes(const Ctx *c) {
@@ -157,9 +176,19 @@ int bous_physical_state(Ctx *c, const cpx<double> *st, double *out_dev);
 int timederiv_dev(Ctx *c, const cpx<double> *in, cpx<double> *out);
 int ns2d_tl_run(Ctx *c, int nsteps, bool first);
 int ns3d_tl_run(Ctx *c, int nsteps, bool first);
-// nsteps of the plain (kind 0) or tangent-linear (kind 1) stepper: an eager
-// first step, then replays of a captured graph of graph_steps steps
+int ns2d_rec_run(Ctx *c, int nsteps, bool first);
+int ns2d_tlrec_run(Ctx *c, int nsteps, bool first);
+// nsteps of the plain (kind 0), tangent-linear (kind 1), base-recording (kind 2)
+// or recorded tangent-linear (kind 3) stepper: an eager first step, then
+// replays of a captured graph of graph_steps steps
 int run_steps(Ctx *c, int kind, int nsteps);
+// recorded-base flows (2D, one member): rec_flow = Phi_T(in) recording its
+// stages (rp: 8 or 4 bytes per recorded value); tl_replay = dPhi_T dz0 about
+// the recorded base, with the same T, dt0 and policy.  rec_usable says
+// whether a record of T/dt0 steps fits (allocating it if need be).
+bool rec_usable(Ctx *c, double T, double dt0, int policy, int rp);
+int rec_flow(Ctx *c, const cpx<double> *in, cpx<double> *out, double T, double dt0, int policy);
+int tl_replay(Ctx *c, const cpx<double> *dz0, cpx<double> *dz1, double T, double dt0, int policy);
 int ns3d_step_rec(Ctx *c, int umask, cpx<double> *wn, cpx<double> *wstar);
 int ns3d_adjoint(Ctx *c, const cpx<double> *z0, const cpx<double> *lamT, int nsteps, int ckpt, cpx<double> *zT,
                  cpx<double> *lam0);

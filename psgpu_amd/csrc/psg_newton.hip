@@ -165,7 +165,7 @@ static int flow(Ctx *c, const cpx<double> *in, cpx<double> *out, double T, doubl
       if (c->cfg.dt != dt0 && (r = psg_set_dt(static_cast<psg_ctx *>(c), dt0))) return r;
       if ((r = psg_run(static_cast<psg_ctx *>(c), nstop))) return r;
     }
-    const double last = T - nstop * dt0;
+    const double last = last_step(T, dt0, nstop);
     if (last != 0.0) {
       if ((r = psg_set_dt(static_cast<psg_ctx *>(c), last))) return r;
       if ((r = psg_run(static_cast<psg_ctx *>(c), 1))) return r;
@@ -472,6 +472,7 @@ void psg_newton_defaults(int flavour, psg_newton_opts *o) {
   o->dt0 = 0.005;
   o->sy = 0.0;
   o->stall_count = is3 ? 5 : 3;
+  o->tl_record = 1;
 }
 
 int psg_flow_map(psg_ctx *c, double *z, double T, double dt0, int policy) {
@@ -508,6 +509,34 @@ int psg_tangent(psg_ctx *c, const double *z, const double *dz0, double T, double
   return r;
 }
 
+int psg_tangent_many(psg_ctx *c, const double *z, int ndir, const double *dz0, double T, double dt0, int policy,
+                     int fp32, double *z1, double *dz1) {
+  if (!c || !z || !dz0 || !dz1 || ndir < 1) { set_error("null argument"); return PSG_EINVAL; }
+  if (c->g.dim == 3 && c->cfg.flavour != PSG_NS3D) { set_error("tangent linear: 2D and 3D Navier-Stokes only"); return PSG_EINVAL; }
+  const long nn = c->nf * c->g.nkt;
+  const size_t bytes = sizeof(cpx<double>) * nn;
+  cpx<double> *d = nullptr;
+  PSG_HIP(hipMalloc(&d, 4 * bytes));
+  cpx<double> *a = d, *b = d + nn, *o1 = b + nn, *o2 = o1 + nn;
+  const int rp = (fp32 || c->cfg.precision == PSG_FP32BULK) ? 4 : 8;
+  const bool old = c->tl_fp32;
+  if (fp32 && !old) { c->tl_fp32 = true; c->graph_gen++; }
+  int r = hipMemcpyAsync(a, z, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess ? 0 : PSG_EHIP;
+  const bool rec = !r && rec_usable(c, T, dt0, policy, rp);
+  if (!r) r = rec ? rec_flow(c, a, o1, T, dt0, policy) : flow(c, a, o1, T, dt0, policy);
+  if (!r && z1 && hipMemcpyAsync(z1, o1, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) r = PSG_EHIP;
+  for (int i = 0; i < ndir && !r; i++) {
+    if (hipMemcpyAsync(b, dz0 + 2 * nn * i, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) { r = PSG_EHIP; break; }
+    r = rec ? tl_replay(c, b, o2, T, dt0, policy) : tl_flow_map(c, a, b, o1, o2, T, dt0, policy, false);
+    if (!r && hipMemcpyAsync(dz1 + 2 * nn * i, o2, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) r = PSG_EHIP;
+  }
+  if (r == PSG_EHIP) set_error("psg_tangent_many: copy");
+  (void)hipStreamSynchronize(c->stream);
+  if (c->tl_fp32 != old) { c->tl_fp32 = old; c->graph_gen++; }
+  (void)hipFree(d);
+  return r;
+}
+
 // jv = 2: the tangent-linear flows of the solve run their passes in FP32-bulk
 // (the residual flow stays in the context's precision); captured step graphs
 // are rebuilt on each switch
@@ -537,6 +566,7 @@ struct FlowReq {
   cpx<double> *out = nullptr;
   double T = 0.0, dt0 = 0.0;
   int policy = 0, rc = 0;
+  bool rec = false;   // record the base for the tangent-linear replays (one-guess solves only)
 };
 struct NTask {
   struct promise_type {
@@ -629,6 +659,14 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
     }                        \
   } while (0)
 #define NFLOW(in, out, TT) NTRY((co_await FlowAwait{FlowReq{(in), (out), (TT), dt0, o->time_policy, 0}}))
+#define NFLOWR(in, out, TT) NTRY((co_await FlowAwait{FlowReq{(in), (out), (TT), dt0, o->time_policy, 0, true}}))
+  // tangent-linear Jv against a recorded base (2D, one member): the record is
+  // taken by the residual flows of the iterate and of the hookstep candidates,
+  // and is valid while it holds the flow of the current (ZNEWT, T, dt0)
+  const int rec_rp = (o->jv == 2 || c->cfg.precision == PSG_FP32BULK) ? 4 : 8;
+  const bool try_rec = o->jv >= 1 && o->tl_record && c->g.dim == 2 && c->batch == 1;
+  bool rec_valid = false;
+  auto can_rec = [&](double TT) { return try_rec && rec_usable(c, TT, dt0, o->time_policy, rec_rp); };
   auto shift_of = [&](const cpx<double> *in, cpx<double> *out, double sx, double sz) {
     V.shift(in, out, sx, sy, ns > 1 ? sz : 0.0);
   };
@@ -653,11 +691,22 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
     V.copy(work, Zr);
   }
   NTRY(at_v2(v2N));
-  NFLOW(ZNEWT, ZN, T);
+  if (can_rec(T)) {
+    NFLOWR(ZNEWT, ZN, T);
+    rec_valid = true;
+  } else {
+    NFLOW(ZNEWT, ZN, T);
+  }
   phi++;
   V.copy(ZN, ZNN);
   shift_of(ZN, ZNSH, sh[0], sh[1]);
-  double NORMZ = norm(ZN);
+  // the Boussinesq solver normalises by ||(Phi_T(z), z)|| (3D_Boussinesq/src/src_gmres/main.F90:775,
+  // 1698); the Navier-Stokes solvers by ||Phi_T(z)|| (3D main.F90:813,1755; 2D main.F90:734,1696)
+  const bool two_norm = c->cfg.flavour == PSG_BOUS3D;
+  auto normz_of = [&](const cpx<double> *zn, const cpx<double> *z0) {
+    return two_norm ? sqrt(V.dot(zn, zn) + V.dot(z0, z0)) : norm(zn);
+  };
+  double NORMZ = normz_of(ZN, ZNEWT);
   V.copy(ZNSH, FN);
   V.axpby(-1.0, ZNEWT, 1.0, FN);
   double NORMFN = norm(FN);
@@ -693,6 +742,12 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
       V.axpby(0.0, dFdv, 0.5 / hv, dFdv);
       NTRY(at_v2(v2N));
     }
+    if (try_rec && !rec_valid && can_rec(T)) {
+      // the accepted step was not the last recorded flow: record the iterate's
+      NFLOWR(ZNEWT, work, T);
+      phi++;
+      rec_valid = true;
+    }
     if (!o->fixed_period) {
       NTRY(timederiv_dev(c, ZNSH, dZNdt));
       NTRY(timederiv_dev(c, ZNEWT, dzdt));
@@ -718,7 +773,9 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
     auto jv_post = [&](const cpx<double> *dz, const double *dsc, cpx<double> *out, double *vn) -> int {
       int r;
       if (o->jv >= 1) {
-        if ((r = tl_flow_map(c, ZNEWT, dz, work, out, T, dt0, o->time_policy, true))) return r;
+        r = rec_valid ? tl_replay(c, dz, out, T, dt0, o->time_policy)
+                      : tl_flow_map(c, ZNEWT, dz, work, out, T, dt0, o->time_policy, true);
+        if (r) return r;
         shift_of(out, out, sh[0], sh[1]);   // S dPhi dz
         if (alc && dsc[3] != 0.0) V.axpby(dsc[3], dFdv, 1.0, out);
       } else {
@@ -804,7 +861,7 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
     if (GMR > o->gmtol && o->max_gmres >= 2000) { status = PSG_NEWTON_GMRES_FAILED; break; }
     // ---------------- hookstep
     int IHOOK = 0;
-    bool halve = false, dbl = false, hook_failed = false;
+    bool halve = false, dbl = false, hook_failed = false, rec_last = false;
     double NORMFNplus = 0, NORMFNpred = 0, NORMFNplus2 = 0, Tplus = T, Tplus2 = T, Splus[2] = {sh[0], sh[1]},
            Splus2[2] = {sh[0], sh[1]}, v2plus = v2N, v2plus2 = v2N;
     for (;;) {
@@ -824,9 +881,20 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
       }
       if (o->fixed_period) dsc[0] = 0.0;
       if (!alc) dsc[3] = 0.0;
-      // predicted residual F + J DIFF
-      double vn[NQ];
-      APPLY_J(DIFF, dsc, v, vn);
+      // predicted residual F + J DIFF.  A tangent-linear J is linear in DIFF = Q_k Y,
+      // so J DIFF = Q_{k+1} (H_k Y) by the Arnoldi relation; the finite-difference
+      // Jv is integrated as the reference does
+      if (o->jv >= 1) {
+        V.axpby(0.0, v, 0.0, v);
+        for (int jj = 0; jj <= k; jj++) {
+          double cj = 0.0;
+          for (int i = (jj > 0 ? jj - 1 : 0); i < k; i++) cj += h(jj, i) * Y[i];
+          if (cj != 0.0) V.axpby(cj, q(jj), 1.0, v);
+        }
+      } else {
+        double vn[NQ];
+        APPLY_J(DIFF, dsc, v, vn);
+      }
       V.axpby(1.0, FN, 1.0, v);
       NORMFNpred = norm(v);
       // const = 0 in the reference (main.F90:135), so the bound is |b| itself (main.F90:1435-1442)
@@ -837,8 +905,24 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
       Tplus = T + dsc[0];
       for (int s = 0; s < ns; s++) Splus[s] = sh[s] + dsc[1 + s];
       v2plus = v2N + dsc[3];
+      if (!(Tplus > 0.0)) {
+        // a step to a non-positive period has no flow to evaluate: the trust region
+        // is halved as for a candidate whose integration blew up (the reference would
+        // step a negative INT(T/dt) count, i.e. only its shortened last step)
+        Delta /= 2.0;
+        halve = true;
+        continue;
+      }
       NTRY(at_v2(v2plus));
-      NFLOW(Zplus, ZN, Tplus);
+      // the Jv of this Newton step are done: a candidate may take the record
+      rec_valid = false;
+      if (v2plus == v2N && can_rec(Tplus)) {
+        NFLOWR(Zplus, ZN, Tplus);
+        rec_last = true;
+      } else {
+        NFLOW(Zplus, ZN, Tplus);
+        rec_last = false;
+      }
       NTRY(at_v2(v2N));
       phi++;
       shift_of(ZN, ZNSH, Splus[0], Splus[1]);
@@ -864,6 +948,7 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
         if (dbl) {
           V.copy(Zplus2, Zplus); Tplus = Tplus2; Splus[0] = Splus2[0]; Splus[1] = Splus2[1]; v2plus = v2plus2;
           V.copy(ZN2, ZN); V.copy(FNplus2, FNplus); NORMFNplus = NORMFNplus2;
+          rec_last = false;
           Delta /= 2.0;
           break;
         }
@@ -874,6 +959,7 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
         if (dbl && NORMFNplus > NORMFNplus2) {
           V.copy(Zplus2, Zplus); Tplus = Tplus2; Splus[0] = Splus2[0]; Splus[1] = Splus2[1]; v2plus = v2plus2;
           V.copy(ZN2, ZN); V.copy(FNplus2, FNplus); NORMFNplus = NORMFNplus2;
+          rec_last = false;
           Delta /= 2.0;
           break;
         }
@@ -893,11 +979,13 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
       // time stepping blew up: halve dt and retry this Newton step (main.F90:1636-1665)
       if (dt0 < 0.1 * dt0_orig) { status = PSG_NEWTON_TSTEP_FAILED; break; }
       dt0 /= 2.0;
+      rec_valid = false;
       shift_of(ZNN, ZNSH, sh[0], sh[1]);
       continue;
     }
     if (hook_failed) { status = PSG_NEWTON_HSTEP_FAILED; break; }
-    // accept
+    // accept (the record is the accepted candidate's when it was the last flowed)
+    rec_valid = rec_last;
     V.copy(ZN, ZNN);
     V.copy(Zplus, ZNEWT);
     T = Tplus;
@@ -906,7 +994,7 @@ NTask newton_core(psg_ctx *c, double *zh, double *Tio, double *sxio, double *szi
     v2N = v2plus;
     NTRY(at_v2(v2N));
     shift_of(ZNN, ZNSH, sh[0], sh[1]);
-    NORMZ = norm(ZNN);
+    NORMZ = normz_of(ZNN, ZNEWT);
     V.copy(FNplus, FN);
     NORMFN = NORMFNplus;
     NER = NORMFN / NORMZ;
@@ -953,7 +1041,8 @@ static int run_direct(Ctx *c, NTask &t) {
   h.resume();
   while (!h.done()) {
     FlowReq &rq = h.promise().req;
-    rq.rc = flow(c, rq.in, rq.out, rq.T, rq.dt0, rq.policy);
+    rq.rc = rq.rec ? rec_flow(c, rq.in, rq.out, rq.T, rq.dt0, rq.policy)
+                   : flow(c, rq.in, rq.out, rq.T, rq.dt0, rq.policy);
     h.resume();
   }
   return h.promise().result;
@@ -1107,14 +1196,22 @@ int psg_floquet(psg_ctx *c, const double *zh, double T, double sx, double sz, co
   };
   if (hipMemcpyAsync(t1, zh, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) { set_error("copy"); return done(PSG_EHIP); }
   V.copy(t1, Z0);
+  // with the tangent linear on a one-member 2D context the orbit's flow is
+  // recorded once and every matvec integrates the perturbation alone
+  const int rec_rp = (o->jv == 2 || c->cfg.precision == PSG_FP32BULK) ? 4 : 8;
+  const bool rec = o->jv >= 1 && rec_usable(c, T, o->dt0, o->time_policy, rec_rp);
   if (o->jv == 0) {
     if ((rc = flow(c, Z0, ZN, T, o->dt0, o->time_policy))) return done(rc);
+  } else if (rec) {
+    if ((rc = rec_flow(c, Z0, ZN, T, o->dt0, o->time_policy))) return done(rc);
   }
   // A q = S_s dPhi_T(z) q
   auto apply_A = [&](const cpx<double> *in, cpx<double> *out) -> int {
     int r;
     if (o->jv >= 1) {
-      if ((r = tl_flow_map(c, Z0, in, t2, out, T, o->dt0, o->time_policy, true))) return r;
+      r = rec ? tl_replay(c, in, out, T, o->dt0, o->time_policy)
+              : tl_flow_map(c, Z0, in, t2, out, T, o->dt0, o->time_policy, true);
+      if (r) return r;
     } else {
       V.copy(Z0, t2);
       V.axpby(o->eps, in, 1.0, t2);

@@ -376,6 +376,197 @@ int ns2d_tl_run(Ctx *c, int nsteps, bool first) {
                                                           : run2tl_t<double>(c, nsteps, first);
 }
 
+// ------------------------------------------------------------------ recorded base
+// A base flow whose x-pass uses Prod2DRec stores the physical (w, u, v) of
+// every stage at slot 4*step + stage, step being the device counter that the
+// last stage's y-forward advances (StepEnd).  The perturbation-only flow
+// (Prod2DTLRec + Upd2DPert) reads them back instead of transforming the base
+// again, so one tangent-linear step costs the transforms of a plain step.
+// Past the record's last slot nothing is stored and zeros are read; the host
+// sizes the record so that this never happens.
+__device__ __forceinline__ long rec_offset(const int *cnt, int stage, long slots, long plane) {
+  const long slot = 4L * (long)(*cnt) + stage;
+  return slot < slots ? slot * 3 * plane : -1;
+}
+
+template <typename P, typename RP> struct Prod2DRec {
+  static constexpr bool ROW = true;
+  RP *rec;
+  const int *cnt;
+  int stage;
+  long slots;
+  int nx, ny;
+  struct Bound {
+    RP *p;
+    long plane;
+    bool on;
+    // in: w, u, v; out: (v w, u w) as Prod2D
+    __device__ __forceinline__ void operator()(int n, const P *in, P *out) const {
+      if (on) {
+        p[n] = (RP)in[0];
+        p[plane + n] = (RP)in[1];
+        p[2 * plane + n] = (RP)in[2];
+      }
+      out[0] = in[2] * in[0];
+      out[1] = in[1] * in[0];
+    }
+  };
+  // every team forms the products; team 0 alone stores the row
+  __device__ __forceinline__ Bound bind(unsigned, unsigned row, int team) const {
+    const long plane = (long)ny * nx;
+    const long o = rec_offset(cnt, stage, slots, plane);
+    return Bound{o >= 0 ? rec + o + (long)row * nx : rec, plane, o >= 0 && team == 0};
+  }
+};
+
+template <typename P, typename RP> struct Prod2DTLRec {
+  static constexpr bool ROW = true;
+  const RP *rec;
+  const int *cnt;
+  int stage;
+  long slots;
+  int nx, ny;
+  struct Bound {
+    const RP *p;
+    long plane;
+    bool on;
+    // in: dw, du, dv; out: (dv w + v dw, du w + u dw) about the recorded (w, u, v)
+    __device__ __forceinline__ void operator()(int n, const P *in, P *out) const {
+      P w = 0, u = 0, v = 0;
+      if (on) {
+        w = (P)p[n];
+        u = (P)p[plane + n];
+        v = (P)p[2 * plane + n];
+      }
+      out[0] = in[2] * w + v * in[0];
+      out[1] = in[1] * w + u * in[0];
+    }
+  };
+  // only team 0's output pair is stored: the other team reads nothing
+  __device__ __forceinline__ Bound bind(unsigned, unsigned row, int team) const {
+    const long plane = (long)ny * nx;
+    const long o = rec_offset(cnt, stage, slots, plane);
+    return Bound{o >= 0 ? rec + o + (long)row * nx : rec, plane, o >= 0 && team == 0};
+  }
+};
+
+// the perturbation half of the tangent-linear update
+template <int STAGE> struct Upd2DPert {
+  Upd2DTL<STAGE> t;
+  __device__ __forceinline__ void operator()(int m, long ik, int, double kx, double ky, cpx<double> (&N)[2]) const {
+    t.pert(m, ik, kx, ky, N);
+  }
+};
+
+// the last stage of a recording or replaying step advances the step counter
+template <class U> struct StepEnd : U {
+  static constexpr bool ONCE = true;
+  int *cnt;
+  __device__ __forceinline__ void once() const { *cnt = *cnt + 1; }
+};
+
+__global__ void k2_zero_int(int *p) { *p = 0; }
+
+// the split-row x-pass alone (the record is indexed by row)
+template <typename P, int FI, int FO, class PROD>
+static int launch_xpass2_split(Ctx *c, const cpx<P> *in, cpx<P> *out, PROD prod) {
+  const Grid &g = c->g;
+  PSG_DISPATCH_N(g.nx, {
+    if (g.iktx != xband<N>()) {
+      set_error("x-pass: IKTX %d is not NX/3+1", g.iktx);
+      return PSG_EINVAL;
+    }
+    if (g.ny >= 2048) {
+      set_error("recorded base: %d rows exceed the split-row x-pass", g.ny);
+      return PSG_EINVAL;
+    }
+    constexpr int NT = xsplit_teams<FI, FO>() * Row<N>::TPX;
+    const size_t lds = xsplit_lds<P, N, FI, FO>();
+    auto k = k_xpass_split<P, N, FI, FO, PROD>;
+    PSG_HIP(lds_attr(k, lds));
+    hipLaunchKernelGGL(k, dim3((unsigned)g.ny), dim3(NT), lds, c->stream, in, out, (long)g.ny, (long)g.ny, g,
+                       twp<P>(c, 0), prod);
+  });
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename P, typename RP> static int run2rec_t(Ctx *c, int nsteps, bool first) {
+  const Grid &g = c->g;
+  if (c->batch != 1 || !c->rec || !c->rec_cnt) {
+    set_error("recorded base: one member and an allocated record only");
+    return PSG_EINVAL;
+  }
+  if (first) {
+    hipLaunchKernelGGL(k2_setz0, dim3((unsigned)((g.nkt + 255) / 256 < 4096 ? (g.nkt + 255) / 256 : 4096)),
+                       dim3(256), 0, c->stream, c->state, c->stage, c->mask, g.nkt, g.nkt);
+    PSG_HIP(hipGetLastError());
+  }
+  const double famp = c->cfg.force == PSG_FORCE_NONE ? 0.0 : c->cfg.ampfor;
+  cpx<P> *B = (cpx<P> *)c->bufB, *Cb = (cpx<P> *)c->bufC;
+  for (int nt = 0; nt < nsteps; nt++) {
+#define PSG_REC_STAGE(S, UPD)                                                                                \
+    {                                                                                                        \
+      Upd2D<S> u{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, famp, c->cfg.kf, nullptr}; \
+      Prod2DRec<P, RP> pr{(RP *)c->rec, c->rec_cnt, S, c->rec_slots, g.nx, g.ny};                            \
+      PSG_TRY2(launch_yinv2<P>(c, c->stage, B, 1));                                                          \
+      PSG_TRY2((launch_xpass2_split<P, 3, 2>(c, B, Cb, pr)));                                               \
+      PSG_TRY2(launch_yfwd2<P>(c, Cb, UPD, 1));                                                              \
+    }
+    PSG_REC_STAGE(0, u) PSG_REC_STAGE(1, u) PSG_REC_STAGE(2, u)
+    PSG_REC_STAGE(3, (StepEnd<Upd2D<3>>{u, c->rec_cnt}))
+#undef PSG_REC_STAGE
+  }
+  return 0;
+}
+
+template <typename P> static int run2tlrec_t(Ctx *c, int nsteps, bool first) {
+  const Grid &g = c->g;
+  if (c->batch != 1 || !c->rec || !c->rec_cnt || c->rec_p != (int)sizeof(P)) {
+    set_error("recorded tangent linear: one member and a record of the pass precision only");
+    return PSG_EINVAL;
+  }
+  if (first) {
+    hipLaunchKernelGGL(k2_setz0, dim3((unsigned)((g.nkt + 255) / 256 < 4096 ? (g.nkt + 255) / 256 : 4096)),
+                       dim3(256), 0, c->stream, c->tl_state, c->tl_stage, c->mask, g.nkt, g.nkt);
+    PSG_HIP(hipGetLastError());
+  }
+  cpx<P> *B = (cpx<P> *)c->tl_bufB, *Cb = (cpx<P> *)c->tl_bufC;
+  for (int nt = 0; nt < nsteps; nt++) {
+#define PSG_TLR_STAGE(S, UPD)                                                                                  \
+    {                                                                                                          \
+      Upd2DPert<S> u{{{c->state, c->stage, c->rh, g.nkt, c->mask, c->cfg.dt, c->cfg.v2, 0.0, c->cfg.kf, nullptr}, \
+                      c->tl_state, c->tl_stage, c->tl_rh}};                                                    \
+      Prod2DTLRec<P, P> pr{(const P *)c->rec, c->rec_cnt, S, c->rec_slots, g.nx, g.ny};                       \
+      PSG_TRY2(launch_yinv2<P>(c, c->tl_stage, B, 1));                                                         \
+      PSG_TRY2((launch_xpass2_split<P, 3, 2>(c, B, Cb, pr)));                                                 \
+      PSG_TRY2(launch_yfwd2<P>(c, Cb, UPD, 1));                                                                \
+    }
+    PSG_TLR_STAGE(0, u) PSG_TLR_STAGE(1, u) PSG_TLR_STAGE(2, u)
+    PSG_TLR_STAGE(3, (StepEnd<Upd2DPert<3>>{u, c->rec_cnt}))
+#undef PSG_TLR_STAGE
+  }
+  return 0;
+}
+
+int ns2d_rec_run(Ctx *c, int nsteps, bool first) {
+  if (c->cfg.precision == PSG_FP32BULK) {
+    if (c->rec_p != 4) { set_error("recorded base: an FP32-bulk context records FP32"); return PSG_EINVAL; }
+    return run2rec_t<float, float>(c, nsteps, first);
+  }
+  return c->rec_p == 4 ? run2rec_t<double, float>(c, nsteps, first) : run2rec_t<double, double>(c, nsteps, first);
+}
+
+int ns2d_tlrec_run(Ctx *c, int nsteps, bool first) {
+  return c->rec_p == 4 ? run2tlrec_t<float>(c, nsteps, first) : run2tlrec_t<double>(c, nsteps, first);
+}
+
+int ns2d_rec_reset(Ctx *c) {
+  hipLaunchKernelGGL(k2_zero_int, dim3(1), dim3(1), 0, c->stream, c->rec_cnt);
+  PSG_HIP(hipGetLastError());
+  return 0;
+}
+
 int ns2d_init(Ctx *c) {
   const Grid &g = c->g;
   size_t b = c->cfg.precision == PSG_FP32BULK ? sizeof(cpx<float>) : sizeof(cpx<double>);

@@ -20,6 +20,8 @@
 // precision P of the pass buffers (double: plain form, float: FP32-bulk);
 // state, tables, update arithmetic and diagnostics stay FP64.
 #pragma once
+#include <type_traits>
+
 #include "psg_internal.hpp"
 
 namespace psg {
@@ -567,6 +569,16 @@ k_xpass(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow_m, lo
 // team q forms the product at its points and forward-transforms output pair
 // q.  Every team runs both phases (on zeros where it has no pair) so that the
 // synchronisation is the same in every thread.
+// A product that needs the row and column of each point (the recorded-base
+// products of the 2D tangent linear) declares ROW = true and provides
+// bind(member, row, team) returning an object called as (n, in, out).
+template <class T, class = void> struct prod_row : std::false_type {};
+template <class T> struct prod_row<T, std::void_t<decltype(T::ROW)>> : std::bool_constant<T::ROW> {};
+// An update that must act once per launch (the step counter of the recorded
+// base) provides once(), called by one thread of the first tile.
+template <class T, class = void> struct upd_once : std::false_type {};
+template <class T> struct upd_once<T, std::void_t<decltype(T::ONCE)>> : std::bool_constant<T::ONCE> {};
+
 template <int FI, int FO> constexpr int xsplit_teams() { return (FI + 1) / 2 > (FO + 1) / 2 ? (FI + 1) / 2 : (FO + 1) / 2; }
 template <typename P, int N, int FI, int FO>
 constexpr size_t xsplit_lds() {
@@ -609,6 +621,10 @@ k_xpass_split(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow
   __syncthreads();
   const int q = team < PO ? team : PO - 1;
   cpx<P> v[E];
+  auto bound = [&]() {
+    if constexpr (prod_row<PROD>::value) return prod.bind(m, rr, team);
+    else return 0;
+  }();
 #pragma unroll
   for (int mm = 0; mm < E; mm++) {
     const int n = j + TPX * mm;
@@ -617,9 +633,11 @@ k_xpass_split(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, long nrow
     for (int f = 0; f < 2 * PI; f++) pin[f] = phys[f * N + n];
 #pragma unroll
     for (int f = 0; f < 2 * PO; f++) pout[f] = 0;
-    prod(pin, pout);
+    if constexpr (prod_row<PROD>::value) bound(n, pin, pout);
+    else prod(pin, pout);
     v[mm] = cpx<P>{pout[2 * q], pout[2 * q + 1]};
   }
+  (void)bound;
   fft_pencil<P, N, E, false, WAVE, SW>(v, tl, j, 1, 0, tw);
   team_sync<WAVE>();
 #pragma unroll
@@ -768,6 +786,9 @@ k2_yfwd(const cpx<P> *__restrict__ in, long in_ms, Grid g, const cpx<P> *__restr
       cpx<double> t = up(v[mm]);
       res[f][mm] = cpx<double>{t.x * g.inr, t.y * g.inr};
     }
+  }
+  if constexpr (upd_once<UPD>::value) {
+    if (tid.x == 0 && tid.y == 0 && tid.z == 0 && threadIdx.x == 0) upd.once();
   }
   if (!valid) return;
   const double kx = (double)ikx * g.alpha;

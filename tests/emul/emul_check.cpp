@@ -210,6 +210,99 @@ static void newton2d_case(int jv) {
   check(r.status == PSG_NEWTON_CONVERGED, jv ? "newton 2d laminar (TL Jv)" : "newton 2d laminar (FD Jv)", r.ner);
 }
 
+// the tangent linear about a recorded base (psg_tangent_many) against the
+// combined base + perturbation stepper (psg_tangent), FP64 and FP32 records,
+// over both time policies; and Newton with the tangent-linear Jv with and
+// without the record
+static void tangent_rec_case() {
+  const int n = 32, nd = 3;
+  psg_config c = cfg3(PSG_NS2D, n, n, 1, 1.0, PSG_FP64, 1, 1 / 40.0, 0.01, PSG_FORCE_KOLMO);
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, "tangent rec create", 1); return; }
+  or_grid g;
+  or_grid2(&g, n, n, 1.0);
+  V z(2 * g.nkt), d((size_t)2 * g.nkt * nd);
+  or_ic_ns2d_cfg(&g, 3, 1.0, (orc *)z.data());
+  for (int i = 0; i < nd; i++) or_ic_ns2d_cfg(&g, 20 + i, 0.3, (orc *)(d.data() + (size_t)2 * g.nkt * i));
+  for (int policy = 0; policy < 2; policy++) {
+    const double T = policy ? 0.237 : 0.2;
+    V z1(2 * g.nkt), zm(2 * g.nkt), dm(d.size()), d1(2 * g.nkt);
+    double worst = 0, worst32 = 0, wz = 0;
+    int rc = psg_tangent_many(ctx, z.data(), nd, d.data(), T, 0.01, policy, 0, zm.data(), dm.data());
+    for (int i = 0; i < nd && !rc; i++) {
+      V di(d.begin() + (size_t)2 * g.nkt * i, d.begin() + (size_t)2 * g.nkt * (i + 1));
+      rc = psg_tangent(ctx, z.data(), di.data(), T, 0.01, policy, z1.data(), d1.data());
+      V mi(dm.begin() + (size_t)2 * g.nkt * i, dm.begin() + (size_t)2 * g.nkt * (i + 1));
+      worst = fmax(worst, rel(mi, d1));
+      wz = fmax(wz, rel(zm, z1));
+    }
+    V d32(d.size());
+    if (!rc) rc = psg_tangent_many(ctx, z.data(), nd, d.data(), T, 0.01, policy, 1, nullptr, d32.data());
+    worst32 = rel(d32, dm);
+    char nm[96];
+    snprintf(nm, sizeof nm, "tangent about a recorded base, policy %d (fp64)", policy);
+    check(rc == 0 && worst < 1e-12 && wz < 1e-13, nm, worst);
+    snprintf(nm, sizeof nm, "tangent about a recorded base, policy %d (fp32 record)", policy);
+    check(rc == 0 && worst32 < 1e-4 && worst32 > 0, nm, worst32);
+  }
+  psg_destroy(ctx);
+  // Newton: the recorded and the combined tangent linear take the same steps
+  c = cfg3(PSG_NS2D, n, n, 1, 1.0, 0, 1, 1 / 5.0, 0.01, PSG_FORCE_KOLMO);
+  if (psg_create(&c, &ctx)) { check(false, "newton rec create", 1); return; }
+  std::vector<double> kx(g.nkt), ky(g.nkt);
+  std::vector<int> L(g.nkt);
+  or_tables(&g, kx.data(), ky.data(), nullptr, L.data());
+  V z0(2 * g.nkt, 0.0), dd(2 * g.nkt);
+  or_ic_ns2d_cfg(&g, 9, 0.05, (orc *)dd.data());
+  for (long i = 0; i < g.nkt; i++) {
+    if (kx[i] == 0 && ky[i] == 4) z0[2 * i] = -5.0 / 8;
+    z0[2 * i] += dd[2 * i]; z0[2 * i + 1] += dd[2 * i + 1];
+  }
+  psg_newton_result r[2];
+  V zs[2] = {z0, z0};
+  for (int k = 0; k < 2; k++) {
+    psg_newton_opts o;
+    psg_newton_defaults(PSG_NS2D, &o);
+    o.nshift = 0; o.fixed_period = 1; o.jv = 1; o.dt0 = 0.01; o.max_gmres = 12; o.max_newton = 8;
+    o.tl_record = k;
+    double T = 0.2, sx = 0, sz = 0;
+    psg_newton_solve(ctx, zs[k].data(), &T, &sx, &sz, &o, &r[k]);
+  }
+  psg_destroy(ctx);
+  check(r[0].status == PSG_NEWTON_CONVERGED && r[1].status == PSG_NEWTON_CONVERGED &&
+            r[0].newton_iters == r[1].newton_iters && r[0].gmres_iters == r[1].gmres_iters && rel(zs[1], zs[0]) < 1e-12,
+        "newton 2d laminar, TL Jv with and without the record", rel(zs[1], zs[0]));
+}
+
+// Boussinesq Newton with both shifts and the period free, two Newton steps
+// from a developed 16^3 state (status and residual printed)
+static void bous_newton_case(int fixed_period) {
+  const int n = 16;
+  psg_config c = cfg3(PSG_BOUS3D, n, n, n, 1.0, PSG_FP64, 1, 0.02, 0.01, PSG_FORCE_KOLMO);
+  psg_ctx *ctx;
+  if (psg_create(&c, &ctx)) { check(false, "bous newton create", 1); return; }
+  V z = ic3(n, n, n, 1.0, 73, 4);
+  {
+    or_grid g;
+    or_grid3(&g, n, n, n, 1.0);
+    orb_ctx *o = orb_create(n, n, n, 1.0, c.v2, c.sc, c.ri, c.theta, 0.01, c.kf, c.ampfor);
+    orc *w = (orc *)z.data();
+    orb_run(o, w, w + g.nkt, w + 2 * g.nkt, w + 3 * g.nkt, 30);
+    orb_destroy(o);
+  }
+  psg_newton_opts o;
+  psg_newton_defaults(PSG_BOUS3D, &o);
+  o.fixed_period = fixed_period; o.dt0 = 0.01; o.max_gmres = 30; o.max_newton = 2; o.verbose = 2;
+  psg_newton_result r;
+  double T = 0.3, sx = 0.05, sz = -0.05;
+  const int rc = psg_newton_solve(ctx, z.data(), &T, &sx, &sz, &o, &r);
+  if (rc) printf("bous newton error: %s\n", psg_last_error());
+  printf("bous newton: status %d iters %d ner %.6e T %.10f sx %.6f sz %.6f\n", r.status, r.newton_iters, r.ner, T, sx,
+         sz);
+  psg_destroy(ctx);
+  check(rc == 0 && std::isfinite(r.ner), fixed_period ? "bous newton two shifts, fixed period" : "bous newton two shifts and period", r.ner);
+}
+
 // three guesses of different periods on two lanes against one-guess solves
 static void newton2d_batch_case() {
   const int n = 32, ng = 3;
@@ -462,10 +555,16 @@ int main(int argc, char **argv) {
     printf("%d passed, %d failed\n", g_pass, g_fail);
     return g_fail ? 1 : 0;
   }
+  if (argc > 1 && strcmp(argv[1], "bousnewton") == 0) {
+    bous_newton_case(0);
+    printf("%d passed, %d failed\n", g_pass, g_fail);
+    return g_fail ? 1 : 0;
+  }
   if (argc > 1 && strcmp(argv[1], "newton") == 0) {
     newton2d_case(0);
     newton2d_case(1);
     newton2d_batch_case();
+    tangent_rec_case();
     printf("%d passed, %d failed\n", g_pass, g_fail);
     return g_fail ? 1 : 0;
   }
@@ -537,6 +636,7 @@ int main(int argc, char **argv) {
   newton2d_case(0);
   newton2d_case(1);
   newton2d_batch_case();
+  tangent_rec_case();
   rec_case("rec ns3d 32^3 W 8 shifts 30x2x2 ring 6", PSG_NS3D, 32, PSG_REC_NS, 8, 30, 2, 2, 0.0, 6, 14);
   rec_case("rec bous 32^3 W 8 shifts 64 ring 8 (wraps)", PSG_BOUS3D, 32, PSG_REC_BOUS, 8, 64, 1, 1, 0.7, 8, 18);
   rec_case("rec ns2d 128^2 W 42 shifts 30x4 ring 8", PSG_NS2D, 128, PSG_REC_NS2D, 42, 30, 4, 1, 0.0, 8, 12);

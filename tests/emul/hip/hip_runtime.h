@@ -123,6 +123,11 @@ inline hipError_t hipMalloc(void **p, size_t n) {
 }
 template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void **)p, n); }
 inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+// the emulated device reports 16 GiB, all of it free
+inline hipError_t hipMemGetInfo(size_t *fr, size_t *tot) {
+  *fr = *tot = (size_t)16 << 30;
+  return hipSuccess;
+}
 // a file-scope device array is ordinary host memory here: its address is the symbol's
 inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { memmove(d, s, n); return hipSuccess; }

@@ -38,6 +38,55 @@ def test_tl_2d_matches_oracle_central_fd(gpu):
     assert np.abs(dz2 - 3.0 * dz1).max() < 1e-12 * np.abs(dz2).max()
 
 
+@pytest.mark.parametrize("policy", [0, 1])
+def test_tangent_about_recorded_base(gpu, policy):
+    # psg_tangent_many records the base flow's stages once and integrates each
+    # direction alone against the record: the same Jv as the combined stepper,
+    # and the oracle's central difference
+    n, v2, dt0 = 64, 1 / 40.0, 0.01
+    T = 0.4 if policy == 0 else 0.437
+    g = O.grid2(n, n)
+    z = O.NS2D(n, v2=v2, dt=dt0).run(O.ic_ns2d_cfg(g, 3, 1.0), 50)
+    ds = np.array([masked_random(g, 30 + i) for i in range(3)])
+    with P.Stepper(flavour="ns2d", n=n, v2=v2, dt=dt0, kf=4.0, ampfor=-2.0) as s:
+        z1, dm = s.tangent_many(z[None], ds[:, None, :], T, dt0, policy)
+        _, d32 = s.tangent_many(z[None], ds[:, None, :], T, dt0, policy, fp32=True)
+        for i in range(3):
+            zc, dc = s.tangent(z[None], ds[i][None], T, dt0, policy)
+            assert np.abs(dm[i] - dc).max() <= 1e-12 * np.abs(dc).max(), i
+            assert np.abs(z1 - zc).max() <= 1e-13 * np.abs(zc).max()
+            err32 = np.abs(d32[i] - dc).max() / np.abs(dc).max()
+            assert 0 < err32 < 1e-5, err32
+    nst = int(T / dt0)
+    eps = 1e-6
+    if policy == 0:
+        orc = O.NS2D(n, v2=v2, dt=T / nst)
+        fd = (orc.run(z + eps * ds[0], nst) - orc.run(z - eps * ds[0], nst)) / (2 * eps)
+        assert np.abs(dm[0, 0] - fd).max() / np.abs(fd).max() < 1e-7
+
+
+@pytest.mark.parametrize("jv", [1, 2])
+def test_newton_recorded_and_combined_tangent_agree(gpu, jv):
+    # the tangent-linear Jv against the recorded base takes the solver through
+    # the same iterations as the combined base + perturbation stepper
+    n, re_ = 32, 5.0
+    g = O.grid2(n, n)
+    kx, ky, _, L = O.tables(g)
+    lam = np.zeros(g.nkt, complex)
+    lam[np.where((kx == 0) & (ky == 4))[0][0]] = -re_ / 8
+    guess = lam + masked_random(g, 7, 0.05)
+    out = []
+    for rec in (0, 1):
+        with P.Stepper(flavour="ns2d", n=n, v2=1 / re_, dt=0.01, kf=4.0, ampfor=-2.0) as s:
+            out.append(s.newton(guess[None], T=2.0, nshift=0, fixed_period=1, jv=jv, dt0=0.01, max_gmres=40,
+                                tl_record=rec))
+    (z0, r0), (z1, r1) = out
+    print(r0, r1, flush=True)
+    assert r0["status_name"] == "CONVERGED" == r1["status_name"]
+    assert r0["newton_iters"] == r1["newton_iters"] and r0["gmres_iters"] == r1["gmres_iters"]
+    assert np.abs(z1 - z0).max() < 1e-12 and np.abs(z1[0] - lam).max() < 1e-10
+
+
 @pytest.mark.parametrize("jv", [0, 1, 2])
 def test_newton_converges_to_laminar_equilibrium(gpu, jv):
     # below the first instability the Kolmogorov laminar state is the unique
