@@ -41,9 +41,12 @@ k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__r
     w[2][mm] = ld_fill(s0 + 2 * fs, q);
     lm[mm] = (umask && q.idx >= 0) ? (double)mask[q.tidx] : 1.0;
   }
+  // i kx rho and i ky rho are one transform of rho scaled by constants of the
+  // pencil, so nine outputs take eight transforms
 #pragma unroll 1
   for (int f = 0; f < nout; f++) {
     cpx<P> v[EPT];
+    if (f == 7 && nout == 9) continue;   // written with field 6
     if (f == 6) {
       // vorticity no longer needed: reuse its registers for rho
 #pragma unroll
@@ -65,19 +68,30 @@ k_zinv_bous(const cpx<double> *__restrict__ st, long st_ms, long fs, cpx<P> *__r
         cpx<double> u, vv, ww;
         biot_savart(kx, kyd, kzd, w[0][mm], w[1][mm], w[2][mm], wki, u, vv, ww);
         r = f == 3 ? u : (f == 4 ? vv : ww);
-      } else if (nout == 7) {
-        r = w[0][mm];
+      } else if (nout == 7 || f == 6) {
+        r = w[0][mm];   // rho (the gradient's x and y parts are scaled after the transform)
       } else {
-        const double kk = f == 6 ? kx : (f == 7 ? kyd : kzd);   // setFFK: i k rho
-        r = cpx<double>{-kk * w[0][mm].y, kk * w[0][mm].x};
+        r = cpx<double>{-kzd * w[0][mm].y, kzd * w[0][mm].x};   // setFFK: i kz rho
       }
       v[mm] = to<P>(r);
     }
     fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
     if (valid) {
       cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz;
+      if (f == 6 && nout == 9) {
+        // setFFK: i kx rho and i ky rho
+        const P px = (P)kx, py = (P)kyd;
+        cpx<P> *o7 = o + g.zsz;
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_ZINV>::st>(&o[zoff<P>(g, ikx, iky, j + TP * mm)], v[mm]);
+        for (int mm = 0; mm < EPT; mm++) {
+          const long at = zoff<P>(g, ikx, iky, j + TP * mm);
+          st_pass<NT<P, PASS_ZINV>::st>(&o[at], cpx<P>{-px * v[mm].y, px * v[mm].x});
+          st_pass<NT<P, PASS_ZINV>::st>(&o7[at], cpx<P>{-py * v[mm].y, py * v[mm].x});
+        }
+      } else {
+#pragma unroll
+        for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_ZINV>::st>(&o[zoff<P>(g, ikx, iky, j + TP * mm)], v[mm]);
+      }
     }
   }
 }

@@ -11,7 +11,8 @@ Its advection v d/dy vanishes on y-independent fields, the Lamb vector
 u x omega = grad(v^2/2) has no curl, and u.g = 0 leaves rho = 0.  Modes with
 kx = 1, 2 and kz = 0, 2, -1 make every shift other than (0, 0) change the
 state, so a guess started at a shifted (sx, sz) must come back to (0, 0).
-Every y-directed v(x, z) is steady too, so J has a null space and the
+Every y-directed v(x, z) is steady too, so J has a null space (Newton may
+move the state along that family) and the
 inviscid linearisation puts its spectrum near the unit circle: GMRES meets
 the Krylov cap and Newton closes in linearly, about 1.5x per step, instead
 of converging to NETOL in a few steps."""
@@ -116,7 +117,9 @@ def test_bous_newton_two_shifts_return_to_steady_state(gpu):
     assert r0["ner"] > 1e-3
     assert res["ner"] < 1e-3 * r0["ner"], res
     assert abs(res["sx"]) < 1e-5 and abs(res["sz"]) < 1e-5
-    assert np.abs(z - zs).max() < 1e-4 * np.abs(zs).max()
+    # the state may move along the family of steady states, but stays in it
     zT = orc.run(z, int(T / dt0))
     host = ner_host(g, z, zT, res["sx"], res["sz"])
     assert abs(host - res["ner"]) <= 1e-6 * res["ner"], (host, res["ner"])
+    assert ner_host(g, z, zT, 0.0, 0.0) < 1e-3 * r0["ner"]
+    assert np.abs(z).max() > 0.5 * np.abs(zs).max()

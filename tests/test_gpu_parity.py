@@ -240,6 +240,20 @@ def test_ns3d_xpass_variants_match_oracle(gpu, variant):
         assert rel(s.get_state(), want) < 1e-11
 
 
+@pytest.mark.parametrize("variant", range(12))
+def test_bous_xpass_variants_match_oracle(gpu, variant):
+    n = 32
+    st = O.ic_solenoidal(O.grid3(n, n, n), 43, 0.5, 0.1)
+    want = O.BOUS(n, v2=1 / 100.0, ri=1.0, theta=np.pi / 2, dt=0.004).run(st, 10)
+    for prec, tol in (("fp64", 1e-11), ("fp32", 1e-4)):
+        with P.Stepper(flavour="bous", n=n, v2=1 / 100.0, ri=1.0, theta=np.pi / 2, dt=0.004, precision=prec) as s:
+            s.set_variant(variant)
+            assert s.get_variant() == variant
+            s.set_state(st)
+            s.run(10)
+            assert rel(s.get_state(), want) < tol, prec
+
+
 @pytest.mark.parametrize("variant", range(2))
 def test_ns3d_zfwd_variants_match_oracle(gpu, variant):
     n = 32
