@@ -20,6 +20,18 @@ template <int N, typename P = double> struct Col {
   static constexpr int T = T1 * TP > 1024 ? 1024 / TP : T1;
   static constexpr int THREADS = T * TP;
 };
+// y passes: the same tile with their own FP32 minimum width
+#ifndef PSG_COLT32Y
+#define PSG_COLT32Y PSG_COLT32
+#endif
+template <int N, typename P = double> struct ColY {
+  static constexpr int TP = N / EPT;
+  static constexpr int T0 = 256 / TP;
+  static constexpr int TMIN = sizeof(P) == 4 ? PSG_COLT32Y : 8;
+  static constexpr int T1 = T0 < TMIN ? TMIN : (T0 > 32 ? 32 : T0);
+  static constexpr int T = T1 * TP > 1024 ? 1024 / TP : T1;
+  static constexpr int THREADS = T * TP;
+};
 // x-pass row teams
 template <int N> struct Row {
   static constexpr int TPX = N / EPT;

@@ -105,12 +105,12 @@ template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, i
   if (nzs <= 0) z0 = 0, nzs = g.nz;
   if (z0 < 0 || z0 + nzs > g.nz) { set_error("y-inverse: planes %d+%d outside %d", z0, nzs, g.nz); return PSG_EINVAL; }
   PSG_DISPATCH_N(g.ny, {
-    constexpr int T = Col<N, P>::T;
+    constexpr int T = ColY<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, nzs, nmf);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_yinv<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1), z0);
+    hipLaunchKernelGGL(k, grid, dim3(ColY<N, P>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1), z0);
   });
   PSG_HIP(hipGetLastError());
   return 0;
@@ -121,12 +121,12 @@ template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, i
   if (nzs <= 0) z0 = 0, nzs = g.nz;
   if (z0 < 0 || z0 + nzs > g.nz) { set_error("y-forward: planes %d+%d outside %d", z0, nzs, g.nz); return PSG_EINVAL; }
   PSG_DISPATCH_N(g.ny, {
-    constexpr int T = Col<N, P>::T;
+    constexpr int T = ColY<N, P>::T;
     dim3 grid((g.iktx + T - 1) / T, nzs, nmf);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     auto k = k_yfwd<P, N, T>;
     PSG_HIP(lds_attr(k, lds));
-    hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1), z0);
+    hipLaunchKernelGGL(k, grid, dim3(ColY<N, P>::THREADS), lds, c->stream, in, out, g, twp<P>(c, 1), z0);
   });
   PSG_HIP(hipGetLastError());
   return 0;

@@ -41,6 +41,14 @@ __device__ __forceinline__ void add_forcing(const Force &F, double kx, double ky
 template <typename P> int launch_yinv_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
 template <typename P> int launch_yfwd_g(Ctx *c, const cpx<P> *in, cpx<P> *out, int nmf, int z0 = 0, int nzs = 0);
 
+// ky rows per block of the FP32 z-inverse and z-forward (the next row's loads
+// overlap this row's transforms)
+#ifndef PSG_ZPIPE32
+#define PSG_ZPIPE32 1
+#endif
+#ifndef PSG_ZFPIPE32
+#define PSG_ZFPIPE32 1
+#endif
 // LAY: layout of the state st (LayNKT: the context's state and caller
 // buffers; LayW: the stepper's working copy, with the mask in LayW order)
 template <typename P, typename S = double, class LAY = LayNKT>
@@ -52,15 +60,16 @@ static int launch_zinv(Ctx *c, const cpx<S> *st, long st_ms, cpx<P> *out, long o
   const int8_t *mask = W ? c->wmask : c->mask;
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N, P>::T;
-    dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
+    constexpr int PIPE = sizeof(P) == 4 ? PSG_ZPIPE32 : 1;
+    dim3 grid((g.iktx + T - 1) / T, (g.ikty + PIPE - 1) / PIPE, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
     if (bs) {
-      auto k = k_zinv<P, N, T, true, S, LAY>;
+      auto k = k_zinv<P, N, T, true, S, LAY, PIPE>;
       PSG_HIP(lds_attr(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, st_fs, out, out_ms, g,
                          twp<P>(c, 2), mask, umask);
     } else {
-      auto k = k_zinv<P, N, T, false, S, LAY>;
+      auto k = k_zinv<P, N, T, false, S, LAY, PIPE>;
       PSG_HIP(lds_attr(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, st, st_ms, st_fs, out, out_ms, g,
                          twp<P>(c, 2), mask, umask);
@@ -106,9 +115,10 @@ static int launch_zfwd(Ctx *c, const cpx<P> *in, UPD upd, int nm) {
   const Grid &g = c->g;
   PSG_DISPATCH_N(g.nz, {
     constexpr int T = Col<N, P>::T;
-    dim3 grid((g.iktx + T - 1) / T, g.ikty, nm);
+    constexpr int PIPE = (sizeof(P) == 4 && FRONT) ? PSG_ZFPIPE32 : 1;
+    dim3 grid((g.iktx + T - 1) / T, (g.ikty + PIPE - 1) / PIPE, nm);
     size_t lds = (size_t)(N * T + N) * sizeof(cpx<P>);
-    auto k = k_zfwd<P, N, T, FO, UPD, FRONT>;
+    auto k = k_zfwd<P, N, T, FO, UPD, FRONT, PIPE>;
     PSG_HIP(lds_attr(k, lds));
     hipLaunchKernelGGL(k, grid, dim3(Col<N, P>::THREADS), lds, c->stream, in, g, twp<P>(c, 2), upd);
   });

@@ -246,7 +246,10 @@ __device__ __forceinline__ cpx<S> ld_fill(const cpx<S> *__restrict__ f, const Fi
 // S: the state's storage type (double; float for the FP32 working state of
 // an FP32-bulk run)
 // LAY: the state's layout (st_fs its field stride); mask is in LAY's table order
-template <typename P, int N, int T, bool BS, typename S = double, class LAY = LayNKT>
+// PIPE: ky rows per block, one after the other; the next row's state values
+// are requested before this row's transforms, so that their loads are in
+// flight while the row computes (the grid's y extent is IKTY/PIPE rounded up)
+template <typename P, int N, int T, bool BS, typename S = double, class LAY = LayNKT, int PIPE = 1>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zinv(const cpx<S> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict__ out, long out_ms, Grid g,
        const cpx<P> *__restrict__ tw, const int8_t *__restrict__ mask, int umask) {
@@ -254,48 +257,68 @@ k_zinv(const cpx<S> *__restrict__ st, long st_ms, long st_fs, cpx<P> *__restrict
   constexpr int TP = N / EPT;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
-  const int ikx = tid.x * T + col, iky = tid.y, m = tid.z;
+  const int ikx = tid.x * T + col, iky0 = tid.y * PIPE, m = tid.z;
   const bool valid = ikx < g.iktx;
-  const int ky = iky - g.kty;
-  const double kx = (double)ikx * g.alpha, kyd = (double)ky;
+  const double kx = (double)ikx * g.alpha;
   const cpx<S> *s0 = st + (long)m * st_ms;
   // every state value of the pencil is requested before the first transform;
   // it is kept in the pass precision (in FP32-bulk the Biot-Savart product then
   // starts from the rounded vorticity, and the registers halve)
-  cpx<P> w[3][EPT];
-  P lm[EPT];
-#pragma unroll
-  for (int mm = 0; mm < EPT; mm++) {
-    const int kz = valid ? wrapk(j + TP * mm, g.nz, g.ktz) : PSG_NOK;
-    const FillIdx q = fill_index<LAY>(g, ikx, iky, ky, kz);
-    w[0][mm] = to<P>(ld_fill(s0, q));
-    w[1][mm] = to<P>(ld_fill(s0 + st_fs, q));
-    w[2][mm] = to<P>(ld_fill(s0 + 2 * st_fs, q));
-    lm[mm] = (umask && q.idx >= 0) ? (P)mask[q.tidx] : (P)1;
-  }
-  constexpr int NFO = BS ? 6 : 3;
-#pragma unroll
-  for (int f = 0; f < NFO; f++) {
-    cpx<P> v[EPT];
+  auto load = [&](int iky, cpx<P>(&w)[3][EPT], P(&lm)[EPT]) {
+    const bool ok = valid && iky < g.ikty;
 #pragma unroll
     for (int mm = 0; mm < EPT; mm++) {
-      if (f < 3) {
-        v[mm] = w[f][mm];
-      } else {
-        const int kz = wrapk(j + TP * mm, g.nz, g.ktz);
-        const double kzd = kz == PSG_NOK ? 0.0 : (double)kz;
-        const double wk = kx * kx + kyd * kyd + kzd * kzd;
-        const double wki = (double)lm[mm] / (wk > 0.001 ? wk : 0.001);
-        cpx<double> u, vv, ww;
-        biot_savart(kx, kyd, kzd, up(w[0][mm]), up(w[1][mm]), up(w[2][mm]), wki, u, vv, ww);
-        v[mm] = to<P>(f == 3 ? u : (f == 4 ? vv : ww));
+      const int kz = ok ? wrapk(j + TP * mm, g.nz, g.ktz) : PSG_NOK;
+      const FillIdx q = fill_index<LAY>(g, ikx, iky, iky - g.kty, kz);
+      w[0][mm] = to<P>(ld_fill(s0, q));
+      w[1][mm] = to<P>(ld_fill(s0 + st_fs, q));
+      w[2][mm] = to<P>(ld_fill(s0 + 2 * st_fs, q));
+      lm[mm] = (umask && q.idx >= 0) ? (P)mask[q.tidx] : (P)1;
+    }
+  };
+  cpx<P> w[3][EPT];
+  P lm[EPT];
+  load(iky0, w, lm);
+  constexpr int NFO = BS ? 6 : 3;
+#pragma unroll
+  for (int r = 0; r < PIPE; r++) {
+    const int iky = iky0 + r;
+    const double kyd = (double)(iky - g.kty);
+    cpx<P> wn[3][EPT];
+    P lmn[EPT];
+    if (r + 1 < PIPE) load(iky + 1, wn, lmn);
+#pragma unroll
+    for (int f = 0; f < NFO; f++) {
+      cpx<P> v[EPT];
+#pragma unroll
+      for (int mm = 0; mm < EPT; mm++) {
+        if (f < 3) {
+          v[mm] = w[f][mm];
+        } else {
+          const int kz = wrapk(j + TP * mm, g.nz, g.ktz);
+          const double kzd = kz == PSG_NOK ? 0.0 : (double)kz;
+          const double wk = kx * kx + kyd * kyd + kzd * kzd;
+          const double wki = (double)lm[mm] / (wk > 0.001 ? wk : 0.001);
+          cpx<double> u, vv, ww;
+          biot_savart(kx, kyd, kzd, up(w[0][mm]), up(w[1][mm]), up(w[2][mm]), wki, u, vv, ww);
+          v[mm] = to<P>(f == 3 ? u : (f == 4 ? vv : ww));
+        }
+      }
+      fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
+      if (valid && iky < g.ikty) {
+        cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz;
+#pragma unroll
+        for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_ZINV>::st>(&o[zoff<P>(g, ikx, iky, j + TP * mm)], v[mm]);
       }
     }
-    fft_pencil<P, N, EPT, true>(v, lds, j, T, col, tw);
-    if (valid) {
-      cpx<P> *o = out + (long)m * out_ms + (long)f * g.zsz;
+    if (r + 1 < PIPE) {
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) st_pass<NT<P, PASS_ZINV>::st>(&o[zoff<P>(g, ikx, iky, j + TP * mm)], v[mm]);
+      for (int mm = 0; mm < EPT; mm++) {
+        w[0][mm] = wn[0][mm];
+        w[1][mm] = wn[1][mm];
+        w[2][mm] = wn[2][mm];
+        lm[mm] = lmn[mm];
+      }
     }
   }
 }
@@ -357,54 +380,81 @@ k_yfwd(const cpx<P> *__restrict__ in, cpx<P> *__restrict__ out, Grid g, const cp
 // to 4 fields); otherwise each field is loaded just before its transform,
 // which frees 4*(FO-1)*EPT registers (the 6-field tangent-linear/adjoint
 // updates always do this).
-template <typename P, int N, int T, int FO, class UPD, bool FRONT = (FO <= 4)>
+// PIPE (with FRONT): ky rows per block, the next row's inputs requested
+// before this row's transforms (the grid's y extent is IKTY/PIPE rounded up)
+template <typename P, int N, int T, int FO, class UPD, bool FRONT = (FO <= 4), int PIPE = 1>
 __global__ void __launch_bounds__(T *(N / EPT))
 k_zfwd(const cpx<P> *__restrict__ in, Grid g, const cpx<P> *__restrict__ tw, UPD upd) {
+  static_assert(PIPE == 1 || FRONT, "row pipelining needs front-loaded inputs");
   PSG_LDS_TWIDDLES(P, N, EPT);
   constexpr int TP = N / EPT;
   constexpr int NV = FRONT ? FO : 1;
   const int col = threadIdx.x % T, j = threadIdx.x / T;
   const TileId tid = xcd_tile();
-  const int ikx = tid.x * T + col, iky = tid.y, m = tid.z;
+  const int ikx = tid.x * T + col, iky0 = tid.y * PIPE, m = tid.z;
   const bool valid = ikx < g.iktx;
-  cpx<double> res[FO][EPT];
-  cpx<P> vin[NV][EPT];
-  auto load = [&](int f, cpx<P>(&v)[EPT]) {
+  const double kx = (double)ikx * g.alpha;
+  auto load = [&](int f, int iky, cpx<P>(&v)[EPT]) {
     const cpx<P> *src = in + ((long)m * FO + f) * g.zsz;
+    const bool ok = valid && iky < g.ikty;
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) v[mm] = valid ? ld_pass<NT<P, PASS_ZFWD>::ld>(&src[zoff<P>(g, ikx, iky, j + TP * mm)]) : cpx<P>{0, 0};
+    for (int mm = 0; mm < EPT; mm++) v[mm] = ok ? ld_pass<NT<P, PASS_ZFWD>::ld>(&src[zoff<P>(g, ikx, iky, j + TP * mm)]) : cpx<P>{0, 0};
   };
+  cpx<P> vin[NV][EPT];
   if constexpr (FRONT) {
 #pragma unroll
-    for (int f = 0; f < FO; f++) load(f, vin[f]);
+    for (int f = 0; f < FO; f++) load(f, iky0, vin[f]);
   }
 #pragma unroll
-  for (int f = 0; f < FO; f++) {
-    cpx<P> v[EPT];
-    if constexpr (FRONT) {
+  for (int r = 0; r < PIPE; r++) {
+    const int iky = iky0 + r;
+    cpx<P> vnext[PIPE > 1 ? NV : 1][EPT];
+    if constexpr (PIPE > 1) {
+      if (r + 1 < PIPE) {
 #pragma unroll
-      for (int mm = 0; mm < EPT; mm++) v[mm] = vin[f][mm];
-    } else {
-      load(f, v);
+        for (int f = 0; f < FO; f++) load(f, iky + 1, vnext[f]);
+      }
     }
-    fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
+    cpx<double> res[FO][EPT];
 #pragma unroll
-    for (int mm = 0; mm < EPT; mm++) {
-      cpx<double> t = up(v[mm]);
-      res[f][mm] = cpx<double>{t.x * g.inr, t.y * g.inr};
+    for (int f = 0; f < FO; f++) {
+      cpx<P> v[EPT];
+      if constexpr (FRONT) {
+#pragma unroll
+        for (int mm = 0; mm < EPT; mm++) v[mm] = vin[f][mm];
+      } else {
+        load(f, iky, v);
+      }
+      fft_pencil<P, N, EPT, false>(v, lds, j, T, col, tw);
+#pragma unroll
+      for (int mm = 0; mm < EPT; mm++) {
+        cpx<double> t = up(v[mm]);
+        res[f][mm] = cpx<double>{t.x * g.inr, t.y * g.inr};
+      }
     }
-  }
-  if (!valid) return;
-  const double kx = (double)ikx * g.alpha, ky = (double)(iky - g.kty);
+    // rows past IKTY and columns past IKTX run the transforms (the block's
+    // barriers) but update nothing
+    if (valid && iky < g.ikty) {
+      const double ky = (double)(iky - g.kty);
 #pragma unroll
-  for (int mm = 0; mm < EPT; mm++) {
-    const int kz = wrapk(j + TP * mm, g.nz, g.ktz);
-    if (kz == PSG_NOK) continue;
-    const long ik = ((long)(kz + g.ktz) * g.ikty + iky) * g.iktx + ikx;
-    cpx<double> nn[FO];
+      for (int mm = 0; mm < EPT; mm++) {
+        const int kz = wrapk(j + TP * mm, g.nz, g.ktz);
+        if (kz == PSG_NOK) continue;
+        const long ik = ((long)(kz + g.ktz) * g.ikty + iky) * g.iktx + ikx;
+        cpx<double> nn[FO];
 #pragma unroll
-    for (int f = 0; f < FO; f++) nn[f] = res[f][mm];
-    upd(m, ik, ikx, kx, ky, (double)kz, nn);
+        for (int f = 0; f < FO; f++) nn[f] = res[f][mm];
+        upd(m, ik, ikx, kx, ky, (double)kz, nn);
+      }
+    }
+    if constexpr (PIPE > 1) {
+      if (r + 1 < PIPE) {
+#pragma unroll
+        for (int f = 0; f < FO; f++)
+#pragma unroll
+          for (int mm = 0; mm < EPT; mm++) vin[f][mm] = vnext[f][mm];
+      }
+    }
   }
 }
 
