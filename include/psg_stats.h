@@ -57,12 +57,14 @@ int psg_stats_create(psg_ctx *ctx, int member, psg_stats **out);
  * running averages <z> <- (<z> count + z)/(count + 1) updated on the device.
  * vals: PSG_STNS_N or PSG_STB_N doubles. */
 int psg_stats_update(psg_stats *s, long count, double *vals);
-/* At a Tout crossing (Boussinesq, out_low.h:5-76): physical fields of the
- * current state, then, as `what` asks,
+/* At a Tout crossing (Boussinesq, out_low.h:5-76), or every NOUT steps
+ * (3D Navier-Stokes avgVelocity, 3D_NavierStokes/src/cuda.cu:70 and
+ * kernels.h:70-80): physical fields of the current state, then, as `what` asks,
  *   PSG_STP_RMS   <u^2>, <v^2>, <w^2> <- (. count + u^2)/(count + 1) per grid point
  *   PSG_STP_UMAX  *umax = max |u| over the grid (the adaptive step)
  *   PSG_STP_RIG   navg/savg (nz entries, or ny when theta < pi/4) of
- *                 sum (N^2) and sum (shear^2) per plane, *nplanes set */
+ *                 sum (N^2) and sum (shear^2) per plane, *nplanes set
+ *                 (Boussinesq only) */
 enum { PSG_STP_RMS = 1, PSG_STP_UMAX = 2, PSG_STP_RIG = 4 };
 int psg_stats_physical(psg_stats *s, int what, long count, double *umax, double *navg, double *savg,
                        int *nplanes);
@@ -70,9 +72,11 @@ int psg_stats_physical(psg_stats *s, int what, long count, double *umax, double 
  * j <= size (horizontal |k_h| and vertical k_v bins, k_v = ky when theta < pi/4,
  * else kz; negative k_v in bin 0 as max(k_v, 1e-3) puts them); cap >= nx + 1 */
 int psg_stats_spectra(psg_stats *s, double *spch, double *spcv, int cap, int *size);
-/* finalise.h mean fields (Boussinesq): 7 physical fields of nx*ny*nz doubles,
- * x fastest, in the avgs.dat order: rho, u, v, w of the time-averaged state,
- * <v^2>, <w^2>, and <u^2> - <u>^2 */
+/* finalise.h mean fields, physical fields of nx*ny*nz doubles, x fastest, in
+ * the avgs.dat order.  Boussinesq: 7 fields, rho, u, v, w of the time-averaged
+ * state, <v^2>, <w^2>, and <u^2> - <u>^2 (3D_Boussinesq/src/finalise.h);
+ * 3D Navier-Stokes: 5 fields, u, v, w of the time-averaged state, <v^2> and
+ * <u^2> - <u>^2 (3D_NavierStokes/src/finalise.h:25-117) */
 int psg_stats_mean_fields(psg_stats *s, double *out);
 /* the time-averaged state (nf*nkt complex) */
 int psg_stats_tavg(psg_stats *s, double *host);

@@ -628,8 +628,9 @@ class Stats:
         return h, v, size.value
 
     def mean_fields(self):
+        """Boussinesq: rho, u, v, w, <v^2>, <w^2>, <u^2>-<u>^2; 3D Navier-Stokes: u, v, w, <v^2>, <u^2>-<u>^2."""
         c = self.st.cfg
-        out = np.zeros((7, c.nz, c.ny, c.nx))
+        out = np.zeros((7 if c.flavour == BOUS3D else 5, c.nz, c.ny, c.nx))
         _check(lib().psg_stats_mean_fields(self.h, _ptr(out)), "psg_stats_mean_fields")
         return out
 

@@ -161,6 +161,8 @@ int ns3d_work_end(Ctx *c);
 int ns3d_tables_w(Ctx *c);
 int ns3d_nonlin(Ctx *c, int member, const cpx<double> *in_dev, cpx<double> *out_dev);
 int ns3d_physical(Ctx *c, int member, double *out_dev);
+// physical fields (xi, eta, zeta, u, v, w) of a 3D Navier-Stokes state held anywhere in device memory
+int ns3d_physical_state(Ctx *c, const cpx<double> *st, double *out_dev);
 int set_member_dt(Ctx *c, const double *host);
 int ns2d_init(Ctx *c);
 int ns2d_run(Ctx *c, int nsteps, bool first);

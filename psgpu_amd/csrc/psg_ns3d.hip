@@ -403,17 +403,20 @@ template <typename P> int launch_xphys(Ctx *c, const cpx<P> *B, double *out, int
 template int launch_xphys<float>(Ctx *, const cpx<float> *, double *, int);
 template int launch_xphys<double>(Ctx *, const cpx<double> *, double *, int);
 
-template <typename P> static int physical_t(Ctx *c, int member, double *out) {
+template <typename P> static int physical_t(Ctx *c, const cpx<double> *st, double *out) {
   const Grid &g = c->g;
   cpx<P> *A = (cpx<P> *)c->bufA, *B = (cpx<P> *)c->bufB;
-  PSG_TRY(launch_zinv<P>(c, c->state + (long)member * 3 * g.nkt, 3 * g.nkt, A, 6 * g.zsz, true, 1, 1));
+  PSG_TRY(launch_zinv<P>(c, st, 3 * g.nkt, A, 6 * g.zsz, true, 1, 1));
   PSG_TRY(launch_yinv_g<P>(c, A, B, 6));
   return launch_xphys<P>(c, B, out, 6);
 }
 
+int ns3d_physical_state(Ctx *c, const cpx<double> *st, double *out_dev) {
+  return c->cfg.precision == PSG_FP32BULK ? physical_t<float>(c, st, out_dev) : physical_t<double>(c, st, out_dev);
+}
+
 int ns3d_physical(Ctx *c, int member, double *out_dev) {
-  return c->cfg.precision == PSG_FP32BULK ? physical_t<float>(c, member, out_dev)
-                                          : physical_t<double>(c, member, out_dev);
+  return ns3d_physical_state(c, c->state + (long)member * 3 * c->g.nkt, out_dev);
 }
 
 int ns3d_init(Ctx *c) {

@@ -380,12 +380,17 @@ int psg_stats_update(psg_stats *s, long count, double *vals) {
   return 0;
 }
 
+// physical fields of a state: (xi, eta, zeta, u, v, w[, rho]) as the flavour has them
+static int phys_state(psg_stats *s, const cpx<double> *st) {
+  return s->bous ? bous_physical_state(s->c, st, s->phys) : ns3d_physical_state(s->c, st, s->phys);
+}
+
 int psg_stats_physical(psg_stats *s, int what, long count, double *umax, double *navg, double *savg, int *nplanes) {
   if (!s) { set_error("psg_stats_physical: null handle"); return PSG_EINVAL; }
-  if (!s->bous) { set_error("psg_stats_physical: Boussinesq contexts only"); return PSG_EINVAL; }
+  if (!s->bous && (what & PSG_STP_RIG)) { set_error("psg_stats_physical: RiG needs a Boussinesq context"); return PSG_EINVAL; }
   Ctx *c = s->c;
   if (int r = st_alloc_phys(s)) return r;
-  if (int r = bous_physical_state(c, c->state + (long)s->member * s->nf * s->nkt, s->phys)) return r;
+  if (int r = phys_state(s, c->state + (long)s->member * s->nf * s->nkt)) return r;
   const int nb = 2048;
   if (what & PSG_STP_RMS) {
     hipLaunchKernelGGL(k_rms, dim3(nb), dim3(256), 0, c->stream, s->phys, s->nr, s->rms, (double)count);
@@ -464,17 +469,21 @@ int psg_stats_spectra(psg_stats *s, double *spch, double *spcv, int cap, int *si
 
 int psg_stats_mean_fields(psg_stats *s, double *out) {
   if (!s || !out) { set_error("psg_stats_mean_fields: null argument"); return PSG_EINVAL; }
-  if (!s->bous) { set_error("psg_stats_mean_fields: Boussinesq contexts only"); return PSG_EINVAL; }
   Ctx *c = s->c;
   if (int r = st_alloc_phys(s)) return r;
-  if (int r = bous_physical_state(c, s->tavg, s->phys)) return r;
+  if (int r = phys_state(s, s->tavg)) return r;
   const long nr = s->nr;
   hipLaunchKernelGGL(k_uvar, dim3(2048), dim3(256), 0, c->stream, s->rms, s->phys + 3 * nr, nr, s->phys);
   ST_HIP(hipGetLastError());
-  const double *src[7] = {s->phys + 6 * nr, s->phys + 3 * nr, s->phys + 4 * nr, s->phys + 5 * nr,
-                          s->rms + nr, s->rms + 2 * nr, s->phys};
-  for (int f = 0; f < 7; f++)
-    ST_HIP(hipMemcpyAsync(out + f * nr, src[f], sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+  // Boussinesq: rho, u, v, w, <v^2>, <w^2>, <u^2>-<u>^2 (3D_Boussinesq/src/finalise.h:62-160);
+  // Navier-Stokes: u, v, w, <v^2>, <u^2>-<u>^2 (3D_NavierStokes/src/finalise.h:25-117)
+  const double *srcb[7] = {s->phys + 6 * nr, s->phys + 3 * nr, s->phys + 4 * nr, s->phys + 5 * nr,
+                           s->rms + nr, s->rms + 2 * nr, s->phys};
+  const double *srcn[5] = {s->phys + 3 * nr, s->phys + 4 * nr, s->phys + 5 * nr, s->rms + nr, s->phys};
+  const int nfo = s->bous ? 7 : 5;
+  for (int f = 0; f < nfo; f++)
+    ST_HIP(hipMemcpyAsync(out + f * nr, s->bous ? srcb[f] : srcn[f], sizeof(double) * nr, hipMemcpyDeviceToHost,
+                          c->stream));
   ST_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }

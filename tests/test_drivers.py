@@ -314,7 +314,10 @@ def test_dns_driver_ns3d_reference_run_matches_oracle(gpu, tmp_path):
     assert struct.unpack("<4i", recs[0]) == (1, g.iktx, g.ikty, g.iktz)
     got = np.frombuffer(recs[1][56:], np.complex128).reshape(3, g.nkt)
     assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
-    assert len(open(os.path.join(d, "JOB0", "stats.dat")).read().strip().splitlines()) == 3
+    # every NOUT = 5 of 20 steps from NT = 5, and finalise.h's closing computeStats
+    assert len(open(os.path.join(d, "JOB0", "stats.dat")).read().strip().splitlines()) == 4
+    for f in ("avgs.dat", "means.dat", "pdfs.dat"):
+        assert os.path.getsize(os.path.join(d, "JOB0", f)) > 0
     assert struct.unpack("<4i", read_recs(os.path.join(d, "JOB0", "Zk.out"))[0]) == (1, g.iktx, g.ikty, g.iktz)
 
 

@@ -92,7 +92,7 @@ def test_legacy_ns3d_dns_outputs(gpu, tmp_path, monkeypatch):
     assert rel(p1, orc.physical(orc.run(st, 10))[2].ravel()) < 1e-12
     assert rel(zr, p1) == 0.0
     lines = open("JOB0/stats.dat").read().strip().splitlines()
-    assert len(lines) == 3                    # after steps NT = 5, 10, 15
+    assert len(lines) == 4                    # after steps NT = 5, 10, 15, and finalise.h's last one
     t, en = map(float, lines[0].split()[:2])
     assert abs(t - 6 * dt) < 1e-12            # tme after iteration NT=5 (6 steps)
     assert abs(en - orc.diag(orc.run(st, 6))[0]) < 1e-6 * en
