@@ -30,7 +30,16 @@
  * continuation in Re of IGUESSEND steps (trace_branch).  A type L record with
  * RsymFLAG set is rejected (the reflection symmetry is not built).
  *
- * usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift]
+ * Boussinesq: 3D_Boussinesq/src/src_gmres/main.F90 (guess loop 222-1860,
+ * NETOL 1e-10, GMTOL 1e-7, trust region 0.05, the x and z shifts as unknowns;
+ * Theta in degrees, params_update parameters.F90:30-47); records (0, v1, Ri,
+ * alpha, T, sx, sy, sz, NER, xi, eta, zeta, rho) in UPOs.out after a header
+ * and in Zk.out, which holds the last iterate only (main.F90:1739-1744,
+ * 1834-1838); UPOinfo.dat lines in format 1502 (main.F90:1876-1877), whose
+ * "Re" columns carry the ParFLAG parameter (v1, Ri or alpha, main.F90:262-264).
+ * Continuation and Floquet analysis are not built for this flavour.
+ *
+ * usage: psg_gmres ns2d|ns3d|bous NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift]
  * (--jv tl: tangent-linear Jv; tl32: the same in FP32-bulk passes under the
  * FP64 residual; --noshift: the x shift is not an unknown, for equilibria and
  * other shift-free solutions; --list: print each guess as read and adjusted,
@@ -44,9 +53,11 @@
 #include "psg_host.h"
 #include "psg_newton.h"
 
-static const char *status_text[] = {"   \"CONVERGED\"   ", "\"GMRES FAILED\"  ", "\"H-STEP FAILED\" ",
-                                    "\"T-STEP FAILED\" ", "\"NEWTON STALLED\" ", "\"PERIOD TOO SMALL\" ",
-                                    "\"NEWTON FAILED\" "};
+/* between " : " and " : " these give the reference's status fields, e.g.
+ * ' :   "CONVERGED"   : ' and ' : "GMRES FAILED"  : ' (main.F90:1245-1810) */
+static const char *status_text[] = {"  \"CONVERGED\"  ", "\"GMRES FAILED\" ", "\"H-STEP FAILED\"",
+                                    "\"T-STEP FAILED\"", "\"NEWTON STALLED\"", "\"PERIOD TOO SMALL\"",
+                                    "\"NEWTON FAILED\""};
 
 /* Fortran Ew.d: [-]0.ddddddE+xx right-justified in w characters (out: 64 bytes) */
 static void fort_e(char *out, int w, int d, double v) {
@@ -168,6 +179,35 @@ static void report(outputs *w, long ig, const psg_newton_result *r, const double
   else
     printf("guess %ld: %s NER=%.3e Re=%.8g T=%.8f sx=%.6f (%d Newton, %d GMRES, %.1f s)\n", ig,
            status_text[r->status], r->ner, 1.0 / v2, T, sx, r->newton_iters, r->gmres_iters, r->seconds);
+  fflush(stdout);
+}
+
+/* the Boussinesq outputs of one Newton solve (3D_Boussinesq/src/src_gmres/main.F90:
+ * 1739-1744 Zk.out, rewritten in place; 1754-1810 UPOinfo.dat, format 1502;
+ * 1827-1838 UPOs.out); par: the ParFLAG parameter printed in the "Re" columns */
+static void report_bous(outputs *w, long ig, const psg_newton_result *r, const double *z, double v1, double ri,
+                        double alpha, double par, double T, double sx, double sy, double sz) {
+  const double out[9] = {0.0, v1, ri, alpha, T, sx, sy, sz, r->ner};
+  rewind(w->fzk);
+  psgh_zk_record(w->fzk, out, 9, z, w->ncpx);
+  fflush(w->fzk);
+  char e1[64], e2[64], e3[64], e4[64], e5[64];
+  fort_e(e1, 16, 6, w->guess_t);
+  fort_e(e2, 16, 6, w->guess_v2);
+  fort_e(e3, 14, 5, w->guess_ner);
+  fort_e(e4, 16, 6, par);
+  fort_e(e5, 14, 5, r->ner);
+  fprintf(w->finfo, "%5ld    %s%s%12.5f%12.5f%12.5f%12.5f%s : %s : %4d%s%10.5f%10.5f%10.5f%10.5f%s%6d%6d%6d\n", ig, e1,
+          e2, w->guess_T, w->guess_sx, w->guess_sy, w->guess_sz, e3, status_text[r->status], w->nupo + 1, e4, T, sx,
+          sy, sz, e5, r->newton_iters, r->gmres_iters, r->hook_iters);
+  fflush(w->finfo);
+  if (r->status == PSG_NEWTON_CONVERGED) {
+    psgh_zk_record(w->fupo, out, 9, z, w->ncpx);
+    fflush(w->fupo);
+    w->nupo++;
+  }
+  printf("guess %ld: %s NER=%.3e T=%.8f sx=%.6f sz=%.6f (%d Newton, %d GMRES, %.1f s)\n", ig, status_text[r->status],
+         r->ner, T, sx, sz, r->newton_iters, r->gmres_iters, r->seconds);
   fflush(stdout);
 }
 
@@ -328,10 +368,10 @@ static int die(const char *what) {
 
 int main(int argc, char **argv) {
   if (argc < 3)
-    return die("usage: psg_gmres ns2d|ns3d NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift] "
-               "[--list]");
-  const int is2d = strcmp(argv[1], "ns2d") == 0;
-  if (!is2d && strcmp(argv[1], "ns3d") != 0) return die("flavour must be ns2d or ns3d");
+    return die("usage: psg_gmres ns2d|ns3d|bous NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] "
+               "[--noshift] [--list]");
+  const int is2d = strcmp(argv[1], "ns2d") == 0, bous = strcmp(argv[1], "bous") == 0;
+  if (!is2d && !bous && strcmp(argv[1], "ns3d") != 0) return die("flavour must be ns2d, ns3d or bous");
   int nx = atoi(argv[2]), ny = nx, nz = nx, ai = 3, jv = 0, maxnewton = -1, noshift = 0, list = 0;
   double alpha = 1.0;
   const char *dir = ".";
@@ -351,25 +391,31 @@ int main(int argc, char **argv) {
   char path[1024], msg[256];
   snprintf(path, sizeof path, "%s/%s", dir, is2d ? "params_gmres.txt" : "params.txt");
   static psgh_kv kv;
-  if (psgh_params_read(path, is2d ? PSGH_PARAMS_NS2D_GMRES : PSGH_PARAMS_NS3D_GMRES, &kv, msg, sizeof msg))
+  if (psgh_params_read(path, is2d ? PSGH_PARAMS_NS2D_GMRES : (bous ? PSGH_PARAMS_BOUS_GMRES : PSGH_PARAMS_NS3D_GMRES),
+                       &kv, msg, sizeof msg))
     return die(msg);
   static psgh_kv kx;   /* paramsEXTRA.txt (paramsEXTRA_read_txt) */
   snprintf(path, sizeof path, "%s/paramsEXTRA.txt", dir);
-  if (psgh_params_read(path, is2d ? PSGH_EXTRA_NS2D : PSGH_EXTRA_NS3D, &kx, msg, sizeof msg)) return die(msg);
+  if (psgh_params_read(path, is2d ? PSGH_EXTRA_NS2D : (bous ? PSGH_EXTRA_BOUS : PSGH_EXTRA_NS3D), &kx, msg,
+                       sizeof msg))
+    return die(msg);
   const double Re = psgh_kv_float(&kv, "Re", 40.0), delt = psgh_kv_float(&kv, "DELT", 0.005);
   const long irest = psgh_kv_int(&kv, "IREST", 1);
   const long iguessend = psgh_kv_int(&kx, "IGUESSEND", irest);
   const char *infile = psgh_kv_str(&kv, "infile"), *itype = psgh_kv_str(&kv, "infile type");
-  const int family = is2d ? PSGH_REC_NS2D : PSGH_REC_NS3D;
+  const int family = is2d ? PSGH_REC_NS2D : (bous ? PSGH_REC_BOUS : PSGH_REC_NS3D);
   if (!infile || !psgh_restart_known(family, itype)) return die("infile type unknown for this flavour");
   const int arclc = (int)psgh_kv_int(&kx, "arclcFLAG", 0);
   const int arnoldi = (int)psgh_kv_int(&kx, "ArnoldiFLAG", 0);
   if (arclc && arnoldi) return die("arclcFLAG and ArnoldiFLAG are exclusive");
+  if (bous && (arclc || arnoldi)) return die("continuation and Floquet analysis are not built for the Boussinesq flavour");
+  if (bous && jv) return die("the Boussinesq flavour takes finite-difference Jv only");
+  const int parflag = bous ? (int)psgh_kv_int(&kx, "ParFLAG", 1) : 1;
   if (irest < 1) return die("IREST must name the first guess record (>= 1)");
 
   psg_config c;
   memset(&c, 0, sizeof c);
-  c.flavour = is2d ? PSG_NS2D : PSG_NS3D;
+  c.flavour = is2d ? PSG_NS2D : (bous ? PSG_BOUS3D : PSG_NS3D);
   c.precision = PSG_FP64;
   c.batch = 1;
   c.nx = nx; c.ny = ny; c.nz = is2d ? 1 : nz; c.alpha = alpha;
@@ -377,6 +423,13 @@ int main(int argc, char **argv) {
   c.dt = delt;
   if (is2d) {
     c.force = PSG_FORCE_KOLMO; c.kf = 4.0; c.ampfor = -2.0;   /* NZKX -= 2 at (0,+-4), kernels.h:264-266 */
+  } else if (bous) {
+    /* params_update (3D_Boussinesq/src/src_gmres/parameters.F90:30-47): Theta in degrees */
+    c.force = PSG_FORCE_KOLMO; c.kf = (double)psgh_kv_int(&kv, "KF", 1);
+    c.ampfor = -0.5 * c.kf;
+    c.ri = psgh_kv_float(&kv, "Ri", 1.0);
+    c.sc = psgh_kv_float(&kv, "Sc", 1.0);
+    c.theta = psgh_kv_float(&kv, "Theta", 90.0) * 4.0 * asin(1.0) / 360.0;
   } else {
     c.force = PSG_FORCE_GMRES; c.kf = (double)psgh_kv_int(&kv, "KF", 1);
     c.ampfor = -0.25 * c.kf;                                  /* src_gmres/parameters.F90 params_update */
@@ -391,11 +444,12 @@ int main(int argc, char **argv) {
   o.fixed_period = (int)psgh_kv_int(&kx, "Fixed Period", 0);
   if (maxnewton > 0) o.max_newton = maxnewton;
   if (noshift) o.nshift = 0;
-  const int nsc = is2d ? 6 : 7;      /* scalars per output record */
-  const long ncpx = (is2d ? 1 : 3) * nkt;
+  const int nsc = is2d ? 6 : 7;      /* scalars per output record (Boussinesq: 9, report_bous) */
+  const int nf = is2d ? 1 : (bous ? 4 : 3);
+  const long ncpx = nf * nkt;
 
   snprintf(path, sizeof path, "%s/%s", dir, infile);
-  psgh_restart_file *fin = psgh_restart_open(path, family, itype, (int)iktx, (int)ikty, (int)iktz, is2d ? 1 : 3, nkt,
+  psgh_restart_file *fin = psgh_restart_open(path, family, itype, (int)iktx, (int)ikty, (int)iktz, nf, nkt,
                                              msg, sizeof msg);
   if (!fin) return die(msg);
   double *z = malloc(sizeof(double) * 2 * ncpx);
@@ -407,11 +461,16 @@ int main(int argc, char **argv) {
   snprintf(path, sizeof path, "%s/Zk.out", dir);
   FILE *fzk = fopen(path, "wb");
   if (!finfo || !fupo || !fzk) return die("cannot create output files");
-  fprintf(finfo, "Guess No : Start time : guess Re :guess Period :guess Shiftx:guess Shifty:%s guess Res  :      "
-                 "Status     : UPO No :   Re   : Period : Shift x : Shift y :%s Residual     : #NEWTON : #GMRES : "
-                 "#HOOK \n", is2d ? "" : "guess Shiftz:", is2d ? "" : " Shift z :");
+  if (bous)   /* format 1501, 3D_Boussinesq/src/src_gmres/main.F90:169-172 */
+    fprintf(finfo, "Guess No : Start time : guess Re :guess Period :guess Shiftx: guess Shifty: guess Shiftz: guess "
+                   "Res  :      Status     : UPO No :   Re   : Period : Shift x : Shift y : Shift z : Residual     : "
+                   "#NEWTON : #GMRES : #HOOK \n");
+  else
+    fprintf(finfo, "Guess No : Start time : guess Re :guess Period :guess Shiftx:guess Shifty:%s guess Res  :      "
+                   "Status     : UPO No :   Re   : Period : Shift x : Shift y :%s Residual     : #NEWTON : #GMRES : "
+                   "#HOOK \n", is2d ? "" : "guess Shiftz:", is2d ? "" : " Shift z :");
   psgh_zk_header(fupo, 1, (int)iktx, (int)ikty, is2d ? 0 : (int)iktz);
-  psgh_zk_header(fzk, 1, (int)iktx, (int)ikty, is2d ? 0 : (int)iktz);
+  if (!bous) psgh_zk_header(fzk, 1, (int)iktx, (int)ikty, is2d ? 0 : (int)iktz);
   outputs out;
   memset(&out, 0, sizeof out);
   out.finfo = finfo; out.fupo = fupo; out.fzk = fzk; out.is2d = is2d; out.nsc = nsc; out.ncpx = ncpx;
@@ -445,11 +504,15 @@ int main(int argc, char **argv) {
       rc = 1;
       break;
     }
-    /* the guess's Re: the record's v2 where its type carries one (v2in = v2, main.F90:688-695) */
+    /* the guess's Re: the record's v2 where its type carries one (v2in = v2, main.F90:688-695);
+     * Boussinesq records may also carry Ri and alpha (types F-H) */
     const double v2in = rr.has_v2 ? rr.v2 : 1.0 / Re;
-    if (!list && (!ctx || c.v2 != v2in)) {
+    const double riin = (bous && rr.has_ri) ? rr.ri : c.ri, alin = (bous && rr.has_alpha) ? rr.alpha : c.alpha;
+    if (!list && (!ctx || c.v2 != v2in || c.ri != riin || c.alpha != alin)) {
       if (ctx) psg_destroy(ctx);
       c.v2 = v2in;
+      c.ri = riin;
+      c.alpha = alin;
       if (psg_create(&c, &ctx)) return die(psg_last_error());
     }
     const double tin = rr.time, Tin = rr.period, sxin = rr.shiftx, syin = rr.shifty, szin = rr.shiftz;
@@ -478,17 +541,22 @@ int main(int argc, char **argv) {
       if (trace_branch(ctx, &out, &o, &kx, Lmask, ig, iguessend, z, T, sx, v2in)) rc = 1;
       break;
     }
-    const double in_sc[7] = {tin, v2in, Tin, sxin, o.sy, is2d ? nerin : szin, nerin};
-    psgh_zk_record(fzk, in_sc, nsc, z, ncpx);
+    /* the parameter the Boussinesq solver prints as "Re" (ParFLAG, main.F90:262-264) */
+    const double par = parflag == 2 ? c.ri : (parflag == 3 ? c.alpha : c.v2);
+    if (!bous) {
+      const double in_sc[7] = {tin, v2in, Tin, sxin, o.sy, is2d ? nerin : szin, nerin};
+      psgh_zk_record(fzk, in_sc, nsc, z, ncpx);
+    }
     psg_newton_result r;
     if (psg_newton_solve(ctx, z, &T, &sx, &sz, &o, &r)) {
       fprintf(stderr, "psg_gmres: %s\n", psg_last_error());
       rc = 1;
       break;
     }
-    out.guess_v2 = v2in; out.guess_T = Tin; out.guess_sx = sxin; out.guess_sy = syin; out.guess_sz = szin;
-    out.guess_t = tin; out.guess_ner = nerin;
-    report(&out, ig, &r, z, c.v2, T, sx, o.sy, sz);
+    out.guess_v2 = bous ? par : v2in; out.guess_T = Tin; out.guess_sx = sxin; out.guess_sy = syin;
+    out.guess_sz = szin; out.guess_t = tin; out.guess_ner = nerin;
+    if (bous) report_bous(&out, ig, &r, z, c.v2, c.ri, c.alpha, par, T, sx, o.sy, sz);
+    else report(&out, ig, &r, z, c.v2, T, sx, o.sy, sz);
   }
   nupo = out.nupo;
   free(Lmask);

@@ -79,7 +79,7 @@ def test_driver_built_and_usage():
     assert os.path.exists(EXE)
     r = subprocess.run([EXE], capture_output=True, text=True)
     assert r.returncode == 1 and "usage" in r.stderr
-    r = subprocess.run([EXE, "bous", "32"], capture_output=True, text=True)
+    r = subprocess.run([EXE, "euler", "32"], capture_output=True, text=True)
     assert r.returncode == 1 and "flavour" in r.stderr
 
 
@@ -97,6 +97,51 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
     os.remove(os.path.join(d, "paramsEXTRA.txt"))
     r = run(["ns2d", "32"], d)
     assert r.returncode == 1
+
+
+# order-fixed labels of 3D_Boussinesq/src/src_gmres/parameters.F90:140-160, 179-195
+MAIN_BOUS_GMRES = ["ISEED", "IREST", "infile", "infile type", "KF", "Re", "Ri", "Theta", "Sc", "TSTEP",
+                   "ResidualThreshold", "outfreq1", "outfreq2", "DELT", "AMPV", "OUTPLANE", "STATSFLAG", "RCFLAG",
+                   "UPOflag", "ADAPTflag", "ISTOP"]
+EXTRA_BOUS = ["IGUESSEND", "maxPERIOD", "gmres ResidualThreshold", "arclcFLAG", "1stOrder", "2ndOrder PredOnly",
+              "Full 2ndOrder", "drN_reset", "Fixed Period", "Fixed Period Size", "delta_v2", "ArnoldiFLAG", "ParFLAG"]
+
+
+def setup_bous(d, n, z, sc, **kw):
+    """params.txt, paramsEXTRA.txt and a type-D guess record (header jw, IKTX, IKTY, IKTZ; time, v2, T,
+    sx, sy, sz, NER, then xi, eta, zeta, rho; 3D_Boussinesq/src/src_gmres/init.F90:75-93)"""
+    g = P.hgrid("bous", n, n, n)
+    v = {"ISEED": 1, "IREST": 1, "infile": "guess.out", "infile type": "D", "KF": 4, "Re": "50.0", "Ri": "1.0",
+         "Theta": "90.0", "Sc": "1.0", "TSTEP": "1.0", "ResidualThreshold": "1.0", "outfreq1": "1.0",
+         "outfreq2": "1.0", "DELT": "0.01", "AMPV": "1.0", "OUTPLANE": 0, "STATSFLAG": 0, "RCFLAG": 0,
+         "UPOflag": 1, "ADAPTflag": 0, "ISTOP": 0}
+    e = {"IGUESSEND": 1, "maxPERIOD": "100.0", "gmres ResidualThreshold": "1.0", "arclcFLAG": "F", "1stOrder": "F",
+         "2ndOrder PredOnly": "F", "Full 2ndOrder": "F", "drN_reset": "F", "Fixed Period": "F",
+         "Fixed Period Size": "0.2", "delta_v2": "0.0", "ArnoldiFLAG": "F", "ParFLAG": 1}
+    for k, x in kw.items():
+        (e if k in e else v)[k] = x
+    write_params(os.path.join(d, "params.txt"), MAIN_BOUS_GMRES, v)
+    write_params(os.path.join(d, "paramsEXTRA.txt"), EXTRA_BOUS, e)
+    with open(os.path.join(d, "guess.out"), "wb") as f:
+        f.write(rec(struct.pack("<4i", 1, g.iktx, g.ikty, g.iktz)))
+        f.write(rec(struct.pack("<7d", *sc) + np.ascontiguousarray(z, np.complex128).tobytes()))
+    return g
+
+
+def test_driver_bous_reads_type_d_guess_and_rejects_what_it_lacks(tmp_path):
+    d = str(tmp_path)
+    g = P.hgrid("bous", 16, 16, 16)
+    # a guess period 1e-12 off a whole number of steps is snapped to it (main.F90:694-697)
+    setup_bous(d, 16, np.zeros((4, g.nkt), complex), (1.5, 0.02, 0.3 + 1e-12, 0.25, 0.5, -0.125, 0.75))
+    r = run(["bous", "16", "--list"], d)
+    assert r.returncode == 0, r.stderr
+    row = listed(r.stdout)[0]
+    assert row == {"t": 1.5, "v2": 0.02, "T": 30 * 0.01, "sx": 0.25, "sy": 0.5, "sz": -0.125, "NER": 0.75}
+    r = run(["bous", "16", "--jv", "tl"], d)
+    assert r.returncode == 1 and "finite-difference" in r.stderr
+    setup_bous(d, 16, np.zeros((4, g.nkt), complex), (0, 0.02, 0.3, 0, 0, 0, 1), arclcFLAG="T")
+    r = run(["bous", "16"], d)
+    assert r.returncode == 1 and "not built for the Boussinesq" in r.stderr
 
 
 def listed(out):
@@ -195,6 +240,44 @@ def test_driver_converges_laminar_state_and_writes_outputs(gpu, tmp_path, jv):
     assert np.abs((z - lam)[L == 1]).max() < 1e-9
     zk = read_recs(os.path.join(d, "Zk.out"))
     assert len(zk) == 3 and np.array_equal(np.frombuffer(zk[1][48:], np.complex128), z0)
+
+
+@pytest.mark.gpu
+def test_driver_bous_two_shift_newton_outputs(gpu, tmp_path):
+    # two Newton steps from a developed 16^3 state: the reference's UPOinfo.dat line
+    # (format 1502), one Zk.out record of the last iterate (no header), and its
+    # residual re-integrated on the oracle with both shifts applied on the host
+    import oracle_lib as O
+    from test_gpu_newton_bous import flow_host, ner_host
+    d, n, dt0 = str(tmp_path), 16, 0.01
+    g = O.grid3(n, n, n)
+    z0 = O.BOUS(n, v2=0.02, ri=1.0, theta=np.pi / 2, dt=0.01).run(O.ic_solenoidal(g, 73, 0.5, 0.2), 30)
+    setup_bous(d, n, z0, (0.0, 0.02, 0.3, 0.05, 0.0, -0.05, 0.3))
+    r = run(["bous", str(n), "--maxnewton", "2"], d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    info = open(os.path.join(d, "UPOinfo.dat")).read().splitlines()
+    assert info[0].startswith("Guess No : Start time : guess Re :guess Period :guess Shiftx: guess Shifty: guess "
+                              "Shiftz:")
+    assert len(info) == 2
+    line = info[1]
+    assert line[:9] == "    1    " and line[9:25] == "    0.000000E+00" and line[25:41] == "    0.200000E-01"
+    assert line[41:89] == "%12.5f%12.5f%12.5f%12.5f" % (0.3, 0.05, 0.0, -0.05)
+    st = line[103:line.index(":", 106) + 2]
+    assert st in (' :   "CONVERGED"   : ', ' : "GMRES FAILED"  : ', ' : "H-STEP FAILED" : ',
+                  ' : "T-STEP FAILED" : ', ' : "NEWTON STALLED" : ', ' : "PERIOD TOO SMALL" : ',
+                  ' : "NEWTON FAILED" : '), repr(st)
+    zk = read_recs(os.path.join(d, "Zk.out"))
+    assert len(zk) == 1 and len(zk[0]) == 9 * 8 + 4 * 16 * g.nkt
+    sc = struct.unpack("<9d", zk[0][:72])
+    assert sc[:4] == (0.0, 0.02, 1.0, 1.0) and sc[6] == 0.0
+    z = np.frombuffer(zk[0][72:], np.complex128).reshape(4, g.nkt)
+    T, sx, sz, ner = sc[4], sc[5], sc[7], sc[8]
+    zT = flow_host(n, dict(v2=0.02, ri=1.0, theta=np.pi / 2), z, T, dt0)
+    host = ner_host(g, z, zT, sx, sz)
+    assert abs(host - ner) <= 1e-8 * ner, (host, ner)
+    assert ner < 0.3
+    upos = read_recs(os.path.join(d, "UPOs.out"))
+    assert struct.unpack("<4i", upos[0]) == (1, g.iktx, g.ikty, g.iktz)
 
 
 @pytest.mark.gpu
