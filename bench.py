@@ -434,6 +434,8 @@ def main():
     ap.add_argument("--newton-guesses", type=int, default=8)
     ap.add_argument("--newton-batch-iters", type=int, default=10, help="Newton iterations per batched guess")
     ap.add_argument("--newton-batch-tmax", type=float, default=8.0, help="longest guess period of the batch")
+    ap.add_argument("--newton-batch-no-seq", action="store_true",
+                    help="skip the one-guess-at-a-time comparison of the batched case")
     ap.add_argument("--save-orbit", default=None, metavar="NPZ",
                     help="write the first orbit the finite-difference Newton converges (active modes, T, shift)")
     a = ap.parse_args()
@@ -502,7 +504,8 @@ def main():
     if a.newton_batch:
         log("bench: newton 2D %d^2, %d guesses on %d lanes" % (a.newton_n, a.newton_guesses, a.newton_batch))
         nbatch = newton_batch_case(P, n=a.newton_n, nguess=a.newton_guesses, lanes=a.newton_batch,
-                                   max_newton=a.newton_batch_iters, tmax=a.newton_batch_tmax)
+                                   max_newton=a.newton_batch_iters, tmax=a.newton_batch_tmax,
+                                   sequential=not a.newton_batch_no_seq)
     cpu = None
     if not a.no_cpu:
         log("bench: cpu baseline (%d steps)" % a.cpu_steps)
