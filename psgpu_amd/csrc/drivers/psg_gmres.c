@@ -37,7 +37,9 @@
  * and in Zk.out, which holds the last iterate only (main.F90:1739-1744,
  * 1834-1838); UPOinfo.dat lines in format 1502 (main.F90:1876-1877), whose
  * "Re" columns carry the ParFLAG parameter (v1, Ri or alpha, main.F90:262-264).
- * Continuation and Floquet analysis are not built for this flavour.
+ * Its ArnoldiFLAG analysis takes the 3D settings and eigenvalues.dat layout
+ * (main.F90:69-70,430,968-983, the same as 3D Navier-Stokes) with
+ * finite-difference Jv.  Continuation is not built for this flavour.
  *
  * usage: psg_gmres ns2d|ns3d|bous NX [NY NZ ALPHA] [--dir D] [--jv fd|tl|tl32] [--maxnewton K] [--noshift]
  * (--jv tl: tangent-linear Jv; tl32: the same in FP32-bulk passes under the
@@ -408,7 +410,7 @@ int main(int argc, char **argv) {
   const int arclc = (int)psgh_kv_int(&kx, "arclcFLAG", 0);
   const int arnoldi = (int)psgh_kv_int(&kx, "ArnoldiFLAG", 0);
   if (arclc && arnoldi) return die("arclcFLAG and ArnoldiFLAG are exclusive");
-  if (bous && (arclc || arnoldi)) return die("continuation and Floquet analysis are not built for the Boussinesq flavour");
+  if (bous && arclc) return die("continuation is not built for the Boussinesq flavour");
   if (bous && jv) return die("the Boussinesq flavour takes finite-difference Jv only");
   const int parflag = bous ? (int)psgh_kv_int(&kx, "ParFLAG", 1) : 1;
   if (irest < 1) return die("IREST must name the first guess record (>= 1)");

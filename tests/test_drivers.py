@@ -143,6 +143,27 @@ def test_driver_bous_reads_type_d_guess_and_rejects_what_it_lacks(tmp_path):
     r = run(["bous", "16"], d)
     assert r.returncode == 1 and "not built for the Boussinesq" in r.stderr
 
+@pytest.mark.gpu
+def test_driver_bous_arnoldi_flag_writes_floquet_rows(gpu, tmp_path):
+    # ArnoldiFLAG on a Boussinesq guess: the 3D eigenvalues.dat layout (I3,1x,2F13.5 rows of
+    # the unstable exponents, a blank row, then IREST, their sum and T; main.F90:968-983)
+    import oracle_lib as O
+    d, n = str(tmp_path), 16
+    g = O.grid3(n, n, n)
+    z0 = O.BOUS(n, v2=0.02, ri=1.0, theta=np.pi / 2, dt=0.01).run(O.ic_solenoidal(g, 75, 0.5, 0.2), 30)
+    setup_bous(d, n, z0, (0.0, 0.02, 0.2, 0.0, 0.0, 0.0, 0.1), ArnoldiFLAG="T")
+    r = run(["bous", str(n)], d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = open(os.path.join(d, "eigenvalues.dat")).read().split("\n")
+    blank = rows.index("     ")
+    for row in rows[:blank]:
+        i, lr, li = row.split()
+        assert float(lr) > 0.0 and len(row) == 30
+    last = rows[blank + 1].split()
+    assert int(last[0]) == 1 and float(last[2]) == pytest.approx(0.2)
+    assert float(last[1]) == pytest.approx(sum(float(x.split()[1]) for x in rows[:blank]), abs=1e-4)
+
+
 
 def listed(out):
     rows = []
