@@ -57,6 +57,13 @@ def test_tangent_about_recorded_base(gpu, policy):
             assert np.abs(z1 - zc).max() <= 1e-13 * np.abs(zc).max()
             err32 = np.abs(d32[i] - dc).max() / np.abs(dc).max()
             assert 0 < err32 < 1e-5, err32
+    # an FP32-bulk context records in FP32 and replays in FP32 passes
+    with P.Stepper(flavour="ns2d", n=n, v2=v2, dt=dt0, kf=4.0, ampfor=-2.0, precision="fp32") as s32:
+        _, db = s32.tangent_many(z[None], ds[:, None, :], T, dt0, policy)
+        for i in range(3):
+            _, dc = s32.tangent(z[None], ds[i][None], T, dt0, policy)
+            err = np.abs(db[i] - dc).max() / np.abs(dc).max()
+            assert err < 1e-5, err
     nst = int(T / dt0)
     eps = 1e-6
     if policy == 0:
